@@ -1,0 +1,861 @@
+// Fast-spectral VHS collision operator Q(f, f) for gfx950.
+//
+// Algorithm (identical in exact arithmetic to the reference dataflow of
+// frfs/solvers/dgfs/scattering.py:227-286 / vhs-gll.mako:103-266):
+//
+//   F      = FFT(f) / n                                   (prep kernel)
+//   for every radial node p and direction q:
+//     E(k)  = exp(i a_pq(k)),  a_pq = (pi/L)(qz_p/2)(l . sigma_q), separable
+//             as ex(kx) ey(ky) ez(kz)
+//     Y     = IFFT(F (E + i conj E))  = P+ + i P-, one complex transform for
+//             the two real-part fields P+- = IFFT(exp(+-i a) F)
+//     W     = IFFT(B), B the anti-Hermitian part of the two spectra, which
+//             is supported on the three Nyquist planes only (the l = -N/2
+//             row has no +N/2 partner, scattering.py:79); W carries the exact
+//             imaginary parts I+- of P+- and costs three 2D transforms
+//     S_p  += Re(P+ P-) = (Re Y + Im W)(Im Y - Re W) - Re W Im W
+//   G^     = sum_p (2 qw_p sw / n) b1_p FFT(S_p)           (gain kernel)
+//   x      = IFFT(G^ + i b2 F),  Q = prefac (Re x - f Im x) (finalize kernel)
+//
+// The reference computes t3 = IFFT(cos a F), t4 = IFFT(sin a F) and
+// S = t3^2 + t4^2 (complex squares, vhs-gll.mako:160-164); t3^2 + t4^2 =
+// (t3 + i t4)(t3 - i t4) = P+ P-.  Only Re S reaches Re G because b1 and the
+// FFT of a real field are Hermitian, so the imaginary part of S is dropped.
+// The sum over q is taken before the forward FFT (linearity), which turns the
+// reference's 3K+3 transforms per point into K + Nrho + 3.
+//
+// Data movement: one workgroup owns a (point, radial group) work item and
+// all of its 3D transforms.  A 3D transform is a slab pass (each kx-plane is
+// 2D-transformed in LDS) plus a pencil pass (x-pencils in registers, one per
+// thread, coalesced over z); the intermediate lives in a per-workgroup scratch
+// region that the persistent grid reuses item after item, so it stays in the
+// L2 / Infinity Cache instead of streaming to HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "cplx.hpp"
+
+namespace dgfs {
+
+std::string& last_error_ref() {
+  static thread_local std::string s;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// workgroup building blocks
+// ---------------------------------------------------------------------------
+
+// 2D FFTs of `cnt` N x N planes staged through LDS (row stride N+1 to spread
+// banks).  load(j, r, c) gives element (r, c) of plane j; the plane is
+// transformed along c then along r; store(j, r, c, v) receives the result.
+template <typename T, int N, int NT, int SIGN, class Load, class Store>
+__device__ __forceinline__ void slab_batch(cpx<T>* lds, int cnt, Load load, Store store) {
+  constexpr int NP = N + 1, PL = N * NP, NN = N * N;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < cnt * NN; e += NT) {
+    const int j = e / NN, rem = e - j * NN, r = rem / N, c = rem - r * N;
+    lds[j * PL + r * NP + c] = load(j, r, c);
+  }
+  __syncthreads();
+  if (tid < cnt * N) {
+    const int j = tid / N, r = tid - j * N;
+    cpx<T>* row = lds + j * PL + r * NP;
+    cpx<T> v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = row[k];
+    FFT<T, N, SIGN>::run(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) row[k] = v[k];
+  }
+  __syncthreads();
+  if (tid < cnt * N) {
+    const int j = tid / N, c = tid - j * N;
+    cpx<T>* col = lds + j * PL + c;
+    cpx<T> v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = col[k * NP];
+    FFT<T, N, SIGN>::run(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) col[k * NP] = v[k];
+  }
+  __syncthreads();
+  for (int e = tid; e < cnt * NN; e += NT) {
+    const int j = e / NN, rem = e - j * NN, r = rem / N, c = rem - r * N;
+    store(j, r, c, lds[j * PL + r * NP + c]);
+  }
+  __syncthreads();
+}
+
+// 1D FFTs along x of the N*N pencils (y, z); pencil idx = y*N + z.
+template <typename T, int N, int NT, int SIGN, class Load, class Epi>
+__device__ __forceinline__ void pencil_pass(Load load, Epi epi) {
+  for (int idx = threadIdx.x; idx < N * N; idx += NT) {
+    cpx<T> v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = load(k, idx);
+    FFT<T, N, SIGN>::run(v);
+    epi(idx, v);
+  }
+  __syncthreads();
+}
+
+template <int N>
+struct Shape {
+  static constexpr int NT = (N >= 16) ? 128 : 64;
+  static constexpr int PB = NT / N;  // planes per slab batch
+  // the Nyquist pass transforms 3 planes in one batch and the phase tables
+  // are filled by 3N threads
+  static_assert(PB >= 3, "slab batch must hold the three Nyquist planes");
+  static_assert(3 * N <= NT, "phase tables need 3N threads");
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PrepArgs {
+  const double* f;  // [npts][n] contiguous
+  cpx<T>* F;        // [npts][n] natural spectral order, scaled by 1/n
+  cpx<T>* Z;        // scratch [nslots][n]
+  int npts;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_prep(PrepArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
+  const T rn = static_cast<T>(1.0 / static_cast<double>(n));
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    const double* f = a.f + static_cast<size_t>(pt) * n;
+    cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    pencil_pass<T, N, NT, -1>(
+        [&](int x, int idx) { return cpx<T>{static_cast<T>(f[x * NN + idx]), T(0)}; },
+        [&](int idx, cpx<T>* v) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) Z[k * NN + idx] = v[k];
+        });
+    for (int kx0 = 0; kx0 < N; kx0 += PB) {
+      const int cnt = min(PB, N - kx0);
+      slab_batch<T, N, NT, -1>(
+          lds, cnt, [&](int j, int r, int c) { return Z[(kx0 + j) * NN + r * N + c]; },
+          [&](int j, int r, int c, cpx<T> v) { F[(kx0 + j) * NN + r * N + c] = rn * v; });
+    }
+  }
+}
+
+template <typename T>
+struct GainArgs {
+  const cpx<T>* F;      // [npts][n]
+  cpx<T>* Ghat;         // [npts*ngroups][n]
+  cpx<T>* Z;            // scratch [nslots][n]
+  T* S;                 // scratch [nslots][n]
+  cpx<T>* W;            // scratch [nslots][3][N][N]
+  const T* b1;          // [Nrho][n]
+  const double* alpha;  // [Nrho] (pi/L) qz_p / 2
+  const double* wgt;    // [Nrho] 2 qw_p sw / n
+  const double* sz;     // [M][3]
+  int nitems, ngroups, ppg, Nrho, M;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_gain(GainArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  __shared__ cpx<T> eph[3][N];
+  const int tid = threadIdx.x;
+  cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
+  T* S = a.S + static_cast<size_t>(blockIdx.x) * n;
+  cpx<T>* W = a.W + static_cast<size_t>(blockIdx.x) * 3 * NN;
+
+  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const int pt = item / a.ngroups, g = item - pt * a.ngroups;
+    const int p0 = g * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
+    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    cpx<T>* G = a.Ghat + static_cast<size_t>(item) * n;
+    for (int i = tid; i < n; i += NT) G[i] = cpx<T>{T(0), T(0)};
+    for (int p = p0; p < p1; ++p) {
+      for (int i = tid; i < n; i += NT) S[i] = T(0);
+      const double alpha = a.alpha[p];
+      for (int q = 0; q < a.M; ++q) {
+        __syncthreads();
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double l = (k < h) ? static_cast<double>(k) : static_cast<double>(k - N);
+          double s, c;
+          sincos(alpha * a.sz[q * 3 + d] * l, &s, &c);
+          eph[d][k] = cpx<T>{static_cast<T>(c), static_cast<T>(s)};
+        }
+        __syncthreads();
+
+        // Nyquist planes: B = (F/2)(conj D - i D), D = E - E~, where E~ takes
+        // the conjugate factor on every axis whose index is the Nyquist row.
+        slab_batch<T, N, NT, +1>(
+            lds, 3,
+            [&](int j, int r, int c) -> cpx<T> {
+              int kx, ky, kz;
+              if (j == 0) {
+                kx = h; ky = r; kz = c;
+              } else if (j == 1) {
+                kx = r; ky = h; kz = c;
+                if (kx == h) return cpx<T>{T(0), T(0)};
+              } else {
+                kx = r; ky = c; kz = h;
+                if (kx == h || ky == h) return cpx<T>{T(0), T(0)};
+              }
+              const cpx<T> ex = eph[0][kx], ey = eph[1][ky], ez = eph[2][kz];
+              const cpx<T> E = ex * ey * ez;
+              const cpx<T> Et = (kx == h ? conj(ex) : ex) * (ky == h ? conj(ey) : ey) *
+                                (kz == h ? conj(ez) : ez);
+              const cpx<T> D = E - Et;
+              const cpx<T> t = conj(D) - mul_i(D);
+              return T(0.5) * (F[kx * NN + ky * N + kz] * t);
+            },
+            [&](int j, int r, int c, cpx<T> v) { W[j * NN + r * N + c] = v; });
+
+        // main transform, yz-planes then x-pencils
+        for (int kx0 = 0; kx0 < N; kx0 += PB) {
+          const int cnt = min(PB, N - kx0);
+          slab_batch<T, N, NT, +1>(
+              lds, cnt,
+              [&](int j, int ky, int kz) -> cpx<T> {
+                const int kx = kx0 + j;
+                const cpx<T> E = eph[0][kx] * eph[1][ky] * eph[2][kz];
+                return F[kx * NN + ky * N + kz] * (E + mul_i(conj(E)));
+              },
+              [&](int j, int y, int z, cpx<T> v) { Z[(kx0 + j) * NN + y * N + z] = v; });
+        }
+        pencil_pass<T, N, NT, +1>(
+            [&](int kx, int idx) { return Z[kx * NN + idx]; },
+            [&](int idx, cpx<T>* v) {
+              const int y = idx / N, z = idx - y * N;
+              const cpx<T> w1 = W[idx];
+              const T sy = (y & 1) ? T(-1) : T(1), szs = (z & 1) ? T(-1) : T(1);
+#pragma unroll
+              for (int x = 0; x < N; ++x) {
+                const T sx = (x & 1) ? T(-1) : T(1);
+                const cpx<T> w2 = W[NN + x * N + z], w3 = W[2 * NN + x * N + y];
+                const T wr = sx * w1.x + sy * w2.x + szs * w3.x;
+                const T wi = sx * w1.y + sy * w2.y + szs * w3.y;
+                S[x * NN + idx] += (v[x].x + wi) * (v[x].y - wr) - wr * wi;
+              }
+            });
+      }
+      // forward FFT of S_p, weighted by 2 qw_p sw / n * b1_p, into G^
+      pencil_pass<T, N, NT, -1>(
+          [&](int x, int idx) { return cpx<T>{S[x * NN + idx], T(0)}; },
+          [&](int idx, cpx<T>* v) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) Z[k * NN + idx] = v[k];
+          });
+      const T wp = static_cast<T>(a.wgt[p]);
+      const T* b1p = a.b1 + static_cast<size_t>(p) * n;
+      for (int kx0 = 0; kx0 < N; kx0 += PB) {
+        const int cnt = min(PB, N - kx0);
+        slab_batch<T, N, NT, -1>(
+            lds, cnt, [&](int j, int r, int c) { return Z[(kx0 + j) * NN + r * N + c]; },
+            [&](int j, int r, int c, cpx<T> v) {
+              const int o = (kx0 + j) * NN + r * N + c;
+              G[o] = G[o] + (wp * b1p[o]) * v;
+            });
+      }
+    }
+  }
+}
+
+template <typename T>
+struct FinArgs {
+  const cpx<T>* Ghat;  // [npts*ngroups][n]
+  const cpx<T>* F;     // [npts][n]
+  const T* b2;         // [n]
+  const double* f;     // [npts][n]
+  double* Q;           // [npts][n]
+  cpx<T>* Z;           // scratch
+  double prefac;
+  int npts, ngroups;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_finalize(FinArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    const cpx<T>* G = a.Ghat + static_cast<size_t>(pt) * a.ngroups * n;
+    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    for (int kx0 = 0; kx0 < N; kx0 += PB) {
+      const int cnt = min(PB, N - kx0);
+      slab_batch<T, N, NT, +1>(
+          lds, cnt,
+          [&](int j, int r, int c) -> cpx<T> {
+            const int o = (kx0 + j) * NN + r * N + c;
+            cpx<T> s = G[o];
+            for (int g = 1; g < a.ngroups; ++g) s = s + G[static_cast<size_t>(g) * n + o];
+            const cpx<T> lf = a.b2[o] * F[o];
+            return s + mul_i(lf);
+          },
+          [&](int j, int r, int c, cpx<T> v) { Z[(kx0 + j) * NN + r * N + c] = v; });
+    }
+    const double* f = a.f + static_cast<size_t>(pt) * n;
+    double* Q = a.Q + static_cast<size_t>(pt) * n;
+    pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z[kx * NN + idx]; },
+                              [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                for (int x = 0; x < N; ++x) {
+                                  const int o = x * NN + idx;
+                                  Q[o] = a.prefac * (static_cast<double>(v[x].x) -
+                                                     f[o] * static_cast<double>(v[x].y));
+                                }
+                              });
+  }
+}
+
+// FP64 projection of Q onto the 5 collision invariants {1, c, |c|^2}: the
+// minimum-norm correction d = -C^T (C C^T)^{-1} C Q with C_jr = phi_j(r) cw.
+struct ProjArgs {
+  double* Q;  // [npts][n]
+  double Ainv[25];
+  double L, cw;
+  int N, npts;
+};
+
+__global__ __launch_bounds__(256) void k_vhs_project(ProjArgs a) {
+  __shared__ double red[5][256];
+  __shared__ double lam[5];
+  const int n = a.N * a.N * a.N, tid = threadIdx.x;
+  const double h = 2.0 * a.L / a.N, c0 = -a.L + 0.5 * h;
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    double* Q = a.Q + static_cast<size_t>(pt) * n;
+    double m[5] = {0, 0, 0, 0, 0};
+    for (int r = tid; r < n; r += 256) {
+      const int I = r / (a.N * a.N), J = (r / a.N) % a.N, K = r % a.N;
+      const double cx = c0 + I * h, cy = c0 + J * h, cz = c0 + K * h, q = Q[r];
+      m[0] += q;
+      m[1] += q * cx;
+      m[2] += q * cy;
+      m[3] += q * cz;
+      m[4] += q * (cx * cx + cy * cy + cz * cz);
+    }
+    for (int j = 0; j < 5; ++j) red[j][tid] = m[j] * a.cw;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s)
+        for (int j = 0; j < 5; ++j) red[j][tid] += red[j][tid + s];
+      __syncthreads();
+    }
+    if (tid < 5) {
+      double acc = 0;
+      for (int k = 0; k < 5; ++k) acc += a.Ainv[tid * 5 + k] * red[k][0];
+      lam[tid] = acc;
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += 256) {
+      const int I = r / (a.N * a.N), J = (r / a.N) % a.N, K = r % a.N;
+      const double cx = c0 + I * h, cy = c0 + J * h, cz = c0 + K * h;
+      Q[r] -= a.cw * (lam[0] + lam[1] * cx + lam[2] * cy + lam[3] * cz +
+                      lam[4] * (cx * cx + cy * cy + cz * cz));
+    }
+    __syncthreads();
+  }
+}
+
+// AoSoA <-> contiguous [pt][r], pt = (elem - e0) * nupts + upt.  One block
+// moves a 32 (r) x 32 (elements of one AoSoA block) tile through LDS so both
+// sides are coalesced.
+__global__ __launch_bounds__(256) void k_aosoa_gather(const double* __restrict__ in,
+                                                      double* __restrict__ out, int nupts, int ldim,
+                                                      int vsize, int e0, int ne) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.x * 32, eb = blockIdx.y, upt = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int elem_t = eb * 32 + tx;
+  for (int rr = ty; rr < 32; rr += 8) {
+    const int r = r0 + rr;
+    if (r < vsize && elem_t >= e0 && elem_t < e0 + ne)
+      tile[rr][tx] = in[static_cast<size_t>(upt) * ldim + (static_cast<size_t>(eb) * vsize + r) * 32 + tx];
+  }
+  __syncthreads();
+  for (int ee = ty; ee < 32; ee += 8) {
+    const int elem = eb * 32 + ee, r = r0 + tx;
+    if (r < vsize && elem >= e0 && elem < e0 + ne)
+      out[(static_cast<size_t>(elem - e0) * nupts + upt) * vsize + r] = tile[tx][ee];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_aosoa_scatter(const double* __restrict__ in,
+                                                       double* __restrict__ out, int nupts, int ldim,
+                                                       int vsize, int e0, int ne) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.x * 32, eb = blockIdx.y, upt = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int ee = ty; ee < 32; ee += 8) {
+    const int elem = eb * 32 + ee, r = r0 + tx;
+    if (r < vsize && elem >= e0 && elem < e0 + ne)
+      tile[tx][ee] = in[(static_cast<size_t>(elem - e0) * nupts + upt) * vsize + r];
+  }
+  __syncthreads();
+  const int elem_t = eb * 32 + tx;
+  for (int rr = ty; rr < 32; rr += 8) {
+    const int r = r0 + rr;
+    if (r < vsize && elem_t >= e0 && elem_t < e0 + ne)
+      out[static_cast<size_t>(upt) * ldim + (static_cast<size_t>(eb) * vsize + r) * 32 + tx] = tile[rr][tx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct VhsHandle {
+  int magic = 0x76687331;
+  int N = 0, n = 0, Nrho = 0, M = 0, prec = 64, project = 0;
+  double L = 0, gamma = 0, prefac = 0, sw = 0, cw = 0;
+  size_t tsize = 8;  // sizeof(T)
+  void* b1 = nullptr;
+  void* b2 = nullptr;
+  double* alpha = nullptr;
+  double* wgt = nullptr;
+  double* szd = nullptr;
+  double Ainv[25] = {};
+  // workspace
+  size_t cap_pts = 0, cap_items = 0;
+  int cap_slots = 0;
+  double* fc = nullptr;  // [cap_pts][n]
+  double* Qc = nullptr;  // [cap_pts][n]
+  void* F = nullptr;     // [cap_pts][n] cpx<T>
+  void* Ghat = nullptr;  // [cap_items][n] cpx<T>
+  void* Z = nullptr;
+  void* S = nullptr;
+  void* W = nullptr;
+  int groups_override = 0;
+  int last_groups = 0, last_slots = 0, threads = 0;
+  // optional per-kernel timing: event pairs recorded around every launch on
+  // the launch stream, resolved (and synchronised) only when queried
+  int timing = 0;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double acc_ms[4] = {0, 0, 0, 0};
+  int acc_n[4] = {0, 0, 0, 0};
+};
+
+static hipEvent_t take_event(VhsHandle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// record a start event for kernel class k (0 prep, 1 gain, 2 finalize, 3 project)
+struct KTimer {
+  VhsHandle* h;
+  int k;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  KTimer(VhsHandle* h_, int k_, hipStream_t st_) : h(h_), k(k_), st(st_) {
+    if (h->timing) {
+      a = take_event(h);
+      if (a) (void)hipEventRecord(a, st);
+    }
+  }
+  void stop() {
+    if (!a) return;
+    hipEvent_t b = take_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    h->pending.push_back({k, {a, b}});
+    a = nullptr;
+  }
+};
+
+static void resolve_timing(VhsHandle* h) {
+  for (auto& pr : h->pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(pr.second.second);
+    if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
+      h->acc_ms[pr.first] += ms;
+      h->acc_n[pr.first] += 1;
+    }
+    h->pool.push_back(pr.second.first);
+    h->pool.push_back(pr.second.second);
+  }
+  h->pending.clear();
+}
+
+static void free_ws(VhsHandle* h) {
+  (void)hipFree(h->fc); (void)hipFree(h->Qc); (void)hipFree(h->F); (void)hipFree(h->Ghat);
+  (void)hipFree(h->Z); (void)hipFree(h->S); (void)hipFree(h->W);
+  h->fc = h->Qc = nullptr; h->F = h->Ghat = h->Z = h->S = h->W = nullptr;
+  h->cap_pts = h->cap_items = 0; h->cap_slots = 0;
+}
+
+static int invert5(const double* A, double* Ainv) {
+  double m[5][10];
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 10; ++j) m[i][j] = (j < 5) ? A[i * 5 + j] : (j - 5 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 5; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 5; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (std::fabs(m[piv][c]) < 1e-300) return 1;
+    for (int j = 0; j < 10; ++j) std::swap(m[c][j], m[piv][j]);
+    const double d = m[c][c];
+    for (int j = 0; j < 10; ++j) m[c][j] /= d;
+    for (int r = 0; r < 5; ++r)
+      if (r != c) {
+        const double fct = m[r][c];
+        for (int j = 0; j < 10; ++j) m[r][j] -= fct * m[c][j];
+      }
+  }
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) Ainv[i * 5 + j] = m[i][j + 5];
+  return 0;
+}
+
+static bool supported_N(int N) {
+  switch (N) {
+    case 4: case 6: case 8: case 12: case 16: case 24: case 32: return true;
+    default: return false;
+  }
+}
+
+template <typename T, int N>
+static int occupancy_slots() {
+  int occ = 1;
+  int o1 = 1, o2 = 1, o3 = 1;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, reinterpret_cast<const void*>(k_vhs_gain<T, N>), Shape<N>::NT, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, reinterpret_cast<const void*>(k_vhs_prep<T, N>), Shape<N>::NT, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, reinterpret_cast<const void*>(k_vhs_finalize<T, N>), Shape<N>::NT, 0);
+  occ = std::max(1, std::min(o1, std::min(o2, o3)));
+  return occ * cu_count();
+}
+
+template <typename T, int N>
+static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipStream_t st) {
+  constexpr int n = N * N * N;
+  const int slots = h->cap_slots;
+  int groups = h->groups_override;
+  if (groups <= 0) {
+    groups = std::max(1, std::min(h->Nrho, (2 * slots + npts - 1) / std::max(npts, 1)));
+  }
+  const int ppg = (h->Nrho + groups - 1) / groups;
+  groups = (h->Nrho + ppg - 1) / ppg;
+  const size_t items = static_cast<size_t>(npts) * groups;
+  if (items > h->cap_items) {
+    (void)hipFree(h->Ghat);
+    h->Ghat = nullptr;
+    DGFS_HIP_TRY(hipMalloc(&h->Ghat, items * n * sizeof(cpx<T>)));
+    h->cap_items = items;
+  }
+  h->last_groups = groups;
+  h->last_slots = slots;
+  h->threads = Shape<N>::NT;
+
+  PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Z), npts};
+  KTimer t0(h, 0, st);
+  hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, pa);
+  DGFS_LAUNCH_CHECK("vhs_prep");
+  t0.stop();
+
+  GainArgs<T> ga{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Ghat),
+                 static_cast<cpx<T>*>(h->Z), static_cast<T*>(h->S), static_cast<cpx<T>*>(h->W),
+                 static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
+                 static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+  KTimer t1(h, 1, st);
+  hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
+                     dim3(Shape<N>::NT), 0, st, ga);
+  DGFS_LAUNCH_CHECK("vhs_gain");
+  t1.stop();
+
+  FinArgs<T> fa{static_cast<const cpx<T>*>(h->Ghat), static_cast<const cpx<T>*>(h->F),
+                static_cast<const T*>(h->b2), fc, Qc, static_cast<cpx<T>*>(h->Z),
+                h->prefac, npts, groups};
+  KTimer t2(h, 2, st);
+  hipLaunchKernelGGL((k_vhs_finalize<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, fa);
+  DGFS_LAUNCH_CHECK("vhs_finalize");
+  t2.stop();
+
+  if (h->project) {
+    ProjArgs pr;
+    pr.Q = Qc;
+    std::memcpy(pr.Ainv, h->Ainv, sizeof(pr.Ainv));
+    pr.L = h->L; pr.cw = h->cw; pr.N = N; pr.npts = npts;
+    KTimer t3(h, 3, st);
+    hipLaunchKernelGGL(k_vhs_project, dim3(std::min(npts, 4 * cu_count())), dim3(256), 0, st, pr);
+    DGFS_LAUNCH_CHECK("vhs_project");
+    t3.stop();
+  }
+  return DGFS_OK;
+}
+
+template <typename T, int N>
+static int ensure_ws(VhsHandle* h, int pts) {
+  constexpr int n = N * N * N;
+  if (h->cap_slots == 0) {
+    const int slots = occupancy_slots<T, N>();
+    DGFS_HIP_TRY(hipMalloc(&h->Z, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
+    DGFS_HIP_TRY(hipMalloc(&h->S, static_cast<size_t>(slots) * n * sizeof(T)));
+    DGFS_HIP_TRY(hipMalloc(&h->W, static_cast<size_t>(slots) * 3 * N * N * sizeof(cpx<T>)));
+    h->cap_slots = slots;
+  }
+  if (static_cast<size_t>(pts) > h->cap_pts) {
+    (void)hipFree(h->F); (void)hipFree(h->fc); (void)hipFree(h->Qc);
+    h->F = nullptr; h->fc = h->Qc = nullptr;
+    DGFS_HIP_TRY(hipMalloc(&h->F, static_cast<size_t>(pts) * n * sizeof(cpx<T>)));
+    DGFS_HIP_TRY(hipMalloc(&h->fc, static_cast<size_t>(pts) * n * sizeof(double)));
+    DGFS_HIP_TRY(hipMalloc(&h->Qc, static_cast<size_t>(pts) * n * sizeof(double)));
+    h->cap_pts = pts;
+  }
+  return DGFS_OK;
+}
+
+// points per launch: F + G^ (worst case one partial per radial node) + f + Q
+// kept under 48 GiB of the 288 GB of HBM; launches are balanced in size
+static int chunk_points(const VhsHandle* h, int total) {
+  const double per = static_cast<double>(h->n) * (2.0 * h->tsize * (1 + h->Nrho) + 16.0);
+  const double budget = 48.0 * 1024.0 * 1024.0 * 1024.0;
+  const int cap = std::max(1, static_cast<int>(budget / per));
+  if (total <= cap) return std::max(total, 1);
+  const int nchunks = (total + cap - 1) / cap;
+  return (total + nchunks - 1) / nchunks;
+}
+
+template <typename T, int N>
+static int eval_impl(VhsHandle* h, const double* f, double* Q, bool aosoa, int nupts, int ldim,
+                     int vsize, int e0, int ne, int npts, hipStream_t st) {
+  constexpr int n = N * N * N;
+  if (aosoa) {
+    const int total = nupts * ne;
+    if (total == 0) return DGFS_OK;
+    const int per_chunk_elems =
+        std::max(1, chunk_points(h, total) / std::max(nupts, 1));
+    for (int c0 = 0; c0 < ne; c0 += per_chunk_elems) {
+      const int cne = std::min(per_chunk_elems, ne - c0);
+      const int pts = cne * nupts;
+      int rc = ensure_ws<T, N>(h, pts);
+      if (rc) return rc;
+      const int eb_lo = (e0 + c0) / 32, eb_hi = (e0 + c0 + cne - 1) / 32;
+      dim3 grid((vsize + 31) / 32, eb_hi - eb_lo + 1, nupts);
+      // the kernels take absolute element blocks; shift the base pointer
+      hipLaunchKernelGGL(k_aosoa_gather, grid, dim3(256), 0, st,
+                         f + static_cast<size_t>(eb_lo) * vsize * 32, h->fc, nupts, ldim, vsize,
+                         e0 + c0 - eb_lo * 32, cne);
+      DGFS_LAUNCH_CHECK("aosoa_gather");
+      rc = run_chunk<T, N>(h, h->fc, h->Qc, pts, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_aosoa_scatter, grid, dim3(256), 0, st, h->Qc,
+                         Q + static_cast<size_t>(eb_lo) * vsize * 32, nupts, ldim, vsize,
+                         e0 + c0 - eb_lo * 32, cne);
+      DGFS_LAUNCH_CHECK("aosoa_scatter");
+    }
+    return DGFS_OK;
+  }
+  const int chunk = chunk_points(h, npts);
+  for (int p0 = 0; p0 < npts; p0 += chunk) {
+    const int pts = std::min(chunk, npts - p0);
+    int rc = ensure_ws<T, N>(h, pts);
+    if (rc) return rc;
+    rc = run_chunk<T, N>(h, f + static_cast<size_t>(p0) * n, Q + static_cast<size_t>(p0) * n, pts, st);
+    if (rc) return rc;
+  }
+  return DGFS_OK;
+}
+
+template <typename T>
+static int dispatch(VhsHandle* h, const double* f, double* Q, bool aosoa, int nupts, int ldim,
+                    int vsize, int e0, int ne, int npts, hipStream_t st) {
+  switch (h->N) {
+#define DGFS_CASE(NV) \
+  case NV:            \
+    return eval_impl<T, NV>(h, f, Q, aosoa, nupts, ldim, vsize, e0, ne, npts, st);
+    DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(24) DGFS_CASE(32)
+#undef DGFS_CASE
+    default:
+      return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d is not built", h->N);
+  }
+}
+
+static VhsHandle* as_handle(void* p) {
+  auto* h = static_cast<VhsHandle*>(p);
+  if (!h || h->magic != 0x76687331) return nullptr;
+  return h;
+}
+
+}  // namespace dgfs
+
+using namespace dgfs;
+
+extern "C" const char* dgfs_last_error(void) { return last_error_ref().c_str(); }
+extern "C" const char* dgfs_version(void) { return "dgfs_hip 0.1 (gfx950)"; }
+
+extern "C" int dgfs_vhs_create(const dgfs_vhs_desc* d, void** out) {
+  if (!d || !out) return fail(DGFS_ERR_INVALID_VALUE, "null descriptor or handle pointer");
+  *out = nullptr;
+  if (!supported_N(d->Nv))
+    return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d unsupported (built: 4 6 8 12 16 24 32)", d->Nv);
+  if (d->Nrho < 1 || d->M < 1 || !d->qz || !d->qw || !d->sz)
+    return fail(DGFS_ERR_INVALID_VALUE, "Nrho, M must be >= 1 and qz/qw/sz non-null");
+  if (d->precision != 64 && d->precision != 32)
+    return fail(DGFS_ERR_INVALID_VALUE, "precision must be 64 or 32 (got %d)", d->precision);
+  if (!(d->L > 0) || !(d->cw > 0)) return fail(DGFS_ERR_INVALID_VALUE, "L and cw must be positive");
+
+  auto h = std::make_unique<VhsHandle>();
+  h->N = d->Nv;
+  h->n = d->Nv * d->Nv * d->Nv;
+  h->Nrho = d->Nrho;
+  h->M = d->M;
+  h->prec = d->precision;
+  h->project = d->project ? 1 : 0;
+  h->L = d->L; h->gamma = d->gamma; h->prefac = d->prefac; h->sw = d->sw; h->cw = d->cw;
+  h->tsize = (d->precision == 64) ? 8 : 4;
+  const int N = h->N, n = h->n;
+
+  // b1_p(l) = qz_p^(gamma+2) 4 pi sinc((pi/L)(qz_p/2)|l|),
+  // b2(l) = sum_p qw_p qz_p^(gamma+2) 16 pi^2 sinc((pi/L) qz_p |l|)
+  // with sinc(s) = sin(s + 1e-15)/(s + 1e-15)  (vhs-gll.mako:11-14, 71-101)
+  std::vector<double> b1(static_cast<size_t>(h->Nrho) * n), b2(n, 0.0);
+  const double pi = 3.14159265358979323846;
+  auto sinc = [](double s) { return std::sin(s + 1e-15) / (s + 1e-15); };
+  for (int r = 0; r < n; ++r) {
+    const int I = r / (N * N), J = (r / N) % N, K = r % N;
+    const double lx = (I < N / 2) ? I : I - N, ly = (J < N / 2) ? J : J - N, lz = (K < N / 2) ? K : K - N;
+    const double ln = std::sqrt(lx * lx + ly * ly + lz * lz);
+    double acc = 0.0;
+    for (int p = 0; p < h->Nrho; ++p) {
+      const double qz = d->qz[p], pw = std::pow(qz, h->gamma + 2.0);
+      b1[static_cast<size_t>(p) * n + r] = pw * 4.0 * pi * sinc(pi / h->L * qz / 2.0 * ln);
+      acc += d->qw[p] * pw * 16.0 * pi * pi * sinc(pi / h->L * qz * ln);
+    }
+    b2[r] = acc;
+  }
+  std::vector<double> alpha(h->Nrho), wgt(h->Nrho);
+  for (int p = 0; p < h->Nrho; ++p) {
+    alpha[p] = pi / h->L * d->qz[p] / 2.0;
+    wgt[p] = 2.0 * d->qw[p] * h->sw / static_cast<double>(n);
+  }
+  auto upload = [&](void** dst, const std::vector<double>& src) -> int {
+    if (h->prec == 64) {
+      DGFS_HIP_TRY(hipMalloc(dst, src.size() * sizeof(double)));
+      DGFS_HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> tmp(src.begin(), src.end());
+      DGFS_HIP_TRY(hipMalloc(dst, tmp.size() * sizeof(float)));
+      DGFS_HIP_TRY(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return DGFS_OK;
+  };
+  int rc = upload(&h->b1, b1);
+  if (!rc) rc = upload(&h->b2, b2);
+  if (rc) return rc;
+  DGFS_HIP_TRY(hipMalloc(&h->alpha, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->wgt, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->szd, h->M * 3 * sizeof(double)));
+  DGFS_HIP_TRY(hipMemcpy(h->alpha, alpha.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->wgt, wgt.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->szd, d->sz, h->M * 3 * sizeof(double), hipMemcpyHostToDevice));
+
+  // (C C^T)^{-1} / cw^2 for the projection: A_jk = sum_r phi_j phi_k cw
+  double A[25] = {};
+  const double hh = 2.0 * h->L / N, c0 = -h->L + 0.5 * hh;
+  for (int r = 0; r < n; ++r) {
+    const int I = r / (N * N), J = (r / N) % N, K = r % N;
+    const double cx = c0 + I * hh, cy = c0 + J * hh, cz = c0 + K * hh;
+    const double phi[5] = {1.0, cx, cy, cz, cx * cx + cy * cy + cz * cz};
+    for (int j = 0; j < 5; ++j)
+      for (int k = 0; k < 5; ++k) A[j * 5 + k] += phi[j] * phi[k] * h->cw * h->cw;
+  }
+  if (invert5(A, h->Ainv)) return fail(DGFS_ERR_INVALID_VALUE, "singular invariant Gram matrix");
+  *out = h.release();
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_destroy(void* p) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  resolve_timing(h);
+  for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
+  h->pool.clear();
+  free_ws(h);
+  (void)hipFree(h->b1); (void)hipFree(h->b2); (void)hipFree(h->alpha); (void)hipFree(h->wgt);
+  (void)hipFree(h->szd);
+  h->magic = 0;
+  delete h;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_eval(void* p, const double* f, double* Q, int nupts, int ldim, int vsize,
+                             int neles, int e0, int ne, void* stream) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (!f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null distribution pointer");
+  if (f == Q) return fail(DGFS_ERR_INVALID_VALUE, "f and Q must not alias");
+  if (vsize != h->n) return fail(DGFS_ERR_INVALID_VALUE, "vsize %d != Nv^3 = %d", vsize, h->n);
+  if (nupts < 0 || neles < 0 || e0 < 0 || ne < 0 || e0 + ne > neles)
+    return fail(DGFS_ERR_INVALID_VALUE, "element range [%d, %d) outside [0, %d)", e0, e0 + ne, neles);
+  const long long nparr = ((static_cast<long long>(neles) + 31) / 32) * 32;
+  if (static_cast<long long>(ldim) < nparr * vsize)
+    return fail(DGFS_ERR_INVALID_VALUE, "ldim %d smaller than the AoSoA row %lld", ldim, nparr * vsize);
+  auto st = static_cast<hipStream_t>(stream);
+  return (h->prec == 64) ? dispatch<double>(h, f, Q, true, nupts, ldim, vsize, e0, ne, 0, st)
+                         : dispatch<float>(h, f, Q, true, nupts, ldim, vsize, e0, ne, 0, st);
+}
+
+extern "C" int dgfs_vhs_eval_contig(void* p, const double* f, double* Q, int npts, void* stream) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (!f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null distribution pointer");
+  if (f == Q) return fail(DGFS_ERR_INVALID_VALUE, "f and Q must not alias");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts < 0");
+  if (npts == 0) return DGFS_OK;
+  auto st = static_cast<hipStream_t>(stream);
+  return (h->prec == 64) ? dispatch<double>(h, f, Q, false, 0, 0, 0, 0, 0, npts, st)
+                         : dispatch<float>(h, f, Q, false, 0, 0, 0, 0, 0, npts, st);
+}
+
+extern "C" int dgfs_vhs_get_config(void* p, int* ngroups, int* nslots, int* threads) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (ngroups) *ngroups = h->last_groups;
+  if (nslots) *nslots = h->last_slots;
+  if (threads) *threads = h->threads;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_set_groups(void* p, int ngroups) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (ngroups < 0) return fail(DGFS_ERR_INVALID_VALUE, "ngroups < 0");
+  h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_set_timing(void* p, int on) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  resolve_timing(h);
+  h->timing = on ? 1 : 0;
+  for (int k = 0; k < 4; ++k) {
+    h->acc_ms[k] = 0.0;
+    h->acc_n[k] = 0;
+  }
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (!ms || !counts) return fail(DGFS_ERR_INVALID_VALUE, "null output arrays");
+  resolve_timing(h);
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = h->acc_ms[k];
+    counts[k] = h->acc_n[k];
+  }
+  return DGFS_OK;
+}

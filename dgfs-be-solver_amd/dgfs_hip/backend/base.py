@@ -1,0 +1,111 @@
+"""HIP backend: allocation, matrix types and the kernel-provider walk.
+
+Mirrors the contract of frfs/backends/base/backend.py:28-176: kernels are
+looked up by name across the providers in order, a provider that cannot
+serve the arguments raises NotSuitableError and the next one is tried
+(backend.py:161-170).  Kernels returned here are bound HIP launches on the
+backend stream; ``queue << kern`` and ``runall`` keep the reference's queue
+vocabulary with stream-ordered (immediate) execution.
+"""
+import numpy as np
+
+from dgfs_hip.backend.types import ConstMatrix, Matrix, MatrixBank
+
+
+class NotSuitableError(Exception):
+    pass
+
+
+class Queue(list):
+    """Kernel queue: kernels run in insertion order on the backend stream."""
+
+    def __init__(self, backend):
+        super().__init__()
+        self.backend = backend
+
+    def __lshift__(self, kern):
+        self.append(kern)
+        return self
+
+    def __mod__(self, kern):
+        self.append(kern)
+        self.run()
+        return self
+
+    def run(self):
+        for k in self:
+            k.run(self)
+        self.clear()
+
+
+class HIPBackend:
+    name = "hip"
+
+    def __init__(self, cfg=None, device=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("HIPBackend needs a visible HIP device")
+        devid = 0
+        if cfg is not None and cfg.hasopt("backend-hip", "device-id"):
+            d = cfg.get("backend-hip", "device-id")
+            devid = int(d) if d.isdigit() else 0
+        if device is not None:
+            devid = int(device)
+        self.device = torch.device("cuda", devid)
+        torch.cuda.set_device(self.device)
+        self.alignb = 128
+        self.soasz = 32
+        prec = cfg.get("backend", "precision", "double") if cfg is not None else "double"
+        if prec not in ("double",):
+            raise ValueError("the distribution registers are FP64 (precision = double)")
+        self.fpdtype = np.float64
+        self._providers = []
+        self._stream = torch.cuda.current_stream(self.device)
+
+    @property
+    def stream_ptr(self):
+        return self._stream.cuda_stream
+
+    def _alloc(self, shape, dtype):
+        import torch
+        tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+               np.dtype(np.int32): torch.int32}[np.dtype(dtype)]
+        return torch.zeros(shape, dtype=tdt, device=self.device)
+
+    def matrix(self, ioshape, initval=None, extent=None, aliases=None, tags=set()):
+        return Matrix(self, ioshape, initval, extent, aliases, tags)
+
+    def const_matrix(self, initval, extent=None, tags=set()):
+        return ConstMatrix(self, initval, extent, tags)
+
+    def matrix_bank(self, mats, initbank=0):
+        return MatrixBank(self, mats, initbank)
+
+    def const_array(self, ary, dtype=np.float64):
+        import torch
+        return torch.as_tensor(np.ascontiguousarray(ary, dtype=dtype), device=self.device)
+
+    def queue(self):
+        return Queue(self)
+
+    def runall(self, queues):
+        for q in queues:
+            q.run()
+
+    def kernel(self, name, *args, **kwargs):
+        for prov in self._providers:
+            kern = getattr(prov, name, None)
+            if kern is None:
+                continue
+            try:
+                return kern(*args, **kwargs)
+            except NotSuitableError:
+                continue
+        raise KeyError(f"kernel not found: {name}")
+
+    def register_provider(self, prov):
+        self._providers.append(prov)
+
+    def synchronize(self):
+        import torch
+        torch.cuda.synchronize(self.device)

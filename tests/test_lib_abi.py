@@ -1,0 +1,51 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares.
+CPU only: no compute call is made."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            syms |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dgfs_\w+|frfs_\w+)\s*\(", txt, flags=re.M))
+    return syms
+
+
+def test_library_exports_all_declared_symbols():
+    from dgfs_hip import _lib
+    path = _lib.lib_path()
+    if not os.path.exists(path):
+        pytest.fail(f"{path} missing: run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    syms = declared_symbols()
+    assert len(syms) >= 8
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_declared_symbols():
+    from dgfs_hip import _lib
+    from dgfs_hip import fr  # noqa: F401  registers the FR entry points
+    assert declared_symbols() <= set(_lib.exported_symbols())
+
+
+def test_error_path_without_gpu_work():
+    from dgfs_hip import _lib
+    lib = _lib.load()
+    # a null descriptor is rejected on the host before any device call
+    st = lib.dgfs_vhs_create(None, None)
+    assert st == 1
+    assert b"null" in lib.dgfs_last_error()
+    with pytest.raises(_lib.DGFSInvalidValue):
+        _lib.check(st, "dgfs_vhs_create")
+    assert lib.dgfs_vhs_destroy(None) == 5
+    assert lib.dgfs_version().startswith(b"dgfs_hip")

@@ -1,0 +1,158 @@
+"""Pins of the FP64 oracle (oracle/vhs.py) against what the mathematics and the
+reference's own data files fix.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vhs as O
+
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def test_gauss_legendre_matches_numpy():
+    for n in (1, 2, 5, 8, 16):
+        z, w = O.gauss_jacobi(n)
+        z2, w2 = np.polynomial.legendre.leggauss(n)
+        assert np.allclose(z, z2, atol=1e-14) and np.allclose(w, w2, atol=1e-14)
+
+
+def test_gauss_jacobi_matches_scipy():
+    import scipy.special as ss
+    for (n, a, b) in [(6, 0.5, 1.5), (4, -0.3, 0.2), (12, 1.0, 1.0)]:
+        z, w = O.gauss_jacobi(n, a, b)
+        z2, w2 = ss.roots_jacobi(n, a, b)
+        assert np.allclose(z, z2, atol=1e-13) and np.allclose(w, w2, rtol=1e-12)
+
+
+def test_quadrature_tables_of_reference():
+    # frfs/quadrules/line/gauss-legendre-n3-d5-spu.txt and
+    # gauss-legendre-lobatto-n3-d3-spu.txt (data)
+    z, w = O.gauss_jacobi(3)
+    assert np.allclose(z, [-0.774596669241483377035853079956, 0.0, 0.774596669241483377035853079956], atol=1e-15)
+    assert np.allclose(w, [0.555555555555555555555555555556, 0.888888888888888888888888888889,
+                           0.555555555555555555555555555556], atol=1e-15)
+    z, w = O.gauss_lobatto_jacobi(3)
+    assert np.allclose(z, [-1, 0, 1], atol=1e-15) and np.allclose(w, [1 / 3, 4 / 3, 1 / 3], atol=1e-15)
+    z, w = O.gauss_lobatto_jacobi(4)
+    assert np.allclose(z, [-1, -1 / math.sqrt(5), 1 / math.sqrt(5), 1], atol=1e-14)
+    assert np.allclose(w, [1 / 6, 5 / 6, 5 / 6, 1 / 6], atol=1e-14)
+
+
+def test_designs_match_reference_tables():
+    # first rows of frfs/sphericaldesign/symmetric/womersley-ss005-m00012.txt
+    ref12 = np.array([
+        [0.0, 0.0, 1.0],
+        [8.9442719099991586e-01, 0.0, 4.4721359549995804e-01],
+        [2.7639320225002084e-01, -8.5065080835203999e-01, 4.4721359549995804e-01],
+        [-7.2360679774997894e-01, 5.2573111211913381e-01, 4.4721359549995782e-01],
+        [-7.2360679774997883e-01, -5.2573111211913381e-01, 4.4721359549995804e-01],
+        [2.7639320225002106e-01, 8.5065080835203999e-01, 4.4721359549995782e-01]])
+    d = O.antipodal_design(6)
+    assert np.allclose(d[:6], ref12, atol=2e-16 * 8)
+    assert np.allclose(d[6:], -ref12, atol=2e-16 * 8)
+    d3 = O.antipodal_design(3)
+    assert np.allclose(d3[:3], [[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+
+
+@pytest.mark.parametrize("M", [3, 6, 12])
+def test_design_exactness(M):
+    X = O.antipodal_design(M)
+    assert np.allclose(np.linalg.norm(X, axis=1), 1.0, atol=1e-15)
+    assert np.allclose(X[M:], -X[:M])
+    deg = {3: 3, 6: 5, 12: 5}[M]
+    rng = np.random.default_rng(M)
+    for d in range(1, deg + 1):
+        for _ in range(10):
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            exact = 0.0 if d % 2 else 1.0 / (d + 1)
+            assert abs(np.mean((X @ v) ** d) - exact) < 1e-14
+
+
+def test_tables_closed_forms():
+    vm = O.VelocityMesh(8, 5.0, 6, 6)
+    gamma = 0.38
+    aa, b1, b2 = O.vhs_tables(vm, gamma)
+    # b2 at l = 0 is 16 pi^2 sum_p qw_p qz_p^(gamma+2) ~ 16 pi^2 R^(gamma+3)/(gamma+3)
+    want = 16 * math.pi ** 2 * np.sum(vm.qw * vm.qz ** (gamma + 2))
+    assert abs(b2[0] - want) < 1e-12 * want
+    # b1 at l = 0: sinc(1e-15) == 1 exactly
+    assert np.allclose(b1[:, 0], vm.qz ** (gamma + 2) * 4 * math.pi, rtol=1e-15)
+    # a is linear in l: a(-l) = -a(l) away from the Nyquist row
+    l = O.lvectors(8)
+    r = int(np.where((l[0] == 1) & (l[1] == 2) & (l[2] == -3))[0][0])
+    rm = int(np.where((l[0] == -1) & (l[1] == -2) & (l[2] == 3))[0][0])
+    assert np.allclose(aa[:, r], -aa[:, rm], atol=1e-14)
+
+
+def test_q_matches_bruteforce_dft():
+    rng = np.random.default_rng(0)
+    for (N, L, Nrho, M, gamma) in [(4, 3.0, 2, 3, 1.0), (6, 4.0, 3, 6, 0.38)]:
+        vm = O.VelocityMesh(N, L, Nrho, M)
+        o = O.VHSOracle(vm, gamma, 0.7)
+        f = rng.random(vm.n)
+        a = o.q(f)
+        b = O.q_bruteforce(f, vm, gamma, 0.7)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-13
+
+
+def test_bkw_known_answer():
+    # Maxwell molecules, B = 1/(4 pi): Q(f_BKW) = d f_BKW / dt
+    vm = O.VelocityMesh(32, 6.0, 16, 6)
+    o = O.VHSOracle(vm, 0.0, 1.0 / (4 * math.pi))
+    Q = o.q(O.bkw(vm, 5.5))
+    d = O.bkw_dfdt(vm, 5.5)
+    assert np.linalg.norm(Q - d) / np.linalg.norm(d) < 2e-4
+
+
+def test_bkw_derivative_is_the_time_derivative():
+    vm = O.VelocityMesh(8, 6.0, 2, 3)
+    e = 1e-6
+    fd = (O.bkw(vm, 5.5 + e) - O.bkw(vm, 5.5 - e)) / (2 * e)
+    assert np.allclose(fd, O.bkw_dfdt(vm, 5.5), atol=1e-9)
+
+
+def test_maxwellian_is_equilibrium_and_conservation():
+    vm = O.VelocityMesh(32, 6.0, 8, 6)
+    o = O.VHSOracle(vm, 0.38, 1.0)
+    m = O.maxwellian(vm, 1.0, [0.2, -0.1, 0.3], 1.0)
+    Q = o.q(m)
+    assert np.linalg.norm(Q) / np.linalg.norm(m) < 1e-5
+    # the fast spectral method conserves only approximately: the drift of a
+    # smooth bimodal state at M=6, Nrho=8 is measured at ~1e-2 of int |Q|
+    f = 0.5 * O.maxwellian(vm, 1.0, [0.6, -0.1, 0.3], 0.8) + \
+        0.5 * O.maxwellian(vm, 1.0, [-0.4, 0.2, 0.0], 1.5)
+    Q = o.q(f)
+    inv = O.collision_invariants(vm, Q)
+    scale = O.collision_invariants(vm, np.abs(Q))[0]
+    assert np.all(np.abs(inv) < 2e-2 * scale)
+    # and a Maxwellian at rest on the grid centre is an equilibrium to roundoff
+    m0 = O.maxwellian(vm, 1.0, [0.0, 0.0, 0.0], 1.0)
+    assert np.linalg.norm(o.q(m0)) / np.linalg.norm(m0) < 1e-5
+
+
+def test_translation_equivariance():
+    # shifting f by whole grid cells shifts Q (periodic spectral operator)
+    vm = O.VelocityMesh(8, 5.0, 3, 6)
+    o = O.VHSOracle(vm, 0.38, 1.0)
+    rng = np.random.default_rng(4)
+    f = rng.random(vm.n)
+    Q = o.q(f).reshape(8, 8, 8)
+    fs = np.roll(f.reshape(8, 8, 8), (1, -2, 3), axis=(0, 1, 2)).ravel()
+    Qs = o.q(fs).reshape(8, 8, 8)
+    assert np.allclose(np.roll(Q, (1, -2, 3), axis=(0, 1, 2)), Qs, atol=1e-12 * np.abs(Q).max())
+
+
+def test_golden_fixtures_reproduce():
+    import os
+    for fn in sorted(os.listdir(GOLD)):
+        if not fn.startswith("vhs_") or not fn.endswith(".npz"):
+            continue
+        d = np.load(os.path.join(GOLD, fn))
+        N, Nrho, M = (int(x) for x in d["dims"])
+        L, gamma, prefac = (float(x) for x in d["params"])
+        vm = O.VelocityMesh(N, L, Nrho, M)
+        o = O.VHSOracle(vm, gamma, prefac)
+        Q = o.q(d["f"])
+        assert np.allclose(Q, d["Q"], rtol=1e-12, atol=1e-14 * np.abs(d["Q"]).max()), fn
