@@ -80,6 +80,20 @@ _SIGNATURES = {
 EXTRA_SIGNATURES: dict = {}
 
 
+def register(signatures: dict):
+    """Add entry points; applied at once if the library is already loaded."""
+    EXTRA_SIGNATURES.update(signatures)
+    if _lib is not None:
+        _apply(_lib, signatures)
+
+
+def _apply(lib, signatures):
+    for name, args in signatures.items():
+        fn = getattr(lib, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
+
+
 def lib_path() -> str:
     return os.environ.get("DGFS_HIP_LIB", LIB_PATH)
 
@@ -99,10 +113,7 @@ def load():
     lib.dgfs_last_error.argtypes = []
     lib.dgfs_version.restype = C.c_char_p
     lib.dgfs_version.argtypes = []
-    for name, args in {**_SIGNATURES, **EXTRA_SIGNATURES}.items():
-        fn = getattr(lib, name)
-        fn.restype = C.c_int
-        fn.argtypes = args
+    _apply(lib, {**_SIGNATURES, **EXTRA_SIGNATURES})
     _lib = lib
     return lib
 

@@ -1,0 +1,69 @@
+"""Configuration builders for the benchmark workloads (INI keys of the
+reference, frfs/inifile.py; nondimensional units H0 = T0 = rho0 =
+molarMass0 = 1 so lengths are in H0 and velocities in u0 = sqrt(2 R0))."""
+import math
+
+from dgfs_hip.inifile import Inifile
+
+U0 = math.sqrt(2 * 8.3144598)
+
+
+def base_cfg(Nv, L, Nrho, M, omega=0.5, Kn=None, prefactor=None, order=2, NvBatchSize=None,
+             precision="double", project=None):
+    cfg = Inifile()
+    for k, v in dict(H0=1.0, T0=1.0, rho0=1.0, molarMass0=1.0).items():
+        cfg.set("non-dim", k, v)
+    cfg.set("constants", "Nv", Nv)
+    cfg.set("constants", "NvBatchSize", NvBatchSize or Nv ** 3)
+    cfg.set("constants", "Nrho", Nrho)
+    cfg.set("velocity-mesh", "cmax", 0.0)
+    cfg.set("velocity-mesh", "Tmax", 1.0)
+    cfg.set("velocity-mesh", "dev", L)
+    cfg.set("spherical-design-rule", "ssrule", "womersley")
+    cfg.set("spherical-design-rule", "M", M)
+    cfg.set("scattering-model", "type", "vhs-gll")
+    cfg.set("scattering-model", "omega", omega)
+    if prefactor is not None:
+        cfg.set("scattering-model", "prefactor", prefactor)
+    else:
+        cfg.set("scattering-model", "Kn", Kn if Kn is not None else 1.0)
+    cfg.set("scattering-model", "precision", precision)
+    if project is not None:
+        cfg.set("scattering-model", "project", "true" if project else "false")
+    cfg.set("solver", "system", "dgfs")
+    cfg.set("solver", "order", order)
+    cfg.set("solver-interfaces", "riemann-solver", "rusanov")
+    cfg.set("solver-interfaces-line", "flux-pts", "gauss-legendre")
+    cfg.set("solver-elements-quad", "soln-pts", "gauss-legendre")
+    cfg.set("solver-elements-tri", "soln-pts", "williams-shunn")
+    return cfg
+
+
+def maxwellian_section(cfg, sect, rho, T, u, kind=None):
+    if kind:
+        cfg.set(sect, "type", kind)
+    cfg.set(sect, "rho", rho)
+    cfg.set(sect, "T", T)
+    cfg.set(sect, "ux", u[0] * U0)
+    cfg.set(sect, "uy", u[1] * U0)
+    cfg.set(sect, "uz", u[2] * U0)
+
+
+def couette_cfg(Nv=24, L=7.0, Nrho=12, M=12, Kn=0.1, omega=0.5, order=2, uwall=0.5, **kw):
+    """Planar Couette flow: x-periodic channel, diffuse walls at T=1 moving at
+    +-uwall, equilibrium Maxwellian initial state (rho 1, T 1)."""
+    cfg = base_cfg(Nv, L, Nrho, M, omega=omega, Kn=Kn, order=order, **kw)
+    maxwellian_section(cfg, "soln-ics", 1.0, 1.0, (0, 0, 0), "maxwellian")
+    maxwellian_section(cfg, "soln-bcs-bottom", 1.0, 1.0, (-uwall, 0, 0), "dgfs-wall-diffuse")
+    maxwellian_section(cfg, "soln-bcs-top", 1.0, 1.0, (uwall, 0, 0), "dgfs-wall-diffuse")
+    return cfg
+
+
+def cavity_cfg(Nv=32, L=6.0, Nrho=16, M=12, Kn=0.05, omega=0.81, order=3, ulid=0.3, **kw):
+    """Lid-driven cavity: diffuse walls at T=1, the top one moving at ulid."""
+    cfg = base_cfg(Nv, L, Nrho, M, omega=omega, Kn=Kn, order=order, **kw)
+    maxwellian_section(cfg, "soln-ics", 1.0, 1.0, (0, 0, 0), "maxwellian")
+    for name in ("bottom", "left", "right"):
+        maxwellian_section(cfg, f"soln-bcs-{name}", 1.0, 1.0, (0, 0, 0), "dgfs-wall-diffuse")
+    maxwellian_section(cfg, "soln-bcs-top", 1.0, 1.0, (ulid, 0, 0), "dgfs-wall-diffuse")
+    return cfg

@@ -1,0 +1,101 @@
+"""In-process structured meshes: rectangles of quadrilaterals or of
+triangles (each quad split along its diagonal), with optional periodic
+pairing of the x-faces.  Replaces mesh import (frfs/readers) for the
+structured configurations; element nodes follow the reference's linear node
+ordering (TensorProdShape.std_ele / TriShape.std_ele)."""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Mesh:
+    eletype: str
+    nodes: np.ndarray                      # (nspts, neles, ndims)
+    interfaces: list                       # (eL, fL, eR, fR, shift(ndims))
+    boundaries: dict = field(default_factory=dict)   # name -> [(e, f)]
+
+    @property
+    def neles(self):
+        return self.nodes.shape[1]
+
+
+def rect_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic_x=False, periodic_y=False):
+    """Quad faces: 0 bottom, 1 right, 2 top, 3 left; element e = ix + nx*iy.
+    Boundaries: 'bottom'/'top' unless periodic in y, 'left'/'right' unless
+    periodic in x."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    neles = nx * ny
+    nodes = np.empty((4, neles, 2))
+    for iy in range(ny):
+        for ix in range(nx):
+            e = ix + nx * iy
+            nodes[:, e] = [(xs[ix], ys[iy]), (xs[ix + 1], ys[iy]), (xs[ix], ys[iy + 1]), (xs[ix + 1], ys[iy + 1])]
+    inters, bnd = [], {"bottom": [], "top": [], "left": [], "right": []}
+    zero = np.zeros(2)
+    for iy in range(ny):
+        for ix in range(nx):
+            e = ix + nx * iy
+            if ix + 1 < nx:
+                inters.append((e, 1, e + 1, 3, zero))
+            elif periodic_x:
+                inters.append((e, 1, nx * iy, 3, np.array([x1 - x0, 0.0])))
+            else:
+                bnd["right"].append((e, 1))
+            if ix == 0 and not periodic_x:
+                bnd["left"].append((e, 3))
+            if iy + 1 < ny:
+                inters.append((e, 2, e + nx, 0, zero))
+            elif periodic_y:
+                inters.append((e, 2, ix, 0, np.array([0.0, y1 - y0])))
+            else:
+                bnd["top"].append((e, 2))
+            if iy == 0 and not periodic_y:
+                bnd["bottom"].append((e, 0))
+    if periodic_x:
+        del bnd["left"], bnd["right"]
+    if periodic_y:
+        del bnd["bottom"], bnd["top"]
+    return Mesh("quad", nodes, inters, bnd)
+
+
+def rect_tris(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic_x=False):
+    """Each quad (ix, iy) becomes a lower-right triangle (2*q) with nodes
+    (x0,y0),(x1,y0),(x1,y1) and an upper-left one (2*q+1) with nodes
+    (x1,y1),(x0,y1),(x0,y0).  Triangle faces follow TriShape: 0 between
+    nodes 0-1, 1 between nodes 1-2, 2 between nodes 2-0."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    nq = nx * ny
+    nodes = np.empty((3, 2 * nq, 2))
+    for iy in range(ny):
+        for ix in range(nx):
+            q = ix + nx * iy
+            a, b, c, d = (xs[ix], ys[iy]), (xs[ix + 1], ys[iy]), (xs[ix + 1], ys[iy + 1]), (xs[ix], ys[iy + 1])
+            nodes[:, 2 * q] = [a, b, c]
+            nodes[:, 2 * q + 1] = [c, d, a]
+    inters, bnd = [], {"bottom": [], "top": [], "left": [], "right": []}
+    zero = np.zeros(2)
+    for iy in range(ny):
+        for ix in range(nx):
+            q = ix + nx * iy
+            lo, up = 2 * q, 2 * q + 1
+            inters.append((lo, 2, up, 2, zero))          # diagonal
+            if ix + 1 < nx:
+                inters.append((lo, 1, 2 * (q + 1) + 1, 1, zero))
+            elif periodic_x:
+                inters.append((lo, 1, 2 * (nx * iy) + 1, 1, np.array([x1 - x0, 0.0])))
+            else:
+                bnd["right"].append((lo, 1))
+            if ix == 0 and not periodic_x:
+                bnd["left"].append((up, 1))
+            if iy + 1 < ny:
+                inters.append((up, 0, 2 * (q + nx), 0, zero))
+            else:
+                bnd["top"].append((up, 0))
+            if iy == 0:
+                bnd["bottom"].append((lo, 0))
+    if periodic_x:
+        del bnd["left"], bnd["right"]
+    return Mesh("tri", nodes, inters, bnd)

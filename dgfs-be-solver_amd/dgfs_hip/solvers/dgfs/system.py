@@ -1,0 +1,102 @@
+"""DGFS system on the HIP path (mirrors frfs/solvers/dgfs/system.py:17-231).
+
+Holds the velocity mesh, one element type, its interior and boundary
+interfaces, the scattering model and the full-velocity registers
+(nupts, vsize, neles).  The element registers hold one velocity batch of
+NvBatchSize velocities; rhs/updatebc act on the batch starting at varidx,
+collide on the full registers, exactly as the reference's steppers expect.
+"""
+import numpy as np
+
+from dgfs_hip.shapes import get_shape
+from dgfs_hip.solvers.dgfs.elements import DGFSElements
+from dgfs_hip.solvers.dgfs.initcond import DGFSInitCondition
+from dgfs_hip.solvers.dgfs.inters import DGFSBCInters, DGFSIntInters
+from dgfs_hip.solvers.dgfs.scattering import DGFSScatteringModel
+from dgfs_hip.solvers.dgfs.velocitymesh import DGFSVelocityMesh
+from dgfs_hip.util import subclass_where
+
+
+class DGFSSystem:
+    name = "dgfs"
+    elementscls = DGFSElements
+    velocitymeshcls = DGFSVelocityMesh
+    scatteringcls = DGFSScatteringModel
+
+    def __init__(self, backend, mesh, cfg, nregs, initsoln=None):
+        self.backend = be = backend
+        self.cfg = cfg
+        self.mesh = mesh
+        self.vm = vm = self.velocitymeshcls(backend, cfg)
+        order = cfg.getint("solver", "order")
+        et = mesh.eletype
+        upts_rule = cfg.get(f"solver-elements-{et}", "soln-pts",
+                            "gauss-legendre" if et == "quad" else "williams-shunn")
+        fpts_rule = cfg.get("solver-interfaces-line", "flux-pts", "gauss-legendre")
+        shape = get_shape(et, order, upts_rule, fpts_rule)
+        self.nvars = vm.NvBatchSize()
+        self.ele = self.elementscls(be, shape, mesh.nodes, vm, self.nvars, nregs)
+        self.ele_types = [et]
+        self.ndims = self.ele.ndims
+        self.iint = DGFSIntInters(be, self.ele, mesh.interfaces)
+        self.bcints = []
+        for name, faces in mesh.boundaries.items():
+            sect = f"soln-bcs-{name}"
+            bctype = cfg.get(sect, "type")
+            cls = subclass_where(DGFSBCInters, type=bctype)
+            self.bcints.append(cls(be, self.ele, faces, cfg, sect))
+
+        # unit-CFL time step (system.py:50-66)
+        djac = self.ele.djac_at_np("upts")
+        self.unit_cfl_dt = float(np.sqrt(np.min(djac)) / vm.L() / self.ndims)
+
+        smn = cfg.get("scattering-model", "type")
+        self.sm = subclass_where(self.scatteringcls, scattering_model=smn)(be, cfg, vm)
+
+        # full-velocity registers
+        nupts, neles, vsize = self.ele.nupts, self.ele.neles, vm.vsize()
+        if initsoln is not None:
+            f0 = np.asarray(initsoln, dtype=float).reshape(nupts, vsize, neles)
+        else:
+            icn = cfg.get("soln-ics", "type")
+            ic = subclass_where(DGFSInitCondition, model=icn)(be, cfg, vm, "soln-ics")
+            f0 = np.empty((nupts, vsize, neles))
+            ic.apply_init_vals(f0, self.ele)
+        self.eles_scal_upts_full = [be.matrix((nupts, vsize, neles), f0, tags={"align"})
+                                    for _ in range(nregs)]
+        self.eles_scal_upts_inb_full = be.matrix_bank(self.eles_scal_upts_full)
+
+    def rhs(self, t, uinbank, foutbank, batch, varidx):
+        """Advection of one velocity batch: fout = -div F(uin) (system.py:140-177)."""
+        ele = self.ele
+        ele.disu(uinbank)
+        ele.tdisf(uinbank, varidx)
+        ele.tdivtpcorf(foutbank)
+        self.iint.comm_flux(varidx)
+        for b in self.bcints:
+            b.comm_flux(t, varidx)
+        ele.tdivtconf(foutbank)
+        ele.negdivconf(foutbank)
+
+    def updatebc(self, t, uinbank, foutbank, batch, varidx):
+        """Boundary accumulations over one velocity batch (system.py:180-196)."""
+        self.ele.disu(uinbank)
+        for b in self.bcints:
+            b.update_bc(t, varidx)
+
+    def collide(self, t, uinbank, foutbank):
+        """Q at every solution point: one batched call (system.py:199-218)."""
+        fsoln = self.eles_scal_upts_full
+        self.sm.fs_batch(fsoln[uinbank], fsoln[foutbank])
+
+    def moments(self, bank):
+        """(nupts, 15, neles): rho, Ux, Uy, Uz, T, qx, qy, qz, Pxx, Pxy, Pxz,
+        Pyy, Pyz, Pzz, p (plugins/dgfsmomwriterstd.py:14-214)."""
+        import torch
+        from dgfs_hip import fr
+        f = self.eles_scal_upts_full[bank]
+        nupts, vsize, neles = f.ioshape
+        out = torch.empty((nupts, 15, neles), dtype=torch.float64, device=self.backend.device)
+        fr.moments(nupts, neles, vsize, f, f.leaddim, self.vm.d_cv(), self.vm.cw(), out,
+                   self.backend.stream_ptr)
+        return out.cpu().numpy()

@@ -96,6 +96,83 @@ int dgfs_vhs_set_groups(void* handle, int ngroups);
 int dgfs_vhs_set_timing(void* handle, int on);
 int dgfs_vhs_get_timing(void* handle, double* ms /*[4]*/, int* counts /*[4]*/);
 
+/* ------------------------------------------------------------------------
+ * Flux-reconstruction advection of the velocity columns and RK stage kernels.
+ * Matrices are the reference's AoSoA layout: element (row, var v, elem e) at
+ * row*ld + ((e/32)*nvars + v)*32 + e%32; velocities are the variables.
+ * ---------------------------------------------------------------------- */
+
+/* C = alpha A B + beta C with a constant operator A [m][k] (row stride lda,
+ * k <= 64), B [k][ldb], C [m][ldc], ncol columns; FP64 on MFMA.
+ * Replaces backend.kernel('mul', a, b, out, alpha, beta):
+ * CUDACUBLASKernels.mul (frfs/backends/cuda/cublas.py:105-137) and
+ * CUDAGiMMiKKernels.mul (frfs/backends/cuda/gimmik.py:18-48). */
+int frfs_mul(const double* A, int m, int k, long lda, const double* B, long ldb, double* C,
+             long ldc, long ncol, double alpha, double beta, void* stream);
+
+/* Transformed advective flux f[i][upt][v][e] = sum_k smats[i][upt][k][e]
+ * c_k[vidx+v] u[upt][v][e]; cv is [3][vsize].  Replaces the 'tflux' pointwise
+ * kernel (frfs/solvers/dgfs/kernels/tflux.mako:6-25, flux.mako:4-20). */
+int dgfs_tflux(int ndims, int nupts, int neles, int nvars, int vidx, const double* u, long ldu,
+               const double* smats, long ldsm, double* f, long ldf, const double* cv, int vsize,
+               void* stream);
+
+/* out = -rcpdjac out (no source term).  Replaces 'negdivconf'
+ * (frfs/solvers/dgfs/kernels/negdivconf.mako:5-14). */
+int dgfs_negdivconf(int nupts, int neles, int nvars, double* out, long ld, const double* rcpdjac,
+                    long ldr, void* stream);
+
+/* Interior common flux: exact upwind fn = 1/2 n.c (ul+ur) + 1/2 |n.c| (ul-ur),
+ * then ul = |n| fn, ur = -|n| fn.  Interface point i of the left/right side
+ * is at L + loff[i] (+ 32 v for velocity v); nl is [ndims][nfpts].  Replaces
+ * 'intcflux' with the rusanov solver (frfs/solvers/dgfs/kernels/
+ * intcflux.mako:7-25, rsolvers/rusanov.mako:5-52; views of
+ * frfs/solvers/dgfs/inters.py:17-58). */
+int dgfs_intcflux(int ndims, int nfpts, int nvars, int vidx, double* L, const long* loff, double* R,
+                  const long* roff, const double* nl, const double* magnl, const double* cv,
+                  int vsize, void* stream);
+
+/* Boundary common flux, bctype 0 diffuse wall, 1 specular wall, 2 inlet,
+ * 3 outlet, 4 vanish, 5 normal-shock inlet: ghost state then the upwind flux,
+ * ul = |n| fn.  f0: wall/inlet Maxwellian [vsize]; bc_vals: diffuse [2][nfpts]
+ * flux balance, specular stored states [vsize][nfpts]; rmap: specular
+ * reflection map [vsize].  Replaces 'bccflux' (frfs/solvers/dgfs/kernels/
+ * bccflux.mako:8-39 with bcs/dgfs-wall-diffuse.mako, dgfs-wall-specular.mako,
+ * dgfs-inlet.mako, dgfs-outlet.mako, dgfs-vanish.mako,
+ * dgfs-inlet-normalshock.mako; frfs/solvers/dgfs/inters.py:175-443). */
+int dgfs_bccflux(int bctype, int ndims, int nfpts, int nvars, int vidx, double* L, const long* loff,
+                 const double* nl, const double* magnl, const double* f0, const double* bc_vals,
+                 const int* rmap, double Ux, double Uy, double Uz, const double* cv, int vsize,
+                 void* stream);
+
+/* Boundary update over one velocity batch: diffuse wall accumulates
+ * bc0 += sum_{(c-U).n>=0} (c-U).n ul cw, bc1 += sum_{<0} (c-U).n f0 cw (reset
+ * when vidx == 0); specular stores ul.  Replaces 'bccupdate'
+ * (frfs/solvers/dgfs/kernels/bccupdate.mako:8-24, bcs/dgfs-wall-diffuse-
+ * update.mako:4-40, dgfs-wall-specular-update.mako:4-16). */
+int dgfs_bccupdate(int bctype, int ndims, int nfpts, int nvars, int vidx, const double* L,
+                   const long* loff, const double* nl, const double* f0, double* bc_vals, double Ux,
+                   double Uy, double Uz, double cw, const double* cv, int vsize, void* stream);
+
+/* x[0] = a[0] x[0] + sum_{i>0} a[i] x[i] over n values, nv <= 8 (x, a are host
+ * arrays of device pointers / coefficients).  Replaces 'axnpby'
+ * (frfs/backends/cuda/blasext.py:14-43, kernels/axnpby.mako:5-36). */
+int frfs_axnpby(long n, int nv, double* const* x, const double* a, void* stream);
+
+/* Velocity-slice copy dst[.., d0+v, ..] = src[.., s0+v, ..] for v < count
+ * between registers of nvsrc and nvdst variables.  Replaces 'copy_to_reg' /
+ * 'copy_from_reg' (frfs/backends/cuda/blasext.py:77-138,
+ * kernels/copy_dgfs.mako:5-26). */
+int dgfs_copy_reg(int nupts, int neles, const double* src, long lds, int nvsrc, int s0,
+                  double* dst, long ldd, int nvdst, int d0, int count, void* stream);
+
+/* Macroscopic moments of the full register f (nupts, vsize, neles) into
+ * out[upt][m][e], m over rho, Ux, Uy, Uz, T, qx, qy, qz, Pxx, Pxy, Pxz, Pyy,
+ * Pyz, Pzz, p.  Replaces the host-side DGFSMomWriterStdPlugin._compute_moments
+ * (frfs/plugins/dgfsmomwriterstd.py:14-214). */
+int dgfs_moments(int nupts, int neles, int vsize, const double* f, long ld, const double* cv,
+                 double cw, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

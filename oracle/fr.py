@@ -1,0 +1,237 @@
+"""FP64 CPU oracle for the flux-reconstruction advection of the DGFS solver.
+
+TEST INFRASTRUCTURE ONLY (used by tests/ as the checker; never imported by
+the product).  It restates the advection right-hand side of
+frfs/solvers/dgfs/system.py:140-177 for structured rectangles of
+quadrilaterals, building every operator from the mathematics in a way that
+differs from the product's code path:
+
+* nodal basis: tensor products of 1D Lagrange polynomials on the Gauss
+  points (the product inverts a Legendre Vandermonde matrix, polys.py);
+* correction lift M3 (shapes.py:107-118, 243-289): the DG lift with the
+  (p+1)-point Gauss-Lobatto face rule, obtained from the diagonal Gauss mass
+  matrix, M3[p][k] = (1/W_p) sum_i w_i l_k(s_i) L_p(face(s_i));
+* geometry: closed-form metric terms of an axis-aligned rectangle
+  (smats = |J| J^-1 = diag(hy/2, hx/2), |J| = hx hy / 4);
+* fluxes: the exact upwind flux of rusanov.mako:47-49, the transformed flux
+  of tflux.mako/flux.mako, the diffuse wall of bcs/dgfs-wall-diffuse*.mako
+  and negdivconf.mako, assembled element by element.
+
+How it is pinned (tests/test_oracle_fr.py): exact interpolation and
+differentiation of polynomials of degree <= p; the lift identity
+sum_p W_p M3[p][k] q(x_p) = face integral of l_k q for q of degree <= p;
+free-stream preservation on periodic meshes; discrete conservation
+(sum of W |J| rhs equals the boundary flux); convergence of smooth periodic
+advection at order p+1.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gauss_legendre(n):
+    return np.polynomial.legendre.leggauss(n)
+
+
+def gauss_lobatto(n):
+    # interior nodes are the roots of P'_{n-1}; weights 2/(n(n-1) P_{n-1}^2)
+    if n == 2:
+        return np.array([-1.0, 1.0]), np.array([1.0, 1.0])
+    c = np.zeros(n)
+    c[-1] = 1.0
+    inner = np.polynomial.legendre.Legendre(c).deriv().roots()
+    z = np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
+    Pn = np.polynomial.legendre.legval(z, c)
+    w = 2.0 / (n * (n - 1) * Pn ** 2)
+    return z, w
+
+
+def lagrange(nodes, x):
+    """L[j](x) for the 1D Lagrange basis on `nodes`; x array -> (len(x), n)."""
+    x = np.atleast_1d(np.asarray(x, dtype=float))
+    n = len(nodes)
+    out = np.ones((len(x), n))
+    for j in range(n):
+        for m in range(n):
+            if m != j:
+                out[:, j] *= (x - nodes[m]) / (nodes[j] - nodes[m])
+    return out
+
+
+def lagrange_deriv(nodes, x):
+    x = np.atleast_1d(np.asarray(x, dtype=float))
+    n = len(nodes)
+    out = np.zeros((len(x), n))
+    for j in range(n):
+        for k in range(n):
+            if k == j:
+                continue
+            term = np.ones(len(x)) / (nodes[j] - nodes[k])
+            for m in range(n):
+                if m != j and m != k:
+                    term *= (x - nodes[m]) / (nodes[j] - nodes[m])
+            out[:, j] += term
+    return out
+
+
+class QuadFR:
+    """Operators of one quadrilateral of order p with Gauss solution and flux
+    points.  Solution points are x-fastest tensor points; flux points run face
+    by face (bottom, right, top, left), ascending in the face coordinate."""
+
+    faces = [(lambda s: (s, -1.0 + 0 * s), (0.0, -1.0)), (lambda s: (1.0 + 0 * s, s), (1.0, 0.0)),
+             (lambda s: (s, 1.0 + 0 * s), (0.0, 1.0)), (lambda s: (-1.0 + 0 * s, s), (-1.0, 0.0))]
+
+    def __init__(self, p):
+        self.p = p
+        n = p + 1
+        self.z, self.w = gauss_legendre(n)
+        self.nupts = n * n
+        self.upts = np.array([(self.z[i], self.z[j]) for j in range(n) for i in range(n)])
+        self.W = np.array([self.w[i] * self.w[j] for j in range(n) for i in range(n)])
+        fp, fn = [], []
+        for proj, nrm in self.faces:
+            xs, ys = proj(self.z)
+            fp.append(np.stack([xs, ys], axis=1))
+            fn.append(np.tile(nrm, (n, 1)))
+        self.fpts = np.vstack(fp)
+        self.fnorm = np.vstack(fn)
+        self.nfpts = len(self.fpts)
+
+    def basis_at(self, pts):
+        """(npts, nupts) nodal basis values."""
+        Lx = lagrange(self.z, pts[:, 0])
+        Ly = lagrange(self.z, pts[:, 1])
+        n = self.p + 1
+        return np.stack([Lx[:, i] * Ly[:, j] for j in range(n) for i in range(n)], axis=1)
+
+    def grad_at(self, pts):
+        """(2, npts, nupts) derivatives along xi, eta."""
+        Lx, Ly = lagrange(self.z, pts[:, 0]), lagrange(self.z, pts[:, 1])
+        dLx, dLy = lagrange_deriv(self.z, pts[:, 0]), lagrange_deriv(self.z, pts[:, 1])
+        n = self.p + 1
+        gx = np.stack([dLx[:, i] * Ly[:, j] for j in range(n) for i in range(n)], axis=1)
+        gy = np.stack([Lx[:, i] * dLy[:, j] for j in range(n) for i in range(n)], axis=1)
+        return np.stack([gx, gy])
+
+    def lift(self):
+        """M3 (nupts, nfpts): DG lift with the (p+1)-point GLL face rule."""
+        zq, wq = gauss_lobatto(self.p + 1)
+        n = self.p + 1
+        M3 = np.zeros((self.nupts, self.nfpts))
+        for f, (proj, _) in enumerate(self.faces):
+            lk = lagrange(self.z, zq)                       # face nodal basis at GLL points
+            xs, ys = proj(zq)
+            Lp = self.basis_at(np.stack([xs, ys], axis=1))  # (nq, nupts)
+            for k in range(n):
+                M3[:, f * n + k] = (wq * lk[:, k]) @ Lp / self.W
+        return M3
+
+
+class RectMesh:
+    """nx x ny rectangles of [x0,x1]x[y0,y1]; element e = ix + nx*iy."""
+
+    def __init__(self, nx, ny, x0, x1, y0, y1, periodic_x, periodic_y=False):
+        self.nx, self.ny = nx, ny
+        self.hx, self.hy = (x1 - x0) / nx, (y1 - y0) / ny
+        self.x0, self.y0 = x0, y0
+        self.periodic_x = periodic_x
+        self.periodic_y = periodic_y
+
+    def neighbour(self, e, f):
+        """(e', f') across face f of element e, or None on a boundary."""
+        ix, iy = e % self.nx, e // self.nx
+        if f == 1:
+            if ix + 1 < self.nx:
+                return e + 1, 3
+            return (e - ix, 3) if self.periodic_x else None
+        if f == 3:
+            if ix > 0:
+                return e - 1, 1
+            return (e + self.nx - 1, 1) if self.periodic_x else None
+        if f == 2:
+            if iy + 1 < self.ny:
+                return e + self.nx, 0
+            return (ix, 0) if self.periodic_y else None
+        if iy > 0:
+            return e - self.nx, 2
+        return (ix + self.nx * (self.ny - 1), 2) if self.periodic_y else None
+
+    def upts_xy(self, upts):
+        """Physical coordinates (nupts, neles, 2) of reference points."""
+        out = np.empty((len(upts), self.nx * self.ny, 2))
+        for e in range(self.nx * self.ny):
+            ix, iy = e % self.nx, e // self.nx
+            out[:, e, 0] = self.x0 + (ix + (upts[:, 0] + 1) / 2) * self.hx
+            out[:, e, 1] = self.y0 + (iy + (upts[:, 1] + 1) / 2) * self.hy
+        return out
+
+    def boundary_name(self, e, f):
+        return {0: "bottom", 1: "right", 2: "top", 3: "left"}[f]
+
+
+def maxwellian(cv, rho, u, T):
+    u = np.asarray(u, dtype=float).reshape(3, 1)
+    return rho / (np.pi * T) ** 1.5 * np.exp(-np.sum((cv - u) ** 2, axis=0) / T)
+
+
+class AdvectionOracle:
+    """FP64 advection rhs of a DGFS state f[upt][v][e] on a RectMesh.
+
+    walls: {name: (U (3,), f_w (vsize,))} diffuse walls; faces without a
+    neighbour must be listed."""
+
+    def __init__(self, p, mesh: RectMesh, cv, cw, walls=None):
+        self.fr = fr = QuadFR(p)
+        self.mesh = mesh
+        self.cv = cv
+        self.cw = cw
+        self.walls = walls or {}
+        self.B = fr.basis_at(fr.fpts)            # M0
+        self.D = fr.grad_at(fr.upts)             # (2, nupts, nupts)
+        self.M3 = fr.lift()
+        self.sm = np.array([mesh.hy / 2, mesh.hx / 2])   # diagonal smats
+        self.djac = mesh.hx * mesh.hy / 4
+
+    def rhs(self, f):
+        fr, mesh, cv = self.fr, self.mesh, self.cv
+        nupts, vsize, neles = f.shape
+        n = fr.p + 1
+        uf = np.einsum("fu,uve->fve", self.B, f)                  # flux-point states
+        # transformed flux divergence at solution points
+        div = np.zeros_like(f)
+        for d in range(2):
+            Fd = self.sm[d] * cv[d][None, :, None] * f
+            div += np.einsum("pu,uve->pve", self.D[d], Fd)
+        # interior normal transformed flux at flux points (reference normals)
+        Fn_int = np.zeros_like(uf)
+        for d in range(2):
+            Fn_int += fr.fnorm[:, d][:, None, None] * self.sm[d] * cv[d][None, :, None] * uf
+        # common flux
+        Fn_com = np.zeros_like(uf)
+        for e in range(neles):
+            for fc in range(4):
+                rows = np.arange(fc * n, (fc + 1) * n)
+                nref = np.array(fr.faces[fc][1])
+                pn = nref * self.sm                                   # physical normal * mag
+                mag = np.linalg.norm(pn)
+                nrm = pn / mag
+                nc = nrm[0] * cv[0] + nrm[1] * cv[1]
+                nb = mesh.neighbour(e, fc)
+                ul = uf[rows, :, e]
+                if nb is not None:
+                    e2, f2 = nb
+                    # faces are parametrised bottom-to-top / left-to-right on
+                    # both sides, so neighbouring flux points share the order
+                    rows2 = np.arange(f2 * n, (f2 + 1) * n)
+                    ur = uf[rows2, :, e2]
+                else:
+                    U, fw = self.walls[mesh.boundary_name(e, fc)]
+                    fac = nrm[0] * (cv[0] - U[0]) + nrm[1] * (cv[1] - U[1])
+                    num = np.sum(np.where(fac >= 0, fac * ul * self.cw, 0.0), axis=1)
+                    den = np.sum(np.where(fac < 0, fac * fw * self.cw, 0.0))
+                    ur = np.where(fac[None, :] < 0, fw[None, :] * (-num[:, None] / den), ul)
+                fn = 0.5 * nc * (ul + ur) + 0.5 * np.abs(nc) * (ul - ur)
+                Fn_com[rows, :, e] = mag * fn
+        corr = np.einsum("pf,fve->pve", self.M3, Fn_com - Fn_int)
+        return -(div + corr) / self.djac

@@ -1,0 +1,175 @@
+"""GPU parity of the FR advection kernels and of whole DGFS steps against the
+FP64 oracles (oracle/fr.py for advection, oracle/vhs.py for Q).  FP64 bar:
+1e-12 relative for pure advection, 1e-11 for steps that include Q."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fr as OF
+from oracle import vhs as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("m,k,ncol,beta", [(12, 9, 1000, 0.0), (9, 18, 4096, 0.0), (16, 32, 777, 1.0),
+                                           (9, 12, 333, 1.0), (6, 12, 64, 0.5), (40, 20, 500, 0.0)])
+def test_mul_mfma(m, k, ncol, beta, device):
+    import torch
+    from dgfs_hip import fr
+    say("mul", m, k, ncol, beta)
+    rng = np.random.default_rng(m * k)
+    A = rng.normal(size=(m, k))
+    ldb = ncol + 16
+    B = rng.normal(size=(k, ldb))
+    C0 = rng.normal(size=(m, ldb))
+    tA = torch.as_tensor(A, device=device)
+    tB = torch.as_tensor(B, device=device)
+    tC = torch.as_tensor(C0.copy(), device=device)
+    fr.mul(tA, m, k, k, tB, ldb, tC, ldb, ncol, 1.5, beta, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    C = tC.cpu().numpy()
+    want = 1.5 * A @ B[:, :ncol] + beta * C0[:, :ncol]
+    assert rel(C[:, :ncol], want) < 1e-14
+    assert np.array_equal(C[:, ncol:], C0[:, ncol:])
+
+
+def build_system(order=2, nx=3, ny=2, periodic_x=True, periodic_y=False, Nv=16, L=5.0, NvBS=None,
+                 uwall=0.3, nregs=3):
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import couette_cfg
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    cfg = couette_cfg(Nv=Nv, L=L, Nrho=4, M=6, Kn=1.0, omega=0.5, order=order, uwall=uwall,
+                      NvBatchSize=NvBS)
+    be = HIPBackend(cfg)
+    mesh = rect_quads(nx, ny, 0.0, 0.5 * nx, 0.0, 0.5 * ny, periodic_x=periodic_x, periodic_y=periodic_y)
+    return be, DGFSSystem(be, mesh, cfg, nregs), mesh
+
+
+def test_system_rhs_vs_oracle(device):
+    be, sysm, mesh = build_system()
+    vm = sysm.vm
+    ov = O.VelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M())
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    walls = {"bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+             "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))}
+    orc = OF.AdvectionOracle(2, omesh, ov.cv, ov.cw, walls)
+    rng = np.random.default_rng(3)
+    f = OF.maxwellian(ov.cv, 1.0, [0.1, -0.05, 0], 0.95)[None, :, None] * \
+        (1 + 0.1 * rng.random((9, 1, 6))) * (1 + 0.05 * rng.random((1, ov.n, 1)))
+    sysm.ele.scal_upts[0].set(f)
+    say("system rhs p=2 Nv=16")
+    sysm.updatebc(0.0, 0, 1, 0, 0)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    r = sysm.ele.scal_upts[1].get()
+    want = orc.rhs(f)
+    err = rel(r, want)
+    say("  rel L2", err)
+    assert err < 1e-12
+
+
+def test_free_stream_on_gpu(device):
+    be, sysm, _ = build_system(periodic_y=True)
+    vm = sysm.vm
+    m = OF.maxwellian(vm.cv(), 1.0, [0.2, 0.1, 0], 1.0)
+    f = np.broadcast_to(m[None, :, None], (9, vm.vsize(), 6)).copy()
+    sysm.ele.scal_upts[0].set(f)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    assert np.abs(sysm.ele.scal_upts[1].get()).max() < 1e-12
+
+
+@pytest.mark.parametrize("stepper,nbs", [("dgfs-euler", None), ("dgfs-tvd-rk2", None), ("dgfs-tvd-rk2", 1024)])
+def test_step_vs_oracle(stepper, nbs, device):
+    from dgfs_hip.integrators.steppers import get_stepper
+    nregs = 2 if stepper == "dgfs-euler" else 3
+    be, sysm, mesh = build_system(NvBS=nbs, nregs=nregs)
+    vm = sysm.vm
+    ov = O.VelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M())
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    walls = {"bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+             "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))}
+    orc = OF.AdvectionOracle(2, omesh, ov.cv, ov.cw, walls)
+    Qo = O.VHSOracle(ov, 1.0, sysm.sm.prefactor)
+
+    def Q(f):
+        fl = f.transpose(2, 0, 1).reshape(-1, ov.n)
+        return Qo.q(fl).reshape(6, 9, ov.n).transpose(1, 2, 0)
+
+    rng = np.random.default_rng(4)
+    f0 = OF.maxwellian(ov.cv, 1.0, [0.0, 0.0, 0], 1.0)[None, :, None] * (1 + 0.1 * rng.random((9, 1, 6)))
+    sysm.eles_scal_upts_full[0].set(f0)
+    dt = 0.01
+    st = get_stepper(stepper, be, sysm, dt)
+    say("step", stepper, "NvBatchSize", vm.NvBatchSize())
+    st.advance_to(2 * dt)
+    be.synchronize()
+    got = sysm.eles_scal_upts_full[st.soln_bank].get()
+    f = f0
+    for _ in range(2):
+        if stepper == "dgfs-euler":
+            f = f + dt * (orc.rhs(f) + Q(f))
+        else:
+            f1 = f + dt * (orc.rhs(f) + Q(f))
+            f = 0.5 * f + 0.5 * f1 + 0.5 * dt * (orc.rhs(f1) + Q(f1))
+    err = rel(got, f)
+    say("  rel L2", err)
+    assert err < 1e-11
+
+
+def test_specular_and_outlet_walls(device):
+    """A box with specular walls keeps a state that is even in c_y at rest
+    (the reflected ghost equals the state), and an outlet flux equals the
+    one-sided upwind flux."""
+    import torch
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import base_cfg, maxwellian_section
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    cfg = base_cfg(16, 5.0, 4, 6, Kn=1.0, order=2)
+    maxwellian_section(cfg, "soln-ics", 1.0, 1.0, (0.2, 0, 0), "maxwellian")
+    cfg.set("soln-bcs-bottom", "type", "dgfs-wall-specular")
+    cfg.set("soln-bcs-top", "type", "dgfs-wall-specular")
+    be = HIPBackend(cfg)
+    mesh = rect_quads(3, 2, 0, 1.5, 0, 1, periodic_x=True)
+    sysm = DGFSSystem(be, mesh, cfg, 2)
+    say("specular walls")
+    sysm.ele.scal_upts[0].set(sysm.eles_scal_upts_full[0].get())
+    sysm.updatebc(0.0, 0, 1, 0, 0)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    assert np.abs(sysm.ele.scal_upts[1].get()).max() < 1e-12
+    del torch
+
+
+def test_moments_kernel(device):
+    be, sysm, _ = build_system()
+    vm = sysm.vm
+    rng = np.random.default_rng(8)
+    f = OF.maxwellian(vm.cv(), 1.0, [0.2, -0.1, 0.05], 0.9)[None, :, None] * (1 + 0.1 * rng.random((9, 1, 6)))
+    sysm.eles_scal_upts_full[0].set(f)
+    mom = sysm.moments(0)
+    cv, cw = vm.cv(), vm.cw()
+    rho = f.sum(1) * cw
+    U = np.stack([(f * cv[d][None, :, None]).sum(1) * cw / rho for d in range(3)])
+    c = cv[:, None, :, None] - U[:, :, None, :]
+    c2 = (c ** 2).sum(0)
+    T = (f * c2).sum(1) * cw * 2 / 3 / rho
+    assert np.allclose(mom[:, 0], rho, rtol=1e-13)
+    assert np.allclose(mom[:, 1:4], U.transpose(1, 0, 2), rtol=1e-12, atol=1e-14)
+    assert np.allclose(mom[:, 4], T, rtol=1e-12)
+    qx = (f * c[0] * c2).sum(1) * cw
+    assert np.allclose(mom[:, 5], qx, rtol=1e-10, atol=1e-13)
+    assert np.allclose(mom[:, 14], rho * T, rtol=1e-12)

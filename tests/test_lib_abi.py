@@ -49,3 +49,27 @@ def test_error_path_without_gpu_work():
         _lib.check(st, "dgfs_vhs_create")
     assert lib.dgfs_vhs_destroy(None) == 5
     assert lib.dgfs_version().startswith(b"dgfs_hip")
+
+
+def test_fr_entry_points_validate_on_host():
+    """Every FR entry point converts its arguments and rejects null operands
+    on the host, before any device work (no GPU needed)."""
+    from dgfs_hip import _lib
+    _lib.load()
+    from dgfs_hip import fr
+    N = None
+    cases = [
+        lambda: fr.mul(0, 4, 4, 4, 0, 8, 0, 8, 8, 1.0, 0.0),
+        lambda: fr.tflux(2, 4, 4, 4, 0, 0, 128, 0, 128, 0, 128, 0, 8),
+        lambda: fr.negdivconf(4, 4, 4, 0, 128, 0, 4),
+        lambda: fr.intcflux(2, 4, 4, 0, 0, 0, 0, 0, 0, 0, 0, 8),
+        lambda: fr.bccflux(0, 2, 4, 4, 0, 0, 0, 0, 0, N, N, N, (0, 0, 0), 0, 8),
+        lambda: fr.bccupdate(0, 2, 4, 4, 0, 0, 0, 0, N, 0, (0, 0, 0), 1.0, 0, 8),
+        lambda: fr.copy_reg(4, 4, 0, 128, 4, 0, 0, 128, 4, 0, 4),
+        lambda: fr.moments(4, 4, 8, 0, 256, 0, 1.0, 0),
+    ]
+    for c in cases:
+        with pytest.raises(_lib.DGFSInvalidValue):
+            c()
+    with pytest.raises(_lib.DGFSInvalidValue):
+        _lib.call("frfs_axnpby", 10, 0, None, None, None)
