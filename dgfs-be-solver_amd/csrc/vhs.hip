@@ -32,6 +32,7 @@
 // L2 / Infinity Cache instead of streaming to HBM.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -121,6 +122,7 @@ template <typename T>
 struct PrepArgs {
   const double* f;  // [npts][n] contiguous
   cpx<T>* F;        // [npts][n] natural spectral order, scaled by 1/n
+  cpx<T>* Ft;       // optional copy transposed to [kx][kz][ky] (register-resident gain)
   cpx<T>* Z;        // scratch [nslots][n]
   int npts;
 };
@@ -134,6 +136,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_prep(PrepArgs<T> a) {
   for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
     const double* f = a.f + static_cast<size_t>(pt) * n;
     cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    cpx<T>* Ft = a.Ft ? a.Ft + static_cast<size_t>(pt) * n : nullptr;
     pencil_pass<T, N, NT, -1>(
         [&](int x, int idx) { return cpx<T>{static_cast<T>(f[x * NN + idx]), T(0)}; },
         [&](int idx, cpx<T>* v) {
@@ -144,7 +147,10 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_prep(PrepArgs<T> a) {
       const int cnt = min(PB, N - kx0);
       slab_batch<T, N, NT, -1>(
           lds, cnt, [&](int j, int r, int c) { return Z[(kx0 + j) * NN + r * N + c]; },
-          [&](int j, int r, int c, cpx<T> v) { F[(kx0 + j) * NN + r * N + c] = rn * v; });
+          [&](int j, int r, int c, cpx<T> v) {
+            F[(kx0 + j) * NN + r * N + c] = rn * v;
+            if (Ft) Ft[(kx0 + j) * NN + c * N + r] = rn * v;
+          });
     }
   }
 }
@@ -264,6 +270,344 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_gain(GainArgs<T> a) {
               G[o] = G[o] + (wp * b1p[o]) * v;
             });
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Nyquist-plane corrections for every (point, p, q): W = IFFT(B) on the three
+// planes kx = N/2, ky = N/2 (kx != N/2), kz = N/2 (kx, ky != N/2), written to
+// Wbuf[pt][p*M + q][3][N][N].  Consumed by the register-resident gain kernel.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NyqArgs {
+  const cpx<T>* F;      // [npts][n]
+  cpx<T>* W;            // [npts][K][3][N][N]
+  const double* alpha;  // [Nrho]
+  const double* sz;     // [M][3]
+  int npts, Nrho, M;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_nyq(NyqArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  __shared__ cpx<T> eph[3][N];
+  const int tid = threadIdx.x;
+  const int K = a.Nrho * a.M;
+  const long nitems = static_cast<long>(a.npts) * K;
+  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int pt = static_cast<int>(item / K), pq = static_cast<int>(item - static_cast<long>(pt) * K);
+    const int p = pq / a.M, q = pq - p * a.M;
+    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    cpx<T>* W = a.W + static_cast<size_t>(item) * 3 * NN;
+    __syncthreads();
+    if (tid < 3 * N) {
+      const int d = tid / N, k = tid - d * N;
+      const double l = (k < h) ? static_cast<double>(k) : static_cast<double>(k - N);
+      double s, c;
+      sincos(a.alpha[p] * a.sz[q * 3 + d] * l, &s, &c);
+      eph[d][k] = cpx<T>{static_cast<T>(c), static_cast<T>(s)};
+    }
+    __syncthreads();
+    slab_batch<T, N, NT, +1>(
+        lds, 3,
+        [&](int j, int r, int c) -> cpx<T> {
+          int kx, ky, kz;
+          if (j == 0) {
+            kx = h; ky = r; kz = c;
+          } else if (j == 1) {
+            kx = r; ky = h; kz = c;
+            if (kx == h) return cpx<T>{T(0), T(0)};
+          } else {
+            kx = r; ky = c; kz = h;
+            if (kx == h || ky == h) return cpx<T>{T(0), T(0)};
+          }
+          const cpx<T> ex = eph[0][kx], ey = eph[1][ky], ez = eph[2][kz];
+          const cpx<T> E = ex * ey * ez;
+          const cpx<T> Et = (kx == h ? conj(ex) : ex) * (ky == h ? conj(ey) : ey) * (kz == h ? conj(ez) : ez);
+          const cpx<T> D = E - Et;
+          return T(0.5) * (F[kx * NN + ky * N + kz] * (conj(D) - mul_i(D)));
+        },
+        [&](int j, int r, int c, cpx<T> v) { W[j * NN + r * N + c] = v; });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident gain kernel.  A workgroup of N*N/2 threads holds one whole
+// N^3 complex field in registers, two pencils per thread; thread t belongs
+// to plane group g = t / N (N threads, one kx plane per slot) and lane
+// l = t % N.  Per (p, q):
+//   load   F plane kx = g + (N/2) j (slot j) through LDS, z-pencil ky = l
+//   z-IFFT, in-plane LDS transpose, y-IFFT       -> y-pencil (kx, z = l)
+//   two LDS rounds (kx halves) exchange kx <-> y -> x-pencil (y = g + (N/2) j, z = l)
+//   x-IFFT, Re(P+ P-) with the precomputed Nyquist planes, accumulate S_p
+// S_p stays in registers across q; at the end of p it is forward transformed
+// by the reverse sequence and accumulated into G^ with weight w_p b1_p.
+// LDS planes are swizzled (column c stored at c ^ r) so row and column
+// accesses are both bank-conflict free.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GainRRArgs {
+  const cpx<T>* Ft;     // [npts][n] spectrum transposed to [kx][kz][ky]
+  const cpx<T>* W;      // [npts][K][3][N][N]
+  cpx<T>* Ghat;         // [npts*ngroups][n]
+  const T* b1;          // [Nrho][n]
+  const double* alpha;  // [Nrho]
+  const double* wgt;    // [Nrho]
+  const double* sz;     // [M][3]
+  int nitems, ngroups, ppg, Nrho, M;
+};
+
+template <int N>
+struct RRShape {
+  static constexpr int NT = N * N / 2;
+  static constexpr int G = N / 2;  // plane groups
+};
+
+// stage helpers of the register-resident kernel (force-inlined; scalars by
+// value, register arrays by reference so they stay in VGPRs).  LDS layout:
+//   Xb : N/4 padded N x (N+1) planes -- the in-plane transposes of half the
+//        plane groups at a time, and the N/2 x N x N/2... exchange buffer
+//   S0 : the slot-0 accumulators, S0[x][thread]
+//   Wl : the three Nyquist planes of the current (p, q)
+// Every LDS access is a per-thread base plus a compile-time offset; padded
+// rows make row and column accesses bank-conflict free.
+template <int N>
+struct RRLds {
+  static constexpr int plane = N * (N + 1);           // complex
+  static constexpr int xbuf = (N / 4) * plane;        // complex; >= (N/2)*N*(N/2)/... checked below
+  static constexpr int ex = (N / 2) * N * (N / 2);    // exchange sub-round: G planes x N z x G y... complex
+  static constexpr int xsize = (xbuf > ex) ? xbuf : ex;
+  static constexpr int wl = 3 * N * N;                // complex
+  static constexpr int s0 = N * N * N / 2;            // real (one per thread and x)
+};
+
+// inverse z then y transform of plane kx for the thread (group g, lane l);
+// the in-plane transpose runs for plane groups [half*G/2, (half+1)*G/2)
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Ft, cpx<T>* Xb,
+                                              const cpx<T> (*eph)[N], int kx, int g, int l) {
+  constexpr int NP = N + 1, NN = N * N, GH = G / 2;
+  // z-pencil (kx, ky = l) from the transposed spectrum Ft[kx][kz][ky]
+  const cpx<T>* Fp = Ft + kx * NN + l;
+  const cpx<T> exy = eph[0][kx] * eph[1][l];
+#pragma unroll
+  for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+    for (int kz = k0; kz < k0 + 8; ++kz) {
+      const cpx<T> E = exy * eph[2][kz];
+      v[kz] = Fp[kz * N] * (E + mul_i(conj(E)));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  FFT<T, N, +1>::run(v);
+  __builtin_amdgcn_sched_barrier(0);
+  const int slot = g % GH;
+  cpx<T>* Pc = Xb + slot * (N * NP) + l;
+  cpx<T>* Pr = Xb + slot * (N * NP) + l * NP;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const bool mine = (g / GH) == half;
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int z = 0; z < N; ++z) Pr[z] = v[z];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+        for (int ky = k0; ky < k0 + 8; ++ky) v[ky] = Pc[ky * NP];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  FFT<T, N, +1>::run(v);
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// exchange for target slot r (y = g + G r): two sub-rounds move the y-half r
+// of plane slot 0 (kx = g) then of plane slot 1 (kx = g + G); then the x-IFFT
+// and Re(P+ P-) with the Nyquist corrections, returned in val[x]
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_x_inverse(T (&val)[N], const cpx<T> (&v0)[N], const cpx<T> (&v1)[N],
+                                             cpx<T>* Xb, const cpx<T>* Wl, int r, int g, int l) {
+  constexpr int NN = N * N;
+  cpx<T> u[N];
+  cpx<T>* Xw = Xb + (g * G) * N + l;  // [kxl][yy][z]
+  const cpx<T>* Xr = Xb + g * N + l;   // read (kxl, yy = g, z = l)
+#pragma unroll
+  for (int sr = 0; sr < 2; ++sr) {
+    const cpx<T> (&v)[N] = sr ? v1 : v0;
+    __syncthreads();
+#pragma unroll
+    for (int yy = 0; yy < G; ++yy) Xw[yy * N] = v[G * r + yy];
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < G; k0 += 8) {
+#pragma unroll
+      for (int kk = k0; kk < k0 + 8; ++kk) u[G * sr + kk] = Xr[kk * G * N];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  FFT<T, N, +1>::run(u);
+  __builtin_amdgcn_sched_barrier(0);
+  const int y = g + G * r, z = l;
+  const cpx<T> w1 = Wl[y * N + z];
+  const cpx<T>* W2 = Wl + NN + z;
+  const cpx<T>* W3 = Wl + 2 * NN + y;
+  const T sy = (y & 1) ? T(-1) : T(1), szs = (z & 1) ? T(-1) : T(1);
+#pragma unroll
+  for (int x0 = 0; x0 < N; x0 += 8) {
+#pragma unroll
+    for (int x = x0; x < x0 + 8; ++x) {
+      const T sx = (x & 1) ? T(-1) : T(1);
+      const cpx<T> w2 = W2[x * N], w3 = W3[x * N];
+      const T wr = sx * w1.x + sy * w2.x + szs * w3.x;
+      const T wi = sx * w1.y + sy * w2.y + szs * w3.y;
+      val[x] = (u[x].x + wi) * (u[x].y - wr) - wr * wi;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// x-FFT of the slot-r accumulator and the reverse exchange into the y-half r
+// of both plane slots (two sub-rounds)
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_x_forward(const T (&sacc)[N], cpx<T> (&v0)[N], cpx<T> (&v1)[N], cpx<T>* Xb,
+                                             int r, int g, int l) {
+  cpx<T> u[N];
+#pragma unroll
+  for (int x = 0; x < N; ++x) u[x] = cpx<T>{sacc[x], T(0)};
+  FFT<T, N, -1>::run(u);
+  __builtin_amdgcn_sched_barrier(0);
+  cpx<T>* Xw = Xb + g * N + l;         // write (kxl, yy = g, z = l)
+  const cpx<T>* Xr = Xb + (g * G) * N + l;
+#pragma unroll
+  for (int sr = 0; sr < 2; ++sr) {
+    cpx<T> (&v)[N] = sr ? v1 : v0;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) Xw[kk * G * N] = u[G * sr + kk];
+    __syncthreads();
+#pragma unroll
+    for (int y0 = 0; y0 < G; y0 += 8) {
+#pragma unroll
+      for (int yy = y0; yy < y0 + 8; ++yy) v[G * r + yy] = Xr[yy * N];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// forward y then z transform of plane kx, weighted accumulation into G^
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx, int g, int l, T wp,
+                                              const T* __restrict__ b1p, cpx<T>* Gh, bool first) {
+  constexpr int NN = N * N, NP = N + 1, GH = G / 2;
+  FFT<T, N, -1>::run(v);
+  __builtin_amdgcn_sched_barrier(0);
+  const int slot = g % GH;
+  cpx<T>* Pc = Xb + slot * (N * NP) + l;
+  cpx<T>* Pr = Xb + slot * (N * NP) + l * NP;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const bool mine = (g / GH) == half;
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int ky = 0; ky < N; ++ky) Pc[ky * NP] = v[ky];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+        for (int z = k0; z < k0 + 8; ++z) v[z] = Pr[z];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  FFT<T, N, -1>::run(v);
+  __builtin_amdgcn_sched_barrier(0);
+  const size_t o = static_cast<size_t>(kx) * NN + l * N;
+  const T* b = b1p + o;
+  cpx<T>* gp = Gh + o;
+#pragma unroll
+  for (int k0 = 0; k0 < N; k0 += 8) {
+    if (first) {
+#pragma unroll
+      for (int kz = k0; kz < k0 + 8; ++kz) gp[kz] = (wp * b[kz]) * v[kz];
+    } else {
+#pragma unroll
+      for (int kz = k0; kz < k0 + 8; ++kz) gp[kz] = gp[kz] + (wp * b[kz]) * v[kz];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a) {
+  constexpr int NT = RRShape<N>::NT, G = RRShape<N>::G, NN = N * N, n = N * N * N, h = N / 2;
+  extern __shared__ char smem_raw[];
+  cpx<T>* Xb = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* Wl = Xb + RRLds<N>::xsize;
+  T* S0 = reinterpret_cast<T*>(Wl + RRLds<N>::wl) + threadIdx.x;  // S0[x * NT]
+  __shared__ cpx<T> eph[3][N];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  const int K = a.Nrho * a.M;
+
+  cpx<T> v0[N], v1[N];
+  T s1[N];
+
+  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const int pt = item / a.ngroups, grp = item - pt * a.ngroups;
+    const int p0 = grp * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
+    const cpx<T>* Ft = a.Ft + static_cast<size_t>(pt) * n;
+    cpx<T>* Gh = a.Ghat + static_cast<size_t>(item) * n;
+    for (int p = p0; p < p1; ++p) {
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        s1[x] = T(0);
+        S0[x * NT] = T(0);
+      }
+      const double alpha = a.alpha[p];
+      for (int q = 0; q < a.M; ++q) {
+        __syncthreads();
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double lv = (k < h) ? static_cast<double>(k) : static_cast<double>(k - N);
+          double sn, cs;
+          sincos(alpha * a.sz[q * 3 + d] * lv, &sn, &cs);
+          eph[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
+        }
+        // this (p, q)'s Nyquist planes into LDS (read after later barriers)
+        const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
+        for (int i = tid; i < 3 * NN; i += NT) Wl[i] = Wq[i];
+        __syncthreads();
+        rr_zy_inverse<T, N, G>(v0, Ft, Xb, eph, g, g, l);
+        rr_zy_inverse<T, N, G>(v1, Ft, Xb, eph, g + G, g, l);
+        T val[N];
+        rr_x_inverse<T, N, G>(val, v0, v1, Xb, Wl, 0, g, l);
+#pragma unroll
+        for (int x = 0; x < N; ++x) S0[x * NT] += val[x];
+        rr_x_inverse<T, N, G>(val, v0, v1, Xb, Wl, 1, g, l);
+#pragma unroll
+        for (int x = 0; x < N; ++x) s1[x] += val[x];
+      }
+      {
+        T s0r[N];
+#pragma unroll
+        for (int x = 0; x < N; ++x) s0r[x] = S0[x * NT];
+        rr_x_forward<T, N, G>(s0r, v0, v1, Xb, 0, g, l);
+      }
+      rr_x_forward<T, N, G>(s1, v0, v1, Xb, 1, g, l);
+      const T wp = static_cast<T>(a.wgt[p]);
+      const T* b1p = a.b1 + static_cast<size_t>(p) * n;
+      rr_yz_forward<T, N, G>(v0, Xb, g, g, l, wp, b1p, Gh, p == p0);
+      rr_yz_forward<T, N, G>(v1, Xb, g + G, g, l, wp, b1p, Gh, p == p0);
     }
   }
 }
@@ -432,14 +776,20 @@ struct VhsHandle {
   void* S = nullptr;
   void* W = nullptr;
   int groups_override = 0;
+  int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident
+  int last_algorithm = 0;
+  void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
+  void* Ft = nullptr;    // spectrum transposed to [kx][kz][ky] [cap_pts][n]
+  size_t cap_ft = 0;
+  size_t cap_wny = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
   // optional per-kernel timing: event pairs recorded around every launch on
   // the launch stream, resolved (and synchronised) only when queried
   int timing = 0;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-  double acc_ms[4] = {0, 0, 0, 0};
-  int acc_n[4] = {0, 0, 0, 0};
+  double acc_ms[5] = {0, 0, 0, 0, 0};
+  int acc_n[5] = {0, 0, 0, 0, 0};
 };
 
 static hipEvent_t take_event(VhsHandle* h) {
@@ -453,7 +803,8 @@ static hipEvent_t take_event(VhsHandle* h) {
   return e;
 }
 
-// record a start event for kernel class k (0 prep, 1 gain, 2 finalize, 3 project)
+// record a start event for kernel class k (0 prep, 1 gain, 2 finalize, 3 project,
+// 4 Nyquist planes)
 struct KTimer {
   VhsHandle* h;
   int k;
@@ -490,6 +841,12 @@ static void resolve_timing(VhsHandle* h) {
 }
 
 static void free_ws(VhsHandle* h) {
+  (void)hipFree(h->Wny);
+  h->Wny = nullptr;
+  h->cap_wny = 0;
+  (void)hipFree(h->Ft);
+  h->Ft = nullptr;
+  h->cap_ft = 0;
   (void)hipFree(h->fc); (void)hipFree(h->Qc); (void)hipFree(h->F); (void)hipFree(h->Ghat);
   (void)hipFree(h->Z); (void)hipFree(h->S); (void)hipFree(h->W);
   h->fc = h->Qc = nullptr; h->F = h->Ghat = h->Z = h->S = h->W = nullptr;
@@ -537,10 +894,64 @@ static int occupancy_slots() {
   return occ * cu_count();
 }
 
+// the register-resident gain kernel exists for FP32 at power-of-two N <= 32
+template <typename T, int N>
+constexpr bool rr_available() {
+  return std::is_same<T, float>::value && (N == 16 || N == 32);
+}
+
+static bool use_rr(const VhsHandle* h) {
+  const bool avail = (h->prec == 32) && (h->N == 16 || h->N == 32);
+  if (h->algorithm == 1) return false;
+  if (h->algorithm == 2) return avail;
+  return avail;
+}
+
+template <typename T, int N>
+static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items, hipStream_t st) {
+  if constexpr (rr_available<T, N>()) {
+    constexpr int NN = N * N;
+    const int K = h->Nrho * h->M;
+    const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
+    if (wbytes > h->cap_wny) {
+      (void)hipFree(h->Wny);
+      h->Wny = nullptr;
+      DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
+      h->cap_wny = wbytes;
+    }
+    NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Wny), h->alpha, h->szd, npts,
+                  h->Nrho, h->M};
+    const long nyq_items = static_cast<long>(npts) * K;
+    KTimer t4(h, 4, st);
+    hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(nyq_items, 8L * cu_count()))),
+                       dim3(Shape<N>::NT), 0, st, na);
+    DGFS_LAUNCH_CHECK("vhs_nyq");
+    t4.stop();
+    KTimer t1(h, 1, st);
+    const size_t lds = (static_cast<size_t>(RRLds<N>::xsize) + RRLds<N>::wl) * sizeof(cpx<T>) +
+                       static_cast<size_t>(RRLds<N>::s0) * sizeof(T);
+    DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vhs_gain_rr<T, N>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cu_count())));
+    GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
+                     static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
+                     static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+    hipLaunchKernelGGL((k_vhs_gain_rr<T, N>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
+    DGFS_LAUNCH_CHECK("vhs_gain_rr");
+    t1.stop();
+    h->threads = RRShape<N>::NT;
+    h->last_slots = grid;
+    return DGFS_OK;
+  } else {
+    (void)h; (void)npts; (void)groups; (void)ppg; (void)items; (void)st;
+    return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel not built for this precision/N");
+  }
+}
+
 template <typename T, int N>
 static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipStream_t st) {
   constexpr int n = N * N * N;
-  const int slots = h->cap_slots;
+  const int slots = use_rr(h) ? cu_count() : h->cap_slots;
   int groups = h->groups_override;
   if (groups <= 0) {
     groups = std::max(1, std::min(h->Nrho, (2 * slots + npts - 1) / std::max(npts, 1)));
@@ -558,7 +969,17 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   h->last_slots = slots;
   h->threads = Shape<N>::NT;
 
-  PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Z), npts};
+  if (use_rr(h)) {
+    const size_t fbytes = static_cast<size_t>(npts) * n * sizeof(cpx<T>);
+    if (fbytes > h->cap_ft) {
+      (void)hipFree(h->Ft);
+      h->Ft = nullptr;
+      DGFS_HIP_TRY(hipMalloc(&h->Ft, fbytes));
+      h->cap_ft = fbytes;
+    }
+  }
+  PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), use_rr(h) ? static_cast<cpx<T>*>(h->Ft) : nullptr,
+                 static_cast<cpx<T>*>(h->Z), npts};
   KTimer t0(h, 0, st);
   hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, pa);
   DGFS_LAUNCH_CHECK("vhs_prep");
@@ -568,11 +989,18 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
                  static_cast<cpx<T>*>(h->Z), static_cast<T*>(h->S), static_cast<cpx<T>*>(h->W),
                  static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
                  static_cast<int>(items), groups, ppg, h->Nrho, h->M};
-  KTimer t1(h, 1, st);
-  hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
-                     dim3(Shape<N>::NT), 0, st, ga);
-  DGFS_LAUNCH_CHECK("vhs_gain");
-  t1.stop();
+  if (use_rr(h)) {
+    h->last_algorithm = 2;
+    const int rc = run_gain_rr<T, N>(h, npts, groups, ppg, items, st);
+    if (rc) return rc;
+  } else {
+    h->last_algorithm = 1;
+    KTimer t1(h, 1, st);
+    hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
+                       dim3(Shape<N>::NT), 0, st, ga);
+    DGFS_LAUNCH_CHECK("vhs_gain");
+    t1.stop();
+  }
 
   FinArgs<T> fa{static_cast<const cpx<T>*>(h->Ghat), static_cast<const cpx<T>*>(h->F),
                 static_cast<const T*>(h->b2), fc, Qc, static_cast<cpx<T>*>(h->Z),
@@ -619,7 +1047,8 @@ static int ensure_ws(VhsHandle* h, int pts) {
 // points per launch: F + G^ (worst case one partial per radial node) + f + Q
 // kept under 48 GiB of the 288 GB of HBM; launches are balanced in size
 static int chunk_points(const VhsHandle* h, int total) {
-  const double per = static_cast<double>(h->n) * (2.0 * h->tsize * (1 + h->Nrho) + 16.0);
+  double per = static_cast<double>(h->n) * (2.0 * h->tsize * (1 + h->Nrho) + 16.0);
+  if (use_rr(h)) per += 3.0 * h->N * h->N * h->Nrho * h->M * 2.0 * h->tsize + 2.0 * h->tsize * h->n;
   const double budget = 48.0 * 1024.0 * 1024.0 * 1024.0;
   const int cap = std::max(1, static_cast<int>(budget / per));
   if (total <= cap) return std::max(total, 1);
@@ -841,7 +1270,7 @@ extern "C" int dgfs_vhs_set_timing(void* p, int on) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
   resolve_timing(h);
   h->timing = on ? 1 : 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     h->acc_ms[k] = 0.0;
     h->acc_n[k] = 0;
   }
@@ -853,9 +1282,27 @@ extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
   if (!ms || !counts) return fail(DGFS_ERR_INVALID_VALUE, "null output arrays");
   resolve_timing(h);
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     ms[k] = h->acc_ms[k];
     counts[k] = h->acc_n[k];
   }
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_set_algorithm(void* p, int algorithm) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (algorithm < 0 || algorithm > 2) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1 or 2");
+  if (algorithm == 2 && !(h->prec == 32 && (h->N == 16 || h->N == 32)))
+    return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel: FP32 and Nv in {16, 32} only");
+  h->algorithm = algorithm;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_get_algorithm(void* p, int* algorithm) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (!algorithm) return fail(DGFS_ERR_INVALID_VALUE, "null output");
+  *algorithm = h->last_algorithm;
   return DGFS_OK;
 }

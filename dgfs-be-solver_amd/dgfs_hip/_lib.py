@@ -73,6 +73,8 @@ _SIGNATURES = {
     "dgfs_vhs_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "dgfs_vhs_set_groups": [_P, _I],
     "dgfs_vhs_set_timing": [_P, _I],
+    "dgfs_vhs_set_algorithm": [_P, _I],
+    "dgfs_vhs_get_algorithm": [_P, C.POINTER(_I)],
     "dgfs_vhs_get_timing": [_P, C.POINTER(_D), C.POINTER(_I)],
 }
 

@@ -86,15 +86,24 @@ class VHSOperator:
         _lib.call("dgfs_vhs_get_config", self._h, C.byref(g), C.byref(s), C.byref(t))
         return {"ngroups": g.value, "nslots": s.value, "threads": t.value}
 
+    def set_algorithm(self, algorithm):
+        """0 automatic, 1 multi-pass, 2 register-resident (FP32, Nv 16/32)."""
+        _lib.call("dgfs_vhs_set_algorithm", self._h, int(algorithm))
+
+    def get_algorithm(self):
+        a = C.c_int()
+        _lib.call("dgfs_vhs_get_algorithm", self._h, C.byref(a))
+        return a.value
+
     def set_timing(self, on=True):
         _lib.call("dgfs_vhs_set_timing", self._h, int(bool(on)))
 
     def get_timing(self):
-        """{kernel: (total_ms, launches)} for prep, gain, finalize, project."""
-        ms = (C.c_double * 4)()
-        n = (C.c_int * 4)()
+        """{kernel: (total_ms, launches)} for prep, gain, finalize, project, nyquist."""
+        ms = (C.c_double * 5)()
+        n = (C.c_int * 5)()
         _lib.call("dgfs_vhs_get_timing", self._h, ms, n)
-        return {k: (ms[i], n[i]) for i, k in enumerate(("prep", "gain", "finalize", "project"))}
+        return {k: (ms[i], n[i]) for i, k in enumerate(("prep", "gain", "finalize", "project", "nyquist"))}
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -87,14 +87,22 @@ int dgfs_vhs_get_config(void* handle, int* ngroups, int* nslots, int* threads);
 /* Override the radial-group split (0 = automatic). */
 int dgfs_vhs_set_groups(void* handle, int ngroups);
 
+/* Gain-kernel algorithm: 0 automatic, 1 multi-pass (slab + pencil passes
+ * through a per-workgroup scratch, any N and precision), 2 register-resident
+ * (the whole N^3 field in registers, FP32 with Nv 16 or 32; Nyquist-plane
+ * corrections precomputed).  get_algorithm reports the one the last eval ran.
+ * No reference counterpart (the reference has one cuFFT dataflow). */
+int dgfs_vhs_set_algorithm(void* handle, int algorithm);
+int dgfs_vhs_get_algorithm(void* handle, int* algorithm);
+
 /* Kernel timing with HIP events recorded on the launch stream around every
  * launch (no added synchronisation).  set_timing(on) also clears the totals;
  * get_timing synchronises on the recorded events and returns, for the kernel
- * classes {prep, gain, finalize, project}, the summed milliseconds and the
- * number of launches.  The reference only prints wall-clock totals
+ * classes {prep, gain, finalize, project, nyquist}, the summed milliseconds
+ * and the number of launches.  The reference only prints wall-clock totals
  * (integrators/base.py:265-279); this has no direct counterpart. */
 int dgfs_vhs_set_timing(void* handle, int on);
-int dgfs_vhs_get_timing(void* handle, double* ms /*[4]*/, int* counts /*[4]*/);
+int dgfs_vhs_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);
 
 /* ------------------------------------------------------------------------
  * Flux-reconstruction advection of the velocity columns and RK stage kernels.

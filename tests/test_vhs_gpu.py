@@ -185,3 +185,26 @@ def test_bad_arguments_raise(device):
         op.eval_aosoa(8, 16, 1, 1024, 512, 4, 2, 4)         # range outside neles
     with pytest.raises(_lib.DGFSUnsupported):
         make_op(10, 5.0, 4, 6, 0.38, 1.0)
+
+
+@pytest.mark.parametrize("N,L,Nrho,M", [(16, 6.0, 8, 6), (32, 6.0, 16, 12)])
+def test_register_resident_matches(N, L, Nrho, M, device):
+    """The register-resident FP32 gain kernel against the multi-pass FP32
+    kernel (same arithmetic order up to FP32 rounding) and the FP64 oracle."""
+    ov = O.VelocityMesh(N, L, Nrho, M)
+    f = smooth_states(ov, 5, seed=N + 1)
+    Qref = O.VHSOracle(ov, 0.38, 1.0).q(f)
+    op, _ = make_op(N, L, Nrho, M, 0.38, 1.0, precision=32)
+    outs = {}
+    for alg, groups in [(1, 0), (2, 0), (2, 1), (2, 3)]:
+        say("fp32 algorithm", alg, "groups", groups, "N", N)
+        op.set_algorithm(alg)
+        op.set_groups(groups)
+        outs[(alg, groups)] = run_contig(op, f, device)
+        assert op.get_algorithm() == alg
+    e_mp = rel(outs[(1, 0)], Qref)
+    for key, Q in outs.items():
+        e = rel(Q, Qref)
+        say("  ", key, "vs oracle", e, "vs multi-pass", rel(Q, outs[(1, 0)]))
+        assert e < max(3 * e_mp, 1e-4)
+        assert rel(Q, outs[(1, 0)]) < 1e-4
