@@ -46,7 +46,28 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the bench, run 3 plain device copies (torch elementwise "
+                         "multiply by 1) of this many MiB, so that a counter run sees a "
+                         "kernel of known traffic next to the measured ones")
     return ap.parse_args()
+
+
+def calibrate_copy(mib, device):
+    """Three launches of a plain elementwise copy reading and writing `mib` MiB
+    each; a FETCH_SIZE / WRITE_SIZE counter run divides what it reads for
+    them by mib*1024 KiB to get the counter's scale."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device=device)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+    del a, b
+    torch.cuda.empty_cache()
 
 
 def make_states(vm, npts, seed, device, dtype=None):
@@ -69,16 +90,21 @@ def make_states(vm, npts, seed, device, dtype=None):
             params.append((1, np.zeros(6), rng.uniform(0.3, 3.0, 3)))
     cv = torch.as_tensor(vm.cv(), device=device)
     out = torch.empty((npts, vm.vsize()), dtype=torch.float64, device=device)
-    for i, (kind, u, T) in enumerate(params):
-        if kind == 0:
-            f = 0.0
-            for j in range(2):
-                uj = torch.as_tensor(u[3 * j:3 * j + 3], device=device).view(3, 1)
-                f = f + 0.5 * torch.exp(-((cv - uj) ** 2).sum(0) / T[j]) / (math.pi * T[j]) ** 1.5
-        else:
-            Tt = torch.as_tensor(T, device=device).view(3, 1)
-            f = torch.exp(-(cv ** 2 / Tt).sum(0)) / (math.pi ** 1.5 * math.sqrt(T.prod()))
-        out[i] = f / (f.sum() * vm.cw())
+    kind = np.array([k for k, _, _ in params])
+    U = torch.as_tensor(np.stack([u for _, u, _ in params]), device=device)
+    Tb = np.stack([np.resize(T, 3) for _, _, T in params])
+    T = torch.as_tensor(Tb, device=device)
+    isbi = torch.as_tensor(kind == 0, device=device).view(-1, 1)
+    for s0 in range(0, npts, 256):
+        s1 = min(npts, s0 + 256)
+        u1, u2 = U[s0:s1, 0:3, None], U[s0:s1, 3:6, None]
+        t = T[s0:s1, :, None]
+        c = cv[None]
+        bi = (0.5 * torch.exp(-((c - u1) ** 2).sum(1) / t[:, 0]) / (math.pi * t[:, 0]) ** 1.5
+              + 0.5 * torch.exp(-((c - u2) ** 2).sum(1) / t[:, 1]) / (math.pi * t[:, 1]) ** 1.5)
+        an = torch.exp(-(c ** 2 / t).sum(1)) / (math.pi ** 1.5 * torch.sqrt(t.prod(1)))
+        f = torch.where(isbi[s0:s1], bi, an)
+        out[s0:s1] = f / (f.sum(1, keepdim=True) * vm.cw())
     return out
 
 
@@ -86,6 +112,12 @@ def algorithmic_bytes(nv, K, c):
     """B_Q = 2 c n (3K + 3): one HBM read + write of each of the 3K+3
     transforms of the reference dataflow (complex, c bytes per value)."""
     return 2.0 * c * nv ** 3 * (3 * K + 3)
+
+
+def own_bytes(nv, K, Nrho, c):
+    """Same HBM model priced on this implementation's own transform count:
+    K packed inverse transforms, Nrho forward transforms, 3 more."""
+    return 2.0 * c * nv ** 3 * (K + Nrho + 3)
 
 
 def own_flops(nv, K, Nrho):
@@ -124,6 +156,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream().cuda_stream
+    if args.calibrate_copy_mib > 0:
+        calibrate_copy(args.calibrate_copy_mib, dev)
     nv, P, K = args.nv, args.points, args.M * args.nrho
     gamma = 2.0 * (1.0 - args.omega)
     vm = DGFSVelocityMesh.from_params(None, nv, args.L, args.nrho, args.M)
@@ -148,6 +182,7 @@ def main():
     ev_ms = e0.elapsed_time(e1)
     tim = op32.get_timing()
     cfg = op32.get_config()
+    algo = op32.get_algorithm()
     op32.set_timing(False)
     ms_per_step = 1e3 * wall / args.steps
     value = P * args.steps / wall
@@ -189,13 +224,16 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "k_vhs_gain<float,32>",
+            "kernel": (f"k_vhs_gain_rr<float,{nv}>" if algo == 2 else f"k_vhs_gain<float,{nv}>"),
             "algorithmic_bytes_per_eval": bq32,
             "points_per_launch": pts_per_launch,
             "avg_launch_ms": gain_ms / max(gain_n, 1),
+            "frac_own_transforms": own_bytes(nv, K, args.nrho, 8) * pts_per_launch
+            / (gain_ms / max(gain_n, 1) / 1e3) / HBM_PEAK,
             "note": "algorithmic model B_Q = 2*8*n*(3K+3) (every transform of the reference "
-                    "dataflow crossing HBM once); this kernel keeps transforms on chip, so "
-                    "frac > 1 is possible",
+                    "dataflow crossing HBM once); this kernel cuts the transform count to "
+                    "K+Nrho+3 and keeps transforms on chip, so frac > 1 is possible "
+                    "(algorithmic); frac_own_transforms prices this build's own count",
         },
         "compute": {
             "flops_per_eval": own_flops(nv, K, args.nrho),
