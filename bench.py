@@ -131,8 +131,10 @@ def own_flops(nv, K, Nrho):
 
 
 def read_traffic(name):
-    path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path):
+    """HBM bytes per dispatch from the committed FETCH_SIZE / WRITE_SIZE
+    counter runs (already scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", name) if name else ""
+    if not name or not os.path.isfile(path):
         return None
     with open(path) as fh:
         return json.load(fh)
@@ -192,7 +194,7 @@ def main():
     pts_per_launch = P * args.steps / max(gain_n, 1)
     achieved = bq32 * pts_per_launch / (gain_ms / max(gain_n, 1) / 1e3)
     traffic = None
-    tr = read_traffic("vhs_gain_fp32_n32_traffic.json")
+    tr = read_traffic(f"vhs_gain_rr_fp32_n{nv}_traffic.json" if algo == 2 else "")
     if tr and tr.get("points_per_dispatch"):
         traffic = tr["bytes_per_dispatch"] * pts_per_launch / tr["points_per_dispatch"]
 
