@@ -39,6 +39,7 @@
 
 #include "common.hpp"
 #include "cplx.hpp"
+#include "spectral.hpp"
 
 namespace dgfs {
 
@@ -47,113 +48,10 @@ std::string& last_error_ref() {
   return s;
 }
 
-// ---------------------------------------------------------------------------
-// workgroup building blocks
-// ---------------------------------------------------------------------------
-
-// 2D FFTs of `cnt` N x N planes staged through LDS (row stride N+1 to spread
-// banks).  load(j, r, c) gives element (r, c) of plane j; the plane is
-// transformed along c then along r; store(j, r, c, v) receives the result.
-template <typename T, int N, int NT, int SIGN, class Load, class Store>
-__device__ __forceinline__ void slab_batch(cpx<T>* lds, int cnt, Load load, Store store) {
-  constexpr int NP = N + 1, PL = N * NP, NN = N * N;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < cnt * NN; e += NT) {
-    const int j = e / NN, rem = e - j * NN, r = rem / N, c = rem - r * N;
-    lds[j * PL + r * NP + c] = load(j, r, c);
-  }
-  __syncthreads();
-  if (tid < cnt * N) {
-    const int j = tid / N, r = tid - j * N;
-    cpx<T>* row = lds + j * PL + r * NP;
-    cpx<T> v[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = row[k];
-    FFT<T, N, SIGN>::run(v);
-#pragma unroll
-    for (int k = 0; k < N; ++k) row[k] = v[k];
-  }
-  __syncthreads();
-  if (tid < cnt * N) {
-    const int j = tid / N, c = tid - j * N;
-    cpx<T>* col = lds + j * PL + c;
-    cpx<T> v[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = col[k * NP];
-    FFT<T, N, SIGN>::run(v);
-#pragma unroll
-    for (int k = 0; k < N; ++k) col[k * NP] = v[k];
-  }
-  __syncthreads();
-  for (int e = tid; e < cnt * NN; e += NT) {
-    const int j = e / NN, rem = e - j * NN, r = rem / N, c = rem - r * N;
-    store(j, r, c, lds[j * PL + r * NP + c]);
-  }
-  __syncthreads();
-}
-
-// 1D FFTs along x of the N*N pencils (y, z); pencil idx = y*N + z.
-template <typename T, int N, int NT, int SIGN, class Load, class Epi>
-__device__ __forceinline__ void pencil_pass(Load load, Epi epi) {
-  for (int idx = threadIdx.x; idx < N * N; idx += NT) {
-    cpx<T> v[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = load(k, idx);
-    FFT<T, N, SIGN>::run(v);
-    epi(idx, v);
-  }
-  __syncthreads();
-}
-
-template <int N>
-struct Shape {
-  static constexpr int NT = (N >= 16) ? 128 : 64;
-  static constexpr int PB = NT / N;  // planes per slab batch
-  // the Nyquist pass transforms 3 planes in one batch and the phase tables
-  // are filled by 3N threads
-  static_assert(PB >= 3, "slab batch must hold the three Nyquist planes");
-  static_assert(3 * N <= NT, "phase tables need 3N threads");
-};
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <typename T>
-struct PrepArgs {
-  const double* f;  // [npts][n] contiguous
-  cpx<T>* F;        // [npts][n] natural spectral order, scaled by 1/n
-  cpx<T>* Ft;       // optional copy transposed to [kx][kz][ky] (register-resident gain)
-  cpx<T>* Z;        // scratch [nslots][n]
-  int npts;
-};
-
-template <typename T, int N>
-__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_prep(PrepArgs<T> a) {
-  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N;
-  __shared__ cpx<T> lds[PB * N * (N + 1)];
-  cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
-  const T rn = static_cast<T>(1.0 / static_cast<double>(n));
-  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
-    const double* f = a.f + static_cast<size_t>(pt) * n;
-    cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
-    cpx<T>* Ft = a.Ft ? a.Ft + static_cast<size_t>(pt) * n : nullptr;
-    pencil_pass<T, N, NT, -1>(
-        [&](int x, int idx) { return cpx<T>{static_cast<T>(f[x * NN + idx]), T(0)}; },
-        [&](int idx, cpx<T>* v) {
-#pragma unroll
-          for (int k = 0; k < N; ++k) Z[k * NN + idx] = v[k];
-        });
-    for (int kx0 = 0; kx0 < N; kx0 += PB) {
-      const int cnt = min(PB, N - kx0);
-      slab_batch<T, N, NT, -1>(
-          lds, cnt, [&](int j, int r, int c) { return Z[(kx0 + j) * NN + r * N + c]; },
-          [&](int j, int r, int c, cpx<T> v) {
-            F[(kx0 + j) * NN + r * N + c] = rn * v;
-            if (Ft) Ft[(kx0 + j) * NN + c * N + r] = rn * v;
-          });
-    }
-  }
-}
 
 template <typename T>
 struct GainArgs {
@@ -751,6 +649,29 @@ __global__ __launch_bounds__(256) void k_aosoa_scatter(const double* __restrict_
   }
 }
 
+int launch_aosoa_gather(const double* f, double* out, int nupts, int ldim, int vsize, int e0, int ne,
+                        hipStream_t st) {
+  if (ne <= 0 || nupts <= 0) return DGFS_OK;
+  const int eb_lo = e0 / 32, eb_hi = (e0 + ne - 1) / 32;
+  dim3 grid((vsize + 31) / 32, eb_hi - eb_lo + 1, nupts);
+  // the kernel indexes element blocks from the shifted base pointer
+  hipLaunchKernelGGL(k_aosoa_gather, grid, dim3(256), 0, st, f + static_cast<size_t>(eb_lo) * vsize * 32,
+                     out, nupts, ldim, vsize, e0 - eb_lo * 32, ne);
+  DGFS_LAUNCH_CHECK("aosoa_gather");
+  return DGFS_OK;
+}
+
+int launch_aosoa_scatter(const double* in, double* Q, int nupts, int ldim, int vsize, int e0, int ne,
+                         hipStream_t st) {
+  if (ne <= 0 || nupts <= 0) return DGFS_OK;
+  const int eb_lo = e0 / 32, eb_hi = (e0 + ne - 1) / 32;
+  dim3 grid((vsize + 31) / 32, eb_hi - eb_lo + 1, nupts);
+  hipLaunchKernelGGL(k_aosoa_scatter, grid, dim3(256), 0, st, in, Q + static_cast<size_t>(eb_lo) * vsize * 32,
+                     nupts, ldim, vsize, e0 - eb_lo * 32, ne);
+  DGFS_LAUNCH_CHECK("aosoa_scatter");
+  return DGFS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -792,53 +713,6 @@ struct VhsHandle {
   int acc_n[5] = {0, 0, 0, 0, 0};
 };
 
-static hipEvent_t take_event(VhsHandle* h) {
-  if (!h->pool.empty()) {
-    hipEvent_t e = h->pool.back();
-    h->pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-// record a start event for kernel class k (0 prep, 1 gain, 2 finalize, 3 project,
-// 4 Nyquist planes)
-struct KTimer {
-  VhsHandle* h;
-  int k;
-  hipStream_t st;
-  hipEvent_t a = nullptr;
-  KTimer(VhsHandle* h_, int k_, hipStream_t st_) : h(h_), k(k_), st(st_) {
-    if (h->timing) {
-      a = take_event(h);
-      if (a) (void)hipEventRecord(a, st);
-    }
-  }
-  void stop() {
-    if (!a) return;
-    hipEvent_t b = take_event(h);
-    if (!b) return;
-    (void)hipEventRecord(b, st);
-    h->pending.push_back({k, {a, b}});
-    a = nullptr;
-  }
-};
-
-static void resolve_timing(VhsHandle* h) {
-  for (auto& pr : h->pending) {
-    float ms = 0.f;
-    (void)hipEventSynchronize(pr.second.second);
-    if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
-      h->acc_ms[pr.first] += ms;
-      h->acc_n[pr.first] += 1;
-    }
-    h->pool.push_back(pr.second.first);
-    h->pool.push_back(pr.second.second);
-  }
-  h->pending.clear();
-}
 
 static void free_ws(VhsHandle* h) {
   (void)hipFree(h->Wny);
@@ -922,12 +796,12 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
     NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Wny), h->alpha, h->szd, npts,
                   h->Nrho, h->M};
     const long nyq_items = static_cast<long>(npts) * K;
-    KTimer t4(h, 4, st);
+    KTimerT<VhsHandle> t4(h, 4, st);
     hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(nyq_items, 8L * cu_count()))),
                        dim3(Shape<N>::NT), 0, st, na);
     DGFS_LAUNCH_CHECK("vhs_nyq");
     t4.stop();
-    KTimer t1(h, 1, st);
+    KTimerT<VhsHandle> t1(h, 1, st);
     const size_t lds = (static_cast<size_t>(RRLds<N>::xsize) + RRLds<N>::wl) * sizeof(cpx<T>) +
                        static_cast<size_t>(RRLds<N>::s0) * sizeof(T);
     DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vhs_gain_rr<T, N>),
@@ -980,7 +854,7 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   }
   PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), use_rr(h) ? static_cast<cpx<T>*>(h->Ft) : nullptr,
                  static_cast<cpx<T>*>(h->Z), npts};
-  KTimer t0(h, 0, st);
+  KTimerT<VhsHandle> t0(h, 0, st);
   hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, pa);
   DGFS_LAUNCH_CHECK("vhs_prep");
   t0.stop();
@@ -995,7 +869,7 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
     if (rc) return rc;
   } else {
     h->last_algorithm = 1;
-    KTimer t1(h, 1, st);
+    KTimerT<VhsHandle> t1(h, 1, st);
     hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
                        dim3(Shape<N>::NT), 0, st, ga);
     DGFS_LAUNCH_CHECK("vhs_gain");
@@ -1005,7 +879,7 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   FinArgs<T> fa{static_cast<const cpx<T>*>(h->Ghat), static_cast<const cpx<T>*>(h->F),
                 static_cast<const T*>(h->b2), fc, Qc, static_cast<cpx<T>*>(h->Z),
                 h->prefac, npts, groups};
-  KTimer t2(h, 2, st);
+  KTimerT<VhsHandle> t2(h, 2, st);
   hipLaunchKernelGGL((k_vhs_finalize<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, fa);
   DGFS_LAUNCH_CHECK("vhs_finalize");
   t2.stop();
@@ -1015,7 +889,7 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
     pr.Q = Qc;
     std::memcpy(pr.Ainv, h->Ainv, sizeof(pr.Ainv));
     pr.L = h->L; pr.cw = h->cw; pr.N = N; pr.npts = npts;
-    KTimer t3(h, 3, st);
+    KTimerT<VhsHandle> t3(h, 3, st);
     hipLaunchKernelGGL(k_vhs_project, dim3(std::min(npts, 4 * cu_count())), dim3(256), 0, st, pr);
     DGFS_LAUNCH_CHECK("vhs_project");
     t3.stop();
@@ -1070,19 +944,12 @@ static int eval_impl(VhsHandle* h, const double* f, double* Q, bool aosoa, int n
       const int pts = cne * nupts;
       int rc = ensure_ws<T, N>(h, pts);
       if (rc) return rc;
-      const int eb_lo = (e0 + c0) / 32, eb_hi = (e0 + c0 + cne - 1) / 32;
-      dim3 grid((vsize + 31) / 32, eb_hi - eb_lo + 1, nupts);
-      // the kernels take absolute element blocks; shift the base pointer
-      hipLaunchKernelGGL(k_aosoa_gather, grid, dim3(256), 0, st,
-                         f + static_cast<size_t>(eb_lo) * vsize * 32, h->fc, nupts, ldim, vsize,
-                         e0 + c0 - eb_lo * 32, cne);
-      DGFS_LAUNCH_CHECK("aosoa_gather");
+      rc = launch_aosoa_gather(f, h->fc, nupts, ldim, vsize, e0 + c0, cne, st);
+      if (rc) return rc;
       rc = run_chunk<T, N>(h, h->fc, h->Qc, pts, st);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_aosoa_scatter, grid, dim3(256), 0, st, h->Qc,
-                         Q + static_cast<size_t>(eb_lo) * vsize * 32, nupts, ldim, vsize,
-                         e0 + c0 - eb_lo * 32, cne);
-      DGFS_LAUNCH_CHECK("aosoa_scatter");
+      rc = launch_aosoa_scatter(h->Qc, Q, nupts, ldim, vsize, e0 + c0, cne, st);
+      if (rc) return rc;
     }
     return DGFS_OK;
   }

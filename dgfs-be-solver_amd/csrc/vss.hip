@@ -1,0 +1,878 @@
+// Multi-species fast-spectral VSS collision operator Q_p = sum_q Q_pq(f_p, f_q)
+// for gfx950.
+//
+// Dataflow of the reference's binary-mixture operator
+// (frfs/solvers/dgfsbi/scattering.py:259-330, kernels/scattering/vss-gll.mako),
+// per ordered species pair (p, q) and spatial point:
+//
+//   F = FFT(f_p)/n,  G = FFT(f_q)/n                              (prep kernel)
+//   for every radial node x and full-sphere direction y, a = (pi/L) qz_x (s_y . l):
+//     t3 = IFFT(exp(+i a m_q/(m_p+m_q)) F)                         (vss-gll.mako:148-151)
+//     t4 = IFFT(exp(-i a m_p/(m_p+m_q)) G)                         (mako:153-156)
+//     G^ += (qw_x sw / n) b_xy FFT(t3 t4)                          (mako:163-188, 224-250)
+//   Q_p += prefac_pq (Re IFFT(G^) - f_p Re IFFT(c G))              (mako:252-300)
+//
+// The phase is separable, exp(i a) = ex(kx) ey(ky) ez(kz), so the reference's
+// a table (Nrho*M*n) is never stored; b_xy (complex, Nrho*M*n per pair) and c
+// (n per pair) are built on the device in FP64 from the Mpre-point design
+// (vss-gll.mako:53-95).
+//
+// All three transforms of every (x, y) stay inside one workgroup: slab passes
+// (kx-planes 2D-transformed in LDS) and pencil passes (x-pencils in registers)
+// through two per-workgroup scratch cubes that the persistent grid reuses item
+// after item, so they live in L2 / Infinity Cache.  The work items of all
+// species pairs go into one launch.
+//
+// Finalize: Re IFFT(Z) = IFFT(H(Z)) with H the Hermitian part,
+// H(Z)(k) = (Z(k) + conj Z(-k))/2, so one complex transform of
+// sum_q prefac_pq [H(G^_pq) + i H(c_pq G_q)] yields the gain (real part) and
+// the loss field (imaginary part) of every pair of species p at once.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "cplx.hpp"
+#include "spectral.hpp"
+
+namespace dgfs {
+
+constexpr int kVssMaxSpecies = 4;
+constexpr int kVssMaxPairs = kVssMaxSpecies * kVssMaxSpecies;
+
+// ---------------------------------------------------------------------------
+// tables (FP64 on the device, stored in T)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lfreq(int k, int N) { return (k < N / 2) ? k : k - N; }
+
+// b[x*M + y][r] = swpre qz_x^(gamma+2) sum_j B_yj exp(-i mpq (pi/L) qz_x (s'_j . l_r)),
+// B_yj = (1 + s_y . s'_j)^eta (vss-gll.mako:59-94).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vss_btab(cpx<T>* b, const double* __restrict__ Byj,
+                                                  const double* __restrict__ szpre, const double* __restrict__ qz,
+                                                  double L, double gamma, double mpq, double swpre, int N,
+                                                  int M, int Mpre) {
+  const int n = N * N * N, r = blockIdx.x * blockDim.x + threadIdx.x, xy = blockIdx.y;
+  if (r >= n) return;
+  const int x = xy / M, y = xy - x * M;
+  const double lx = lfreq(r / (N * N), N), ly = lfreq((r / N) % N, N), lz = lfreq(r % N, N);
+  const double fac = 3.14159265358979323846 / L * qz[x];
+  const double fb = swpre * pow(qz[x], gamma + 2.0);
+  double re = 0.0, im = 0.0;
+  for (int j = 0; j < Mpre; ++j) {
+    const double th = mpq * fac * (szpre[3 * j] * lx + szpre[3 * j + 1] * ly + szpre[3 * j + 2] * lz);
+    double s, c;
+    sincos(th, &s, &c);
+    const double B = Byj[y * Mpre + j];
+    re += B * fb * c;
+    im -= B * fb * s;
+  }
+  b[static_cast<size_t>(xy) * n + r] = cpx<T>{static_cast<T>(re), static_cast<T>(im)};
+}
+
+// c[r] = sum_x qw_x sw swpre qz_x^(gamma+2) sum_j Bsum_j exp(-i (pi/L) qz_x (s'_j . l_r)),
+// Bsum_j = sum_y B_yj (vss-gll.mako:86-92, accumulated over x, y).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vss_ctab(cpx<T>* c, const double* __restrict__ Bsum,
+                                                  const double* __restrict__ szpre, const double* __restrict__ qz,
+                                                  const double* __restrict__ qw, double L, double gamma, double sw,
+                                                  double swpre, int N, int Nrho, int Mpre) {
+  const int n = N * N * N, r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double lx = lfreq(r / (N * N), N), ly = lfreq((r / N) % N, N), lz = lfreq(r % N, N);
+  double re = 0.0, im = 0.0;
+  for (int x = 0; x < Nrho; ++x) {
+    const double fac = 3.14159265358979323846 / L * qz[x];
+    const double fc = qw[x] * sw * swpre * pow(qz[x], gamma + 2.0);
+    for (int j = 0; j < Mpre; ++j) {
+      const double th = fac * (szpre[3 * j] * lx + szpre[3 * j + 1] * ly + szpre[3 * j + 2] * lz);
+      double s, cc;
+      sincos(th, &s, &cc);
+      re += Bsum[j] * fc * cc;
+      im -= Bsum[j] * fc * s;
+    }
+  }
+  c[r] = cpx<T>{static_cast<T>(re), static_cast<T>(im)};
+}
+
+// ---------------------------------------------------------------------------
+// gain: one work item = (pair, point, radial group)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct VssGainArgs {
+  const cpx<T>* F;   // [nspcs][npts][n] spectra, scaled by 1/n
+  cpx<T>* Ghat;      // [npairs][npts][ngroups][n]
+  cpx<T>* Z1;        // scratch [nslots][n]
+  cpx<T>* Z2;        // scratch [nslots][n]
+  const double* alpha;  // [Nrho] (pi/L) qz_x
+  const double* wgt;    // [Nrho] qw_x sw / n
+  const double* sz;     // [M][3]
+  const cpx<T>* b[kVssMaxPairs];  // [Nrho*M][n] per pair
+  int sp[kVssMaxPairs], sq[kVssMaxPairs];
+  double mu1[kVssMaxPairs], mu2[kVssMaxPairs];  // m_q/(m_p+m_q), m_p/(m_p+m_q)
+  int nitems, npts, ngroups, ppg, Nrho, M;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  __shared__ cpx<T> e1[3][N], e2[3][N];
+  const int tid = threadIdx.x;
+  cpx<T>* Z1 = a.Z1 + static_cast<size_t>(blockIdx.x) * n;
+  cpx<T>* Z2 = a.Z2 + static_cast<size_t>(blockIdx.x) * n;
+  const int per_pair = a.npts * a.ngroups;
+
+  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+    const int pair = item / per_pair, rem = item - pair * per_pair;
+    const int pt = rem / a.ngroups, g = rem - pt * a.ngroups;
+    const int x0 = g * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(a.sp[pair]) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(a.sq[pair]) * a.npts + pt) * n;
+    const cpx<T>* bp = a.b[pair];
+    const double mu1 = a.mu1[pair], mu2 = a.mu2[pair];
+    cpx<T>* G = a.Ghat + static_cast<size_t>(item) * n;
+    for (int i = tid; i < n; i += NT) G[i] = cpx<T>{T(0), T(0)};
+    for (int x = x0; x < x1; ++x) {
+      const double alpha = a.alpha[x];
+      const T w = static_cast<T>(a.wgt[x]);
+      for (int y = 0; y < a.M; ++y) {
+        __syncthreads();
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double th = alpha * a.sz[y * 3 + d] * lfreq(k, N);
+          double s, c;
+          sincos(mu1 * th, &s, &c);
+          e1[d][k] = cpx<T>{static_cast<T>(c), static_cast<T>(s)};
+          sincos(mu2 * th, &s, &c);
+          e2[d][k] = cpx<T>{static_cast<T>(c), static_cast<T>(-s)};
+        }
+        __syncthreads();
+        // t3 = IFFT(E1 F) into Z1
+        for (int kx0 = 0; kx0 < N; kx0 += PB) {
+          const int cnt = min(PB, N - kx0);
+          slab_batch<T, N, NT, +1>(
+              lds, cnt,
+              [&](int j, int ky, int kz) -> cpx<T> {
+                const int kx = kx0 + j;
+                return F1[kx * NN + ky * N + kz] * (e1[0][kx] * e1[1][ky] * e1[2][kz]);
+              },
+              [&](int j, int yy, int z, cpx<T> v) { Z2[(kx0 + j) * NN + yy * N + z] = v; });
+        }
+        pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z2[kx * NN + idx]; },
+                                  [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                    for (int xx = 0; xx < N; ++xx) Z1[xx * NN + idx] = v[xx];
+                                  });
+        // t4 = IFFT(E2 G); S = t3 t4 into Z1
+        for (int kx0 = 0; kx0 < N; kx0 += PB) {
+          const int cnt = min(PB, N - kx0);
+          slab_batch<T, N, NT, +1>(
+              lds, cnt,
+              [&](int j, int ky, int kz) -> cpx<T> {
+                const int kx = kx0 + j;
+                return F2[kx * NN + ky * N + kz] * (e2[0][kx] * e2[1][ky] * e2[2][kz]);
+              },
+              [&](int j, int yy, int z, cpx<T> v) { Z2[(kx0 + j) * NN + yy * N + z] = v; });
+        }
+        pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z2[kx * NN + idx]; },
+                                  [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                    for (int xx = 0; xx < N; ++xx) Z1[xx * NN + idx] = Z1[xx * NN + idx] * v[xx];
+                                  });
+        // G^ += w b_xy FFT(S)
+        pencil_pass<T, N, NT, -1>([&](int xx, int idx) { return Z1[xx * NN + idx]; },
+                                  [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                    for (int k = 0; k < N; ++k) Z2[k * NN + idx] = v[k];
+                                  });
+        const cpx<T>* bxy = bp + static_cast<size_t>(x * a.M + y) * n;
+        for (int kx0 = 0; kx0 < N; kx0 += PB) {
+          const int cnt = min(PB, N - kx0);
+          slab_batch<T, N, NT, -1>(
+              lds, cnt, [&](int j, int r, int c) { return Z2[(kx0 + j) * NN + r * N + c]; },
+              [&](int j, int r, int c, cpx<T> v) {
+                const int o = (kx0 + j) * NN + r * N + c;
+                G[o] = G[o] + (w * bxy[o]) * v;
+              });
+        }
+      }
+    }
+  }
+  (void)h;
+}
+
+// ---------------------------------------------------------------------------
+// finalize: one item = (species p, point)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct VssFinArgs {
+  const cpx<T>* Ghat;  // [npairs][npts][ngroups][n]
+  const cpx<T>* F;     // [nspcs][npts][n]
+  const double* f[kVssMaxSpecies];  // [npts][n] per species
+  double* Q[kVssMaxSpecies];        // [npts][n] per species
+  cpx<T>* Z;           // scratch [nslots][n]
+  const cpx<T>* c[kVssMaxPairs];
+  double prefac[kVssMaxPairs];
+  int nspcs, npts, ngroups;
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vss_finalize(VssFinArgs<T> a) {
+  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N;
+  __shared__ cpx<T> lds[PB * N * (N + 1)];
+  cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
+  const int nitems = a.nspcs * a.npts;
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int p = item / a.npts, pt = item - p * a.npts;
+    for (int kx0 = 0; kx0 < N; kx0 += PB) {
+      const int cnt = min(PB, N - kx0);
+      slab_batch<T, N, NT, +1>(
+          lds, cnt,
+          [&](int j, int ky, int kz) -> cpx<T> {
+            const int kx = kx0 + j;
+            const int o = kx * NN + ky * N + kz;
+            const int om = ((N - kx) % N) * NN + ((N - ky) % N) * N + (N - kz) % N;
+            cpx<T> acc{T(0), T(0)};
+            for (int q = 0; q < a.nspcs; ++q) {
+              const int pair = p * a.nspcs + q;
+              const cpx<T>* G = a.Ghat + (static_cast<size_t>(pair) * a.npts + pt) * a.ngroups * n;
+              cpx<T> g0 = G[o], g1 = G[om];
+              for (int gg = 1; gg < a.ngroups; ++gg) {
+                g0 = g0 + G[static_cast<size_t>(gg) * n + o];
+                g1 = g1 + G[static_cast<size_t>(gg) * n + om];
+              }
+              const cpx<T>* Fq = a.F + (static_cast<size_t>(q) * a.npts + pt) * n;
+              const cpx<T>* cq = a.c[pair];
+              const cpx<T> l0 = cq[o] * Fq[o], l1 = cq[om] * Fq[om];
+              const cpx<T> gh = T(0.5) * (g0 + conj(g1)), lh = T(0.5) * (l0 + conj(l1));
+              acc = acc + static_cast<T>(a.prefac[pair]) * (gh + mul_i(lh));
+            }
+            return acc;
+          },
+          [&](int j, int r, int c, cpx<T> v) { Z[(kx0 + j) * NN + r * N + c] = v; });
+    }
+    const double* f = a.f[p] + static_cast<size_t>(pt) * n;
+    double* Q = a.Q[p] + static_cast<size_t>(pt) * n;
+    pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z[kx * NN + idx]; },
+                              [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                for (int x = 0; x < N; ++x) {
+                                  const int o = x * NN + idx;
+                                  Q[o] = static_cast<double>(v[x].x) - f[o] * static_cast<double>(v[x].y);
+                                }
+                              });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 projection onto the mixture's collision invariants: the mass of every
+// species, total momentum sum_s m_s int c Q_s and total energy
+// sum_s m_s int |c|^2 Q_s.  Minimum-norm correction over the stacked vector
+// (Q_0, ..., Q_{S-1}): d = -C^T (C C^T)^{-1} C Q, C_j(s, r) = phi_j(s, r) cw.
+// ---------------------------------------------------------------------------
+constexpr int kVssMaxCon = kVssMaxSpecies + 4;
+
+struct VssProjArgs {
+  double* Q[kVssMaxSpecies];  // [npts][n] per species
+  double Ainv[kVssMaxCon * kVssMaxCon];
+  double m[kVssMaxSpecies];
+  double L, cw;
+  int N, nspcs, npts;
+};
+
+__global__ __launch_bounds__(256) void k_vss_project(VssProjArgs a) {
+  __shared__ double red[kVssMaxCon][256];
+  __shared__ double lam[kVssMaxCon];
+  const int n = a.N * a.N * a.N, tid = threadIdx.x, S = a.nspcs, nc = S + 4;
+  const double hh = 2.0 * a.L / a.N, c0 = -a.L + 0.5 * hh;
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    // red rows: mass of species 0..S-1, then momentum x, y, z and energy
+    double mass[kVssMaxSpecies] = {};
+    double m1 = 0, m2 = 0, m3 = 0, m4 = 0;
+#pragma unroll
+    for (int s = 0; s < kVssMaxSpecies; ++s) {
+      if (s < S) {
+        const double* Q = a.Q[s] + static_cast<size_t>(pt) * n;
+        const double ms = a.m[s];
+        for (int r = tid; r < n; r += 256) {
+          const int I = r / (a.N * a.N), J = (r / a.N) % a.N, K = r % a.N;
+          const double cx = c0 + I * hh, cy = c0 + J * hh, cz = c0 + K * hh, q = Q[r];
+          mass[s] += q;
+          m1 += ms * q * cx;
+          m2 += ms * q * cy;
+          m3 += ms * q * cz;
+          m4 += ms * q * (cx * cx + cy * cy + cz * cz);
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kVssMaxSpecies; ++s)
+      if (s < S) red[s][tid] = mass[s] * a.cw;
+    red[S][tid] = m1 * a.cw;
+    red[S + 1][tid] = m2 * a.cw;
+    red[S + 2][tid] = m3 * a.cw;
+    red[S + 3][tid] = m4 * a.cw;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (tid < st)
+        for (int j = 0; j < nc; ++j) red[j][tid] += red[j][tid + st];
+      __syncthreads();
+    }
+    if (tid < nc) {
+      double v = 0;
+      for (int k = 0; k < nc; ++k) v += a.Ainv[tid * nc + k] * red[k][0];
+      lam[tid] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kVssMaxSpecies; ++s) {
+      if (s < S) {
+        double* Q = a.Q[s] + static_cast<size_t>(pt) * n;
+        const double ms = a.m[s], ls = lam[s];
+        for (int r = tid; r < n; r += 256) {
+          const int I = r / (a.N * a.N), J = (r / a.N) % a.N, K = r % a.N;
+          const double cx = c0 + I * hh, cy = c0 + J * hh, cz = c0 + K * hh;
+          Q[r] -= a.cw * (ls + ms * (lam[S] * cx + lam[S + 1] * cy + lam[S + 2] * cz +
+                                     lam[S + 3] * (cx * cx + cy * cy + cz * cz)));
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct VssHandle {
+  int magic = 0x76737331;
+  int N = 0, n = 0, Nrho = 0, M = 0, Mpre = 0, nspcs = 0, npairs = 0, prec = 64, project = 0;
+  double L = 0, sw = 0, cw = 0;
+  size_t tsize = 8;
+  double masses[kVssMaxSpecies] = {};
+  double prefac[kVssMaxPairs] = {};
+  int sp[kVssMaxPairs] = {}, sq[kVssMaxPairs] = {};
+  double mu1[kVssMaxPairs] = {}, mu2[kVssMaxPairs] = {};
+  void* b[kVssMaxPairs] = {};
+  void* c[kVssMaxPairs] = {};
+  double* alpha = nullptr;
+  double* wgt = nullptr;
+  double* szd = nullptr;
+  double Ainv[kVssMaxCon * kVssMaxCon] = {};
+  // workspace
+  size_t cap_pts = 0, cap_items = 0;
+  int cap_slots = 0;
+  double* fc = nullptr;  // [nspcs][cap_pts][n]
+  double* Qc = nullptr;
+  void* F = nullptr;     // [nspcs][cap_pts][n] cpx<T>
+  void* Ghat = nullptr;  // [cap_items][n] cpx<T>
+  void* Z1 = nullptr;
+  void* Z2 = nullptr;
+  int groups_override = 0;
+  int last_groups = 0, last_slots = 0, threads = 0;
+  int timing = 0;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double acc_ms[5] = {0, 0, 0, 0, 0};
+  int acc_n[5] = {0, 0, 0, 0, 0};
+
+  ~VssHandle() {
+    for (int k = 0; k < kVssMaxPairs; ++k) {
+      (void)hipFree(b[k]);
+      (void)hipFree(c[k]);
+    }
+    (void)hipFree(alpha); (void)hipFree(wgt); (void)hipFree(szd);
+    (void)hipFree(fc); (void)hipFree(Qc); (void)hipFree(F); (void)hipFree(Ghat);
+    (void)hipFree(Z1); (void)hipFree(Z2);
+    for (auto& pr : pending) {
+      (void)hipEventDestroy(pr.second.first);
+      (void)hipEventDestroy(pr.second.second);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+static int invert_n(const double* A, double* Ainv, int m) {
+  std::vector<double> w(static_cast<size_t>(m) * 2 * m);
+  auto at = [&](int i, int j) -> double& { return w[static_cast<size_t>(i) * 2 * m + j]; };
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < 2 * m; ++j) at(i, j) = (j < m) ? A[i * m + j] : (j - m == i ? 1.0 : 0.0);
+  for (int col = 0; col < m; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < m; ++r)
+      if (std::fabs(at(r, col)) > std::fabs(at(piv, col))) piv = r;
+    if (std::fabs(at(piv, col)) < 1e-300) return 1;
+    for (int j = 0; j < 2 * m; ++j) std::swap(at(col, j), at(piv, j));
+    const double d = at(col, col);
+    for (int j = 0; j < 2 * m; ++j) at(col, j) /= d;
+    for (int r = 0; r < m; ++r)
+      if (r != col) {
+        const double fct = at(r, col);
+        for (int j = 0; j < 2 * m; ++j) at(r, j) -= fct * at(col, j);
+      }
+  }
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) Ainv[i * m + j] = at(i, j + m);
+  return 0;
+}
+
+template <typename T, int N>
+static int vss_slots() {
+  int o1 = 1, o2 = 1, o3 = 1;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, reinterpret_cast<const void*>(k_vss_gain<T, N>),
+                                                     Shape<N>::NT, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, reinterpret_cast<const void*>(k_vhs_prep<T, N>),
+                                                     Shape<N>::NT, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, reinterpret_cast<const void*>(k_vss_finalize<T, N>),
+                                                     Shape<N>::NT, 0);
+  return std::max(1, std::min(o1, std::min(o2, o3))) * cu_count();
+}
+
+template <typename T, int N>
+static int vss_ensure_ws(VssHandle* h, int pts) {
+  constexpr int n = N * N * N;
+  if (h->cap_slots == 0) {
+    const int slots = vss_slots<T, N>();
+    DGFS_HIP_TRY(hipMalloc(&h->Z1, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
+    DGFS_HIP_TRY(hipMalloc(&h->Z2, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
+    h->cap_slots = slots;
+  }
+  if (static_cast<size_t>(pts) > h->cap_pts) {
+    (void)hipFree(h->F); (void)hipFree(h->fc); (void)hipFree(h->Qc);
+    h->F = nullptr; h->fc = h->Qc = nullptr;
+    h->cap_pts = 0;
+    const size_t cnt = static_cast<size_t>(h->nspcs) * pts * n;
+    DGFS_HIP_TRY(hipMalloc(&h->F, cnt * sizeof(cpx<T>)));
+    DGFS_HIP_TRY(hipMalloc(&h->fc, cnt * sizeof(double)));
+    DGFS_HIP_TRY(hipMalloc(&h->Qc, cnt * sizeof(double)));
+    h->cap_pts = pts;
+  }
+  return DGFS_OK;
+}
+
+// points per launch: per point the species' f, Q (FP64) and spectra, plus the
+// gain partials of every pair (worst case one per radial node); under 48 GiB
+static int vss_chunk_points(const VssHandle* h, int total) {
+  const double per = static_cast<double>(h->n) *
+                     (h->nspcs * (16.0 + 2.0 * h->tsize) + h->npairs * 2.0 * h->tsize * h->Nrho);
+  const double budget = 48.0 * 1024.0 * 1024.0 * 1024.0;
+  const int cap = std::max(1, static_cast<int>(budget / per));
+  if (total <= cap) return std::max(total, 1);
+  const int nchunks = (total + cap - 1) / cap;
+  return (total + nchunks - 1) / nchunks;
+}
+
+// fs[s], Qs[s]: [npts][n] contiguous per species
+template <typename T, int N>
+static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Qs, int npts, hipStream_t st) {
+  constexpr int n = N * N * N;
+  const int slots = h->cap_slots;
+  int groups = h->groups_override;
+  if (groups <= 0) {
+    const int base = h->npairs * std::max(npts, 1);
+    groups = std::max(1, std::min(h->Nrho, (2 * slots + base - 1) / base));
+  }
+  const int ppg = (h->Nrho + groups - 1) / groups;
+  groups = (h->Nrho + ppg - 1) / ppg;
+  const size_t items = static_cast<size_t>(h->npairs) * npts * groups;
+  if (items > h->cap_items) {
+    (void)hipFree(h->Ghat);
+    h->Ghat = nullptr;
+    h->cap_items = 0;
+    DGFS_HIP_TRY(hipMalloc(&h->Ghat, items * n * sizeof(cpx<T>)));
+    h->cap_items = items;
+  }
+  h->last_groups = groups;
+  h->last_slots = slots;
+  h->threads = Shape<N>::NT;
+
+  cpx<T>* F = static_cast<cpx<T>*>(h->F);
+  {
+    KTimerT<VssHandle> t0(h, 0, st);
+    for (int s = 0; s < h->nspcs; ++s) {
+      PrepArgs<T> pa{fs[s], F + static_cast<size_t>(s) * npts * n, nullptr,
+                     static_cast<cpx<T>*>(h->Z1), npts};
+      hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, pa);
+      DGFS_LAUNCH_CHECK("vss_prep");
+    }
+    t0.stop();
+  }
+
+  VssGainArgs<T> ga{};
+  ga.F = F;
+  ga.Ghat = static_cast<cpx<T>*>(h->Ghat);
+  ga.Z1 = static_cast<cpx<T>*>(h->Z1);
+  ga.Z2 = static_cast<cpx<T>*>(h->Z2);
+  ga.alpha = h->alpha;
+  ga.wgt = h->wgt;
+  ga.sz = h->szd;
+  for (int k = 0; k < h->npairs; ++k) {
+    ga.b[k] = static_cast<const cpx<T>*>(h->b[k]);
+    ga.sp[k] = h->sp[k];
+    ga.sq[k] = h->sq[k];
+    ga.mu1[k] = h->mu1[k];
+    ga.mu2[k] = h->mu2[k];
+  }
+  ga.nitems = static_cast<int>(items);
+  ga.npts = npts;
+  ga.ngroups = groups;
+  ga.ppg = ppg;
+  ga.Nrho = h->Nrho;
+  ga.M = h->M;
+  {
+    KTimerT<VssHandle> t1(h, 1, st);
+    hipLaunchKernelGGL((k_vss_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
+                       dim3(Shape<N>::NT), 0, st, ga);
+    DGFS_LAUNCH_CHECK("vss_gain");
+    t1.stop();
+  }
+
+  VssFinArgs<T> fa{};
+  fa.Ghat = static_cast<const cpx<T>*>(h->Ghat);
+  fa.F = F;
+  for (int sp = 0; sp < h->nspcs; ++sp) {
+    fa.f[sp] = fs[sp];
+    fa.Q[sp] = Qs[sp];
+  }
+  fa.Z = static_cast<cpx<T>*>(h->Z1);
+  for (int k = 0; k < h->npairs; ++k) {
+    fa.c[k] = static_cast<const cpx<T>*>(h->c[k]);
+    fa.prefac[k] = h->prefac[k];
+  }
+  fa.nspcs = h->nspcs;
+  fa.npts = npts;
+  fa.ngroups = groups;
+  {
+    KTimerT<VssHandle> t2(h, 2, st);
+    hipLaunchKernelGGL((k_vss_finalize<T, N>), dim3(std::min(h->nspcs * npts, slots)), dim3(Shape<N>::NT), 0,
+                       st, fa);
+    DGFS_LAUNCH_CHECK("vss_finalize");
+    t2.stop();
+  }
+
+  if (h->project) {
+    VssProjArgs pr{};
+    for (int sp = 0; sp < h->nspcs; ++sp) pr.Q[sp] = Qs[sp];
+    std::memcpy(pr.Ainv, h->Ainv, sizeof(pr.Ainv));
+    for (int s = 0; s < h->nspcs; ++s) pr.m[s] = h->masses[s];
+    pr.L = h->L;
+    pr.cw = h->cw;
+    pr.N = N;
+    pr.nspcs = h->nspcs;
+    pr.npts = npts;
+    KTimerT<VssHandle> t3(h, 3, st);
+    hipLaunchKernelGGL(k_vss_project, dim3(std::min(npts, 4 * cu_count())), dim3(256), 0, st, pr);
+    DGFS_LAUNCH_CHECK("vss_project");
+    t3.stop();
+  }
+  return DGFS_OK;
+}
+
+// f[s], Q[s]: per-species registers (AoSoA) or contiguous [npts][n] arrays
+template <typename T, int N>
+static int vss_eval_impl(VssHandle* h, const double* const* f, double* const* Q, bool aosoa, int nupts,
+                         int ldim, int vsize, int e0, int ne, int npts, hipStream_t st) {
+  constexpr int n = N * N * N;
+  const int S = h->nspcs;
+  if (aosoa) {
+    const int total = nupts * ne;
+    if (total == 0) return DGFS_OK;
+    const int per_chunk_elems = std::max(1, vss_chunk_points(h, total) / std::max(nupts, 1));
+    for (int c0 = 0; c0 < ne; c0 += per_chunk_elems) {
+      const int cne = std::min(per_chunk_elems, ne - c0);
+      const int pts = cne * nupts;
+      int rc = vss_ensure_ws<T, N>(h, pts);
+      if (rc) return rc;
+      for (int s = 0; s < S; ++s) {
+        rc = launch_aosoa_gather(f[s], h->fc + static_cast<size_t>(s) * pts * n, nupts, ldim, vsize, e0 + c0,
+                                 cne, st);
+        if (rc) return rc;
+      }
+      const double* fs[kVssMaxSpecies];
+      double* Qs[kVssMaxSpecies];
+      for (int s = 0; s < S; ++s) {
+        fs[s] = h->fc + static_cast<size_t>(s) * pts * n;
+        Qs[s] = h->Qc + static_cast<size_t>(s) * pts * n;
+      }
+      rc = vss_run_chunk<T, N>(h, fs, Qs, pts, st);
+      if (rc) return rc;
+      for (int s = 0; s < S; ++s) {
+        rc = launch_aosoa_scatter(h->Qc + static_cast<size_t>(s) * pts * n, Q[s], nupts, ldim, vsize, e0 + c0,
+                                  cne, st);
+        if (rc) return rc;
+      }
+    }
+    return DGFS_OK;
+  }
+  const int chunk = vss_chunk_points(h, npts);
+  for (int p0 = 0; p0 < npts; p0 += chunk) {
+    const int pts = std::min(chunk, npts - p0);
+    int rc = vss_ensure_ws<T, N>(h, pts);
+    if (rc) return rc;
+    const double* fs[kVssMaxSpecies];
+    double* Qs[kVssMaxSpecies];
+    for (int s = 0; s < S; ++s) {
+      fs[s] = f[s] + static_cast<size_t>(p0) * n;
+      Qs[s] = Q[s] + static_cast<size_t>(p0) * n;
+    }
+    rc = vss_run_chunk<T, N>(h, fs, Qs, pts, st);
+    if (rc) return rc;
+  }
+  return DGFS_OK;
+}
+
+template <typename T>
+static int vss_dispatch(VssHandle* h, const double* const* f, double* const* Q, bool aosoa, int nupts, int ldim,
+                        int vsize, int e0, int ne, int npts, hipStream_t st) {
+  switch (h->N) {
+#define DGFS_VSS_CASE(NV) \
+  case NV:                \
+    return vss_eval_impl<T, NV>(h, f, Q, aosoa, nupts, ldim, vsize, e0, ne, npts, st);
+    DGFS_VSS_CASE(4) DGFS_VSS_CASE(6) DGFS_VSS_CASE(8) DGFS_VSS_CASE(12) DGFS_VSS_CASE(16) DGFS_VSS_CASE(24)
+    DGFS_VSS_CASE(32)
+#undef DGFS_VSS_CASE
+    default:
+      return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d is not built", h->N);
+  }
+}
+
+static VssHandle* as_vss(void* p) {
+  auto* h = static_cast<VssHandle*>(p);
+  if (!h || h->magic != 0x76737331) return nullptr;
+  return h;
+}
+
+template <typename T>
+static int vss_build_tables(VssHandle* h, const dgfs_vss_desc* d) {
+  const int n = h->n, N = h->N, K = h->Nrho * h->M, Mpre = d->Mpre;
+  const double swpre = 4.0 * 3.14159265358979323846 / Mpre;
+  double *qz = nullptr, *qw = nullptr, *szpre = nullptr, *Byj = nullptr, *Bsum = nullptr;
+  struct Freer {
+    double** ptrs[5];
+    ~Freer() {
+      for (auto p : ptrs) (void)hipFree(*p);
+    }
+  } freer{{&qz, &qw, &szpre, &Byj, &Bsum}};
+  DGFS_HIP_TRY(hipMalloc(&qz, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&qw, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&szpre, static_cast<size_t>(Mpre) * 3 * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&Byj, static_cast<size_t>(h->M) * Mpre * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&Bsum, static_cast<size_t>(Mpre) * sizeof(double)));
+  DGFS_HIP_TRY(hipMemcpy(qz, d->qz, h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(qw, d->qw, h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(szpre, d->szpre, static_cast<size_t>(Mpre) * 3 * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> B(static_cast<size_t>(h->M) * Mpre), Bs(Mpre);
+  for (int k = 0; k < h->npairs; ++k) {
+    const int p = h->sp[k], q = h->sq[k];
+    const double eta = d->eta[p * h->nspcs + q], gamma = d->gamma[p * h->nspcs + q];
+    std::fill(Bs.begin(), Bs.end(), 0.0);
+    for (int y = 0; y < h->M; ++y)
+      for (int j = 0; j < Mpre; ++j) {
+        const double dot = d->sz[3 * y] * d->szpre[3 * j] + d->sz[3 * y + 1] * d->szpre[3 * j + 1] +
+                           d->sz[3 * y + 2] * d->szpre[3 * j + 2];
+        // (1 + s.s')^eta; the base is clamped at 0 where rounding of an
+        // antipodal pair makes it a tiny negative number
+        const double v = std::pow(std::max(0.0, 1.0 + dot), eta);
+        B[static_cast<size_t>(y) * Mpre + j] = v;
+        Bs[j] += v;
+      }
+    DGFS_HIP_TRY(hipMemcpy(Byj, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+    DGFS_HIP_TRY(hipMemcpy(Bsum, Bs.data(), Bs.size() * sizeof(double), hipMemcpyHostToDevice));
+    DGFS_HIP_TRY(hipMalloc(&h->b[k], static_cast<size_t>(K) * n * sizeof(cpx<T>)));
+    DGFS_HIP_TRY(hipMalloc(&h->c[k], static_cast<size_t>(n) * sizeof(cpx<T>)));
+    const double mpq = h->masses[q] / (h->masses[p] + h->masses[q]);
+    hipLaunchKernelGGL(k_vss_btab<T>, dim3((n + 255) / 256, K), dim3(256), 0, 0, static_cast<cpx<T>*>(h->b[k]),
+                       Byj, szpre, qz, h->L, gamma, mpq, swpre, N, h->M, Mpre);
+    DGFS_LAUNCH_CHECK("vss_btab");
+    hipLaunchKernelGGL(k_vss_ctab<T>, dim3((n + 255) / 256), dim3(256), 0, 0, static_cast<cpx<T>*>(h->c[k]), Bsum,
+                       szpre, qz, qw, h->L, gamma, h->sw, swpre, N, h->Nrho, Mpre);
+    DGFS_LAUNCH_CHECK("vss_ctab");
+    // the host arrays are reused for the next pair
+    DGFS_HIP_TRY(hipDeviceSynchronize());
+  }
+  return DGFS_OK;
+}
+
+}  // namespace dgfs
+
+using namespace dgfs;
+
+extern "C" int dgfs_vss_create(const dgfs_vss_desc* d, void** out) {
+  if (!d || !out) return fail(DGFS_ERR_INVALID_VALUE, "null descriptor or handle pointer");
+  *out = nullptr;
+  switch (d->Nv) {
+    case 4: case 6: case 8: case 12: case 16: case 24: case 32: break;
+    default: return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d unsupported (built: 4 6 8 12 16 24 32)", d->Nv);
+  }
+  if (d->nspcs < 1 || d->nspcs > kVssMaxSpecies)
+    return fail(DGFS_ERR_INVALID_VALUE, "nspcs must be 1..%d (got %d)", kVssMaxSpecies, d->nspcs);
+  if (d->Nrho < 1 || d->M < 1 || d->Mpre < 1)
+    return fail(DGFS_ERR_INVALID_VALUE, "Nrho, M and Mpre must be >= 1");
+  if (!d->qz || !d->qw || !d->sz || !d->szpre || !d->masses || !d->gamma || !d->eta || !d->prefac)
+    return fail(DGFS_ERR_INVALID_VALUE, "null table pointer in the descriptor");
+  if (d->precision != 64 && d->precision != 32)
+    return fail(DGFS_ERR_INVALID_VALUE, "precision must be 64 or 32 (got %d)", d->precision);
+  if (!(d->L > 0) || !(d->cw > 0) || !(d->sw > 0)) return fail(DGFS_ERR_INVALID_VALUE, "L, cw and sw must be positive");
+  for (int s = 0; s < d->nspcs; ++s)
+    if (!(d->masses[s] > 0)) return fail(DGFS_ERR_INVALID_VALUE, "species masses must be positive");
+
+  auto h = std::make_unique<VssHandle>();
+  h->N = d->Nv;
+  h->n = d->Nv * d->Nv * d->Nv;
+  h->Nrho = d->Nrho;
+  h->M = d->M;
+  h->Mpre = d->Mpre;
+  h->nspcs = d->nspcs;
+  h->npairs = d->nspcs * d->nspcs;
+  h->prec = d->precision;
+  h->project = d->project ? 1 : 0;
+  h->L = d->L;
+  h->sw = d->sw;
+  h->cw = d->cw;
+  h->tsize = (d->precision == 64) ? 8 : 4;
+  for (int s = 0; s < h->nspcs; ++s) h->masses[s] = d->masses[s];
+  for (int p = 0; p < h->nspcs; ++p)
+    for (int q = 0; q < h->nspcs; ++q) {
+      const int k = p * h->nspcs + q;
+      h->sp[k] = p;
+      h->sq[k] = q;
+      h->prefac[k] = d->prefac[k];
+      h->mu1[k] = h->masses[q] / (h->masses[p] + h->masses[q]);
+      h->mu2[k] = h->masses[p] / (h->masses[p] + h->masses[q]);
+    }
+  const double pi = 3.14159265358979323846;
+  std::vector<double> alpha(h->Nrho), wgt(h->Nrho);
+  for (int x = 0; x < h->Nrho; ++x) {
+    alpha[x] = pi / h->L * d->qz[x];
+    wgt[x] = d->qw[x] * h->sw / static_cast<double>(h->n);
+  }
+  DGFS_HIP_TRY(hipMalloc(&h->alpha, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->wgt, h->Nrho * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->szd, h->M * 3 * sizeof(double)));
+  DGFS_HIP_TRY(hipMemcpy(h->alpha, alpha.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->wgt, wgt.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->szd, d->sz, h->M * 3 * sizeof(double), hipMemcpyHostToDevice));
+  const int rc = (h->prec == 64) ? vss_build_tables<double>(h.get(), d) : vss_build_tables<float>(h.get(), d);
+  if (rc) return rc;
+
+  // Gram matrix of the invariants: rows mass_s (s < S), then m_s c_x, m_s c_y,
+  // m_s c_z and m_s |c|^2 over all species blocks; A_jk = sum phi_j phi_k cw^2
+  const int S = h->nspcs, nc = S + 4, N = h->N;
+  std::vector<double> A(static_cast<size_t>(nc) * nc, 0.0);
+  const double hh = 2.0 * h->L / N, c0 = -h->L + 0.5 * hh;
+  double mom[5][5] = {};
+  for (int r = 0; r < h->n; ++r) {
+    const int I = r / (N * N), J = (r / N) % N, K = r % N;
+    const double cx = c0 + I * hh, cy = c0 + J * hh, cz = c0 + K * hh;
+    const double phi[5] = {1.0, cx, cy, cz, cx * cx + cy * cy + cz * cz};
+    for (int j = 0; j < 5; ++j)
+      for (int k = 0; k < 5; ++k) mom[j][k] += phi[j] * phi[k];
+  }
+  const double w2 = h->cw * h->cw;
+  for (int s = 0; s < S; ++s) {
+    const double m = h->masses[s];
+    A[s * nc + s] += mom[0][0] * w2;
+    for (int j = 0; j < 4; ++j) {
+      A[s * nc + S + j] += m * mom[0][1 + j] * w2;
+      A[(S + j) * nc + s] += m * mom[0][1 + j] * w2;
+      for (int k = 0; k < 4; ++k) A[(S + j) * nc + S + k] += m * m * mom[1 + j][1 + k] * w2;
+    }
+  }
+  if (invert_n(A.data(), h->Ainv, nc)) return fail(DGFS_ERR_INVALID_VALUE, "singular invariant Gram matrix");
+  *out = h.release();
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_destroy(void* p) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  (void)hipDeviceSynchronize();
+  resolve_timing(h);
+  h->magic = 0;
+  delete h;
+  return DGFS_OK;
+}
+
+static int vss_check_ptrs(const VssHandle* h, const double* const* f, double* const* Q) {
+  if (!f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null species pointer array");
+  for (int s = 0; s < h->nspcs; ++s) {
+    if (!f[s] || !Q[s]) return fail(DGFS_ERR_INVALID_VALUE, "null distribution pointer of species %d", s);
+    for (int t = 0; t < h->nspcs; ++t)
+      if (static_cast<const void*>(f[s]) == static_cast<const void*>(Q[t]))
+        return fail(DGFS_ERR_INVALID_VALUE, "f and Q may not alias (species %d / %d)", s, t);
+  }
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_eval(void* p, const double* const* f, double* const* Q, int nupts, int ldim, int vsize,
+                             int neles, int e0, int ne, void* stream) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  int rc = vss_check_ptrs(h, f, Q);
+  if (rc) return rc;
+  if (vsize != h->n) return fail(DGFS_ERR_INVALID_VALUE, "vsize %d != Nv^3 = %d", vsize, h->n);
+  if (nupts < 1 || neles < 0 || e0 < 0 || ne < 0 || e0 + ne > neles)
+    return fail(DGFS_ERR_INVALID_VALUE, "bad element range e0=%d ne=%d neles=%d nupts=%d", e0, ne, neles, nupts);
+  const long need = static_cast<long>((neles + 31) / 32) * vsize * 32;
+  if (ldim < need) return fail(DGFS_ERR_INVALID_VALUE, "ldim %d < %ld (AoSoA row of %d elements)", ldim, need, neles);
+  auto st = static_cast<hipStream_t>(stream);
+  return (h->prec == 64) ? vss_dispatch<double>(h, f, Q, true, nupts, ldim, vsize, e0, ne, 0, st)
+                         : vss_dispatch<float>(h, f, Q, true, nupts, ldim, vsize, e0, ne, 0, st);
+}
+
+extern "C" int dgfs_vss_eval_contig(void* p, const double* const* f, double* const* Q, int npts, void* stream) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  int rc = vss_check_ptrs(h, f, Q);
+  if (rc) return rc;
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  if (npts == 0) return DGFS_OK;
+  auto st = static_cast<hipStream_t>(stream);
+  return (h->prec == 64) ? vss_dispatch<double>(h, f, Q, false, 0, 0, h->n, 0, 0, npts, st)
+                         : vss_dispatch<float>(h, f, Q, false, 0, 0, h->n, 0, 0, npts, st);
+}
+
+extern "C" int dgfs_vss_get_config(void* p, int* ngroups, int* nslots, int* threads) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (ngroups) *ngroups = h->last_groups;
+  if (nslots) *nslots = h->last_slots;
+  if (threads) *threads = h->threads;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (ngroups < 0 || ngroups > h->Nrho) return fail(DGFS_ERR_INVALID_VALUE, "ngroups must be 0..Nrho");
+  h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_set_timing(void* p, int on) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  resolve_timing(h);
+  h->timing = on ? 1 : 0;
+  for (int k = 0; k < 5; ++k) {
+    h->acc_ms[k] = 0.0;
+    h->acc_n[k] = 0;
+  }
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_get_timing(void* p, double* ms, int* counts) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (!ms || !counts) return fail(DGFS_ERR_INVALID_VALUE, "null output");
+  resolve_timing(h);
+  for (int k = 0; k < 5; ++k) {
+    ms[k] = h->acc_ms[k];
+    counts[k] = h->acc_n[k];
+  }
+  return DGFS_OK;
+}

@@ -60,6 +60,18 @@ class dgfs_vhs_desc(C.Structure):
     ]
 
 
+class dgfs_vss_desc(C.Structure):
+    _fields_ = [
+        ("Nv", C.c_int), ("Nrho", C.c_int), ("M", C.c_int), ("Mpre", C.c_int),
+        ("nspcs", C.c_int), ("precision", C.c_int), ("project", C.c_int),
+        ("L", C.c_double), ("sw", C.c_double), ("cw", C.c_double),
+        ("qz", C.POINTER(C.c_double)), ("qw", C.POINTER(C.c_double)),
+        ("sz", C.POINTER(C.c_double)), ("szpre", C.POINTER(C.c_double)),
+        ("masses", C.POINTER(C.c_double)), ("gamma", C.POINTER(C.c_double)),
+        ("eta", C.POINTER(C.c_double)), ("prefac", C.POINTER(C.c_double)),
+    ]
+
+
 _lib = None
 
 _P, _I, _D = C.c_void_p, C.c_int, C.c_double

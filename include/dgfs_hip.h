@@ -105,6 +105,61 @@ int dgfs_vhs_set_timing(void* handle, int on);
 int dgfs_vhs_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);
 
 /* ------------------------------------------------------------------------
+ * Multi-species fast-spectral VSS collision operator
+ * Q_p = sum_q Q_pq(f_p, f_q), up to 4 species.
+ *
+ * Replaces the reference's DGFSBiVSSGLLScatteringModel
+ * (frfs/solvers/dgfsbi/scattering.py:46-330): perform_precomputation
+ * (:96-257, tables of vss-gll.mako:31-95) becomes dgfs_vss_create;
+ * fs(elem, upt, *args) (:323-330), i.e. Qij (:259-320) over every ordered
+ * pair, becomes dgfs_vss_eval over a range of elements.
+ * ---------------------------------------------------------------------- */
+typedef struct dgfs_vss_desc {
+  int Nv;             /* velocity points per axis */
+  int Nrho;           /* radial Gauss points (dgfsbi/velocitymesh.py:47) */
+  int M;              /* full-sphere design points (dgfsbi/velocitymesh.py:131-137) */
+  int Mpre;           /* design points of the b/c precomputation (scattering.py:53-56) */
+  int nspcs;          /* species, 1..4 */
+  int precision;      /* 64 or 32 */
+  int project;        /* 1: project onto species masses, total momentum, total energy */
+  double L;           /* half-width of the velocity box */
+  double sw;          /* 4 pi / M */
+  double cw;          /* (2L/Nv)^3 */
+  const double* qz;   /* [Nrho] radial nodes on [0, R] */
+  const double* qw;   /* [Nrho] radial weights */
+  const double* sz;   /* [M][3] full-sphere directions */
+  const double* szpre;/* [Mpre][3] precomputation directions */
+  const double* masses; /* [nspcs] nondimensional molecular masses */
+  const double* gamma;  /* [nspcs][nspcs] 2 (1 - omega_pq) (scattering.py:66-67) */
+  const double* eta;    /* [nspcs][nspcs] alpha_pq - 1 (scattering.py:63-64) */
+  const double* prefac; /* [nspcs][nspcs] (scattering.py:79-86) */
+} dgfs_vss_desc;
+
+/* Build b_pq (Nrho*M*Nv^3 complex per pair) and c_pq on the device.
+ * Replaces DGFSBiVSSGLLScatteringModel.perform_precomputation
+ * (frfs/solvers/dgfsbi/scattering.py:96-257). */
+int dgfs_vss_create(const dgfs_vss_desc* desc, void** handle);
+
+/* Q of every species at every solution point of elements [e0, e0+ne).
+ * f[s], Q[s]: host arrays of nspcs device pointers to the species' AoSoA
+ * registers (same layout as dgfs_vhs_eval); Q[p] is overwritten with
+ * sum_q Q_pq.  Replaces the per-point loop over DGFSBiVSSGLLScatteringModel.fs
+ * (frfs/solvers/dgfsbi/scattering.py:323-330, called from
+ * frfs/solvers/dgfsbi/system.py:239-251). */
+int dgfs_vss_eval(void* handle, const double* const* f, double* const* Q, int nupts, int ldim,
+                  int vsize, int neles, int e0, int ne, void* stream);
+
+/* Same for npts points stored contiguously per species, f[s][pt*vsize + r]. */
+int dgfs_vss_eval_contig(void* handle, const double* const* f, double* const* Q, int npts,
+                         void* stream);
+int dgfs_vss_destroy(void* handle);
+int dgfs_vss_get_config(void* handle, int* ngroups, int* nslots, int* threads);
+int dgfs_vss_set_groups(void* handle, int ngroups);
+/* Kernel classes {prep, gain, finalize, project, unused}. */
+int dgfs_vss_set_timing(void* handle, int on);
+int dgfs_vss_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);
+
+/* ------------------------------------------------------------------------
  * Flux-reconstruction advection of the velocity columns and RK stage kernels.
  * Matrices are the reference's AoSoA layout: element (row, var v, elem e) at
  * row*ld + ((e/32)*nvars + v)*32 + e%32; velocities are the variables.

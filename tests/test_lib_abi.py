@@ -35,6 +35,7 @@ def test_library_exports_all_declared_symbols():
 def test_binding_covers_declared_symbols():
     from dgfs_hip import _lib
     from dgfs_hip import fr  # noqa: F401  registers the FR entry points
+    from dgfs_hip.solvers.dgfsbi import scattering  # noqa: F401  registers the VSS entry points
     assert declared_symbols() <= set(_lib.exported_symbols())
 
 
@@ -73,3 +74,29 @@ def test_fr_entry_points_validate_on_host():
             c()
     with pytest.raises(_lib.DGFSInvalidValue):
         _lib.call("frfs_axnpby", 10, 0, None, None, None)
+
+
+def test_vss_entry_points_validate_on_host():
+    """The multi-species operator rejects bad descriptors and handles on the
+    host (no GPU needed)."""
+    import ctypes as C
+    import numpy as np
+    from dgfs_hip import _lib
+    from dgfs_hip.solvers.dgfsbi import scattering  # noqa: F401
+    lib = _lib.load()
+    assert lib.dgfs_vss_create(None, None) == 1
+    a = np.zeros(64)
+    dp = C.POINTER(C.c_double)
+    ptr = a.ctypes.data_as(dp)
+    for bad in (dict(nspcs=0), dict(nspcs=5), dict(Nv=10), dict(precision=16), dict(Mpre=0)):
+        kw = dict(Nv=8, Nrho=2, M=6, Mpre=12, nspcs=2, precision=64, project=0, L=5.0,
+                  sw=4 * np.pi / 6, cw=1.0, qz=ptr, qw=ptr, sz=ptr, szpre=ptr, masses=ptr,
+                  gamma=ptr, eta=ptr, prefac=ptr)
+        kw.update(bad)
+        d = _lib.dgfs_vss_desc(**kw)
+        h = C.c_void_p()
+        st = lib.dgfs_vss_create(C.byref(d), C.byref(h))
+        assert st in (1, 2), (bad, st)
+        assert not h.value
+    assert lib.dgfs_vss_destroy(None) == 5
+    assert lib.dgfs_vss_eval_contig(None, None, None, 1, None) == 5

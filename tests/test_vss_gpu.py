@@ -1,0 +1,180 @@
+"""GPU parity of the multi-species VSS collision operator against the FP64
+oracle (oracle/vss.py) and the committed golden vectors.  Tolerances: FP64
+path <= 1e-10 relative L2 per species; FP32 path <= 1e-3; after the FP64
+projection the species masses, total momentum and total energy of Q vanish
+to <= 1e-12 of int |Q|."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import vss as V
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+PAIRS = [(0, 0), (0, 1), (1, 0), (1, 1)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_op(N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac, precision=64, project=False):
+    from dgfs_hip.solvers.dgfsbi.scattering import VSSOperator
+    from dgfs_hip.solvers.dgfsbi.velocitymesh import DGFSBiVelocityMesh
+    from dgfs_hip.sphericaldesign import get_sphquadrule
+    vm = DGFSBiVelocityMesh.from_params(None, N, L, Nrho, M, masses)
+    S = len(masses)
+    gamma = np.array([2.0 * (1.0 - w) for w in omegas]).reshape(S, S)
+    eta = np.full((S, S), alpha - 1.0)
+    szpre = get_sphquadrule("symmetric", npts=Mpre).pts
+    op = VSSOperator(N, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), vm.sz(), szpre,
+                     vm.masses(), gamma, eta, np.reshape(prefac, (S, S)), precision=precision,
+                     project=project)
+    return op, vm
+
+
+def run_contig(op, fs, device):
+    import torch
+    ft = [torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=device) for f in fs]
+    Qt = [torch.full_like(f, float("nan")) for f in ft]
+    op.eval_contig([f.data_ptr() for f in ft], [q.data_ptr() for q in Qt], ft[0].shape[0],
+                   torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return [q.cpu().numpy() for q in Qt]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_q(N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac, fs):
+    vm = V.BiVelocityMesh(N, L, Nrho, M, masses)
+    g = {k: 2.0 * (1.0 - w) for k, w in zip(PAIRS, omegas)}
+    eta = {k: alpha - 1.0 for k in PAIRS}
+    orc = V.VSSOracle(vm, g, eta, dict(zip(PAIRS, prefac)), V.full_sphere_design(Mpre))
+    return orc.q(fs), vm
+
+
+def bimodal_states(vm, npts, seed):
+    rng = np.random.default_rng(seed)
+    out = [[], []]
+    for _ in range(npts):
+        for p in range(2):
+            u1, u2 = rng.uniform(-0.6, 0.6, 3), rng.uniform(-0.6, 0.6, 3)
+            t1, t2 = rng.uniform(0.6, 1.8, 2)
+            out[p].append(0.5 * V.species_maxwellian(vm, p, 1.0, u1, t1)
+                          + 0.5 * V.species_maxwellian(vm, p, 0.6, u2, t2))
+    return [np.array(o) for o in out]
+
+
+@pytest.mark.parametrize("fn", sorted(f for f in os.listdir(GOLD) if f.startswith("vss_") and f.endswith(".npz")))
+def test_golden_vectors_fp64(fn, device):
+    d = np.load(os.path.join(GOLD, fn))
+    N, Nrho, M, Mpre = (int(x) for x in d["dims"])
+    say("golden", fn, N, Nrho, M, Mpre)
+    op, _ = make_op(N, float(d["L"]), Nrho, M, Mpre, tuple(d["masses"]), float(d["alpha"]),
+                    tuple(d["omegas"]), tuple(d["prefac"]))
+    Q0, Q1 = run_contig(op, [d["f0"], d["f1"]], device)
+    e0, e1 = rel(Q0, d["Q0"]), rel(Q1, d["Q1"])
+    say("  rel L2", e0, e1)
+    assert e0 < 1e-10 and e1 < 1e-10
+
+
+CFG16 = (16, 5.0, 8, 12, 24, (1.0, 2.1), 1.0, (0.81, 0.805, 0.805, 0.80), (1.0, 0.9, 0.9, 0.8))
+
+
+def test_oracle_parity_fp64_fp32_16(device):
+    N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac = CFG16
+    vm = V.BiVelocityMesh(N, L, Nrho, M, masses)
+    fs = bimodal_states(vm, 3, seed=5)
+    Qref, _ = oracle_q(*CFG16, fs)
+    say("vss oracle parity 16^3")
+    op64, _ = make_op(*CFG16, precision=64)
+    Q64 = run_contig(op64, fs, device)
+    e64 = [rel(Q64[p], Qref[p]) for p in range(2)]
+    say("  fp64", e64)
+    assert max(e64) < 1e-10
+    op32, _ = make_op(*CFG16, precision=32)
+    Q32 = run_contig(op32, fs, device)
+    e32 = [rel(Q32[p], Qref[p]) for p in range(2)]
+    say("  fp32", e32)
+    assert max(e32) < 1e-3
+
+
+def test_group_split_and_aosoa_path(device):
+    import torch
+    N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac = 8, 5.0, 4, 12, 24, (1.0, 2.1), 1.2, \
+        (0.81, 0.805, 0.805, 0.80), (1.0, 1.0, 1.0, 1.0)
+    op, vm = make_op(N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac)
+    ov = V.BiVelocityMesh(N, L, Nrho, M, masses)
+    nupts, neles = 4, 37
+    fs = bimodal_states(ov, nupts * neles, seed=11)
+    op.set_groups(1)
+    Qa = run_contig(op, fs, device)
+    op.set_groups(Nrho)
+    Qb = run_contig(op, fs, device)
+    say("group split", [rel(Qa[p], Qb[p]) for p in range(2)])
+    assert all(rel(Qa[p], Qb[p]) < 1e-13 for p in range(2))
+    op.set_groups(0)
+    # AoSoA registers: point pt = (elem - e0) * nupts + upt
+    n = N ** 3
+    ldim = ((neles + 31) // 32) * n * 32
+    regs, outs = [], []
+    for p in range(2):
+        R = torch.zeros((nupts, ldim), dtype=torch.float64, device=device)
+        for e in range(neles):
+            for u in range(nupts):
+                col = ((e // 32) * n + np.arange(n)) * 32 + e % 32
+                R[u, torch.as_tensor(col, device=device)] = torch.as_tensor(fs[p][e * nupts + u], device=device)
+        regs.append(R)
+        outs.append(torch.full_like(R, 7.0))
+    e0, ne = 3, 30
+    op.eval_aosoa([r.data_ptr() for r in regs], [o.data_ptr() for o in outs], nupts, ldim, n, neles, e0, ne,
+                  torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    Qc = run_contig(op, fs, device)
+    for p in range(2):
+        O = outs[p].cpu().numpy()
+        for e in range(neles):
+            for u in range(nupts):
+                col = ((e // 32) * n + np.arange(n)) * 32 + e % 32
+                if e0 <= e < e0 + ne:
+                    assert rel(O[u, col], Qc[p][e * nupts + u]) < 1e-14
+                else:
+                    assert np.all(O[u, col] == 7.0)
+
+
+def test_projection_conserves_mixture_invariants(device):
+    N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac = CFG16
+    vm = V.BiVelocityMesh(N, L, Nrho, M, masses)
+    fs = bimodal_states(vm, 4, seed=9)
+    op64p, _ = make_op(*CFG16, precision=64, project=True)
+    op32p, _ = make_op(*CFG16, precision=32, project=True)
+    Q64 = run_contig(op64p, fs, device)
+    Q32 = run_contig(op32p, fs, device)
+    cv, cw, m = vm.cv, vm.cw, vm.masses
+    for Q in (Q64, Q32):
+        scale = sum(np.abs(Q[p]).sum(1) for p in range(2)) * cw
+        for p in range(2):
+            assert np.max(np.abs(Q[p].sum(1) * cw) / scale) < 1e-12
+        mom = sum(m[p] * (Q[p] @ cv.T) for p in range(2)) * cw
+        en = sum(m[p] * (Q[p] @ (cv ** 2).sum(0)) for p in range(2)) * cw
+        assert np.max(np.abs(mom) / scale[:, None]) < 1e-12
+        assert np.max(np.abs(en) / scale) < 1e-12
+    e = [rel(Q32[p], Q64[p]) for p in range(2)]
+    say("fp32 vs fp64 (projected)", e)
+    assert max(e) < 1e-3
+
+
+def test_bad_arguments_raise(device):
+    import torch
+    from dgfs_hip import _lib
+    op, _ = make_op(4, 5.0, 2, 6, 12, (1.0, 2.1), 1.0, (0.8, 0.8, 0.8, 0.8), (1, 1, 1, 1))
+    f = torch.zeros((2, 64), dtype=torch.float64, device=device)
+    with pytest.raises(_lib.DGFSInvalidValue):
+        op.eval_contig([f.data_ptr(), f.data_ptr()], [f.data_ptr(), f.data_ptr()], 2)
+    with pytest.raises(ValueError):
+        op.eval_contig([f.data_ptr()], [f.data_ptr()], 2)
