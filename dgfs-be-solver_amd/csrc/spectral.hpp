@@ -125,6 +125,13 @@ int launch_aosoa_gather(const double* f, double* out, int nupts, int ldim, int v
 int launch_aosoa_scatter(const double* in, double* Q, int nupts, int ldim, int vsize, int e0, int ne,
                          hipStream_t st);
 
+// FP64 minimum-norm projection of Q[npts][N^3] onto the five collision
+// invariants {1, c, |c|^2} (grid [-L, L]^3, weight cw); Ainv25 from
+// invariant_gram_inverse.
+int invariant_gram_inverse(int N, double L, double cw, double* Ainv25);
+int launch_invariant_projection(double* Q, const double* Ainv25, double L, double cw, int N, int npts,
+                                hipStream_t st);
+
 // Per-kernel timing: event pairs recorded around every launch on the launch
 // stream; resolved (and synchronised) only when queried.  H provides
 // `timing`, `pool`, `pending`, `acc_ms[]` and `acc_n[]`.

@@ -1,0 +1,1069 @@
+// Learned collision surrogate for gfx950: Hermite-mode encoder/decoder of
+// velocity distributions, a 4-layer spectral network (forward, backward and
+// Adam in HIP, GEMMs on MFMA with BF16 operands and FP32 accumulation), and
+// the surrogate-with-fallback evaluation of Q(f).
+//
+// Representation.  A point's distribution f on the Nv^3 grid is described in
+// the frame of its own Maxwellian M[f] (density rho, velocity u, temperature
+// T from the discrete moments; DGFS convention M = rho (pi T)^-3/2
+// exp(-|c-u|^2/T), so the per-axis variance is s^2 = T/2).  With
+// c' = (c - u)/s and the probabilists' Hermite polynomials He_n,
+//   a_n = (1/(n! rho)) sum_r f(r) He_n(c'(r)) cw,   n = (nx, ny, nz), n_d < 8,
+// the 8^3 retained modes of f (Grad's expansion f ~ M sum_n a_n He_n).  By
+// construction a_000 = 1 and the momentum / energy modes vanish.  The
+// collision operator's modes b_n (same formula with Q) scale as
+// prefac rho T^{gamma/2} in the continuum, so the network maps the
+// dimensionless a to b^ = b / (prefac rho T^{gamma/2}).
+//
+// Network (8^3 = 512 modes m, width 32 channels c, activations Z[m][p][c]):
+//   Z0[m][p][c] = P0[c] x[p][m] + P1[c] log T[p] + E[m][c]
+//   U_l = K_l Z_l            (mode mixing, K_l 512x512, one GEMM over all points)
+//   V_l = U_l W_l + Z_l R_l + bias_l     (channel mixing, W_l, R_l 32x32)
+//   Z_{l+1} = GELU(V_l) for l < 3, Z_4 = V_3
+//   out[p][m] = sum_c Z_4[m][p][c] Pout[c] + bout[m]
+// Fallback: points whose non-equilibrium indicator ||f - M[f]|| / ||f||
+// exceeds a threshold are evaluated by the exact spectral operator.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "spectral.hpp"
+
+namespace dgfs {
+
+constexpr int kSurD = 8;                    // Hermite degrees per axis
+constexpr int kSurModes = kSurD * kSurD * kSurD;
+constexpr int kSurWidth = 32;
+constexpr int kSurLayers = 4;
+constexpr long kSurLayerSize = static_cast<long>(kSurModes) * kSurModes + 2L * kSurWidth * kSurWidth + kSurWidth;
+constexpr long kSurOffP0 = 0, kSurOffP1 = kSurWidth, kSurOffE = 2 * kSurWidth;
+constexpr long kSurOffLayers = kSurOffE + static_cast<long>(kSurModes) * kSurWidth;
+constexpr long kSurOffPout = kSurOffLayers + kSurLayers * kSurLayerSize;
+constexpr long kSurOffBout = kSurOffPout + kSurWidth;
+constexpr long kSurParams = kSurOffBout + kSurModes;
+
+// ---------------------------------------------------------------------------
+// GEMM  C[M][N] = alpha op(A)[M][K] op(B)[K][N] + beta C, row-major FP32
+// storage; op(A)[m][k] = A[m*lda + k] or (transA) A[k*lda + m], likewise B.
+// BF16: operands rounded to bfloat16, v_mfma_f32_16x16x16bf16_1k; else FP32
+// operands on v_mfma_f32_16x16x4f32.  64x64 block tile, 4 waves of 32x32,
+// K staged through LDS 32 at a time.  Split-K writes raw partial products to
+// part[z][M][N] (reduced by k_gemm_reduce).
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  const float* A;
+  const float* B;
+  float* C;
+  float* part;
+  long lda, ldb, ldc;
+  int transA, transB, M, N, K, kchunk;
+  float alpha, beta;
+};
+
+typedef short sh4 __attribute__((ext_vector_type(4)));
+typedef float fl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<short>(u >> 16);
+}
+
+constexpr int kGT = 64, kGK = 32, kGPad = kGT + 4;
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ float As[kGK][kGPad];
+  __shared__ float Bs[kGK][kGPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kbeg = blockIdx.z * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
+  fl4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = fl4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = kbeg; k0 < kend; k0 += kGK) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i;
+      int mm, kk;
+      if (g.transA) { mm = e & 63; kk = e >> 6; } else { kk = e & 31; mm = e >> 5; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      float v = 0.f;
+      if (gm < g.M && gk < kend) v = g.transA ? g.A[static_cast<long>(gk) * g.lda + gm] : g.A[static_cast<long>(gm) * g.lda + gk];
+      As[kk][mm] = v;
+      int nn, kb;
+      if (g.transB) { kb = e & 31; nn = e >> 5; } else { nn = e & 63; kb = e >> 6; }
+      const int gn = n0 + nn, gkb = k0 + kb;
+      float w = 0.f;
+      if (gn < g.N && gkb < kend) w = g.transB ? g.B[static_cast<long>(gn) * g.ldb + gkb] : g.B[static_cast<long>(gkb) * g.ldb + gn];
+      Bs[kb][nn] = w;
+    }
+    __syncthreads();
+    if constexpr (BF16) {
+#pragma unroll
+      for (int s = 0; s < kGK / 16; ++s) {
+        sh4 av[2], bv[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            av[t][j] = to_bf16(As[s * 16 + 4 * lg + j][wm + t * 16 + li]);
+            bv[t][j] = to_bf16(Bs[s * 16 + 4 * lg + j][wn + t * 16 + li]);
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < kGK / 4; ++s) {
+        float av[2], bv[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          av[t] = As[s * 4 + lg][wm + t * 16 + li];
+          bv[t] = Bs[s * 4 + lg][wn + t * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + a * 16 + 4 * lg + r, col = n0 + wn + b * 16 + li;
+        if (row < g.M && col < g.N) {
+          if (g.part) {
+            g.part[(static_cast<long>(blockIdx.z) * g.M + row) * g.N + col] = acc[a][b][r];
+          } else {
+            float* cp = g.C + static_cast<long>(row) * g.ldc + col;
+            const float v = g.alpha * acc[a][b][r];
+            *cp = (g.beta == 0.f) ? v : v + g.beta * *cp;
+          }
+        }
+      }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_reduce(const float* __restrict__ part, int splits, int M, int N,
+                                                     float alpha, float beta, float* C, long ldc) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<long>(M) * N) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[static_cast<long>(z) * M * N + i];
+  const int row = static_cast<int>(i / N), col = static_cast<int>(i % N);
+  float* cp = C + static_cast<long>(row) * ldc + col;
+  *cp = (beta == 0.f) ? alpha * s : alpha * s + beta * *cp;
+}
+
+// split-K partials: at most 256 / tiles splits of an M x N output covering
+// `tiles` 64x64 tiles, so splits * M * N <= 256 * 64 * 64 floats; allocated
+// once at that size and never resized while work may be in flight
+constexpr size_t kGemmPartFloats = 256u * kGT * kGT;
+struct GemmWs {
+  float* part = nullptr;
+  int init() {
+    if (!part) DGFS_HIP_TRY(hipMalloc(&part, kGemmPartFloats * sizeof(float)));
+    return DGFS_OK;
+  }
+  ~GemmWs() { (void)hipFree(part); }
+};
+
+static int gemm(GemmWs* ws, int transA, int transB, int M, int N, int K, float alpha, const float* A, long lda,
+                const float* B, long ldb, float beta, float* C, long ldc, bool bf16, hipStream_t st) {
+  if (M <= 0 || N <= 0) return DGFS_OK;
+  GemmArgs g{A, B, C, nullptr, lda, ldb, ldc, transA, transB, M, N, K, K, alpha, beta};
+  const int gx = (N + kGT - 1) / kGT, gy = (M + kGT - 1) / kGT;
+  // split the reduction when the output has few tiles and K is long
+  int splits = 1;
+  const int tiles = gx * gy;
+  if (tiles < 256 && K > 4096) splits = std::min(256 / tiles, (K + 2047) / 2048);
+  if (splits > 1 && ws) {
+    int kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + kGK - 1) / kGK * kGK;
+    splits = (K + kchunk - 1) / kchunk;
+    int rc = ws->init();
+    if (rc) return rc;
+    if (static_cast<size_t>(splits) * M * N > kGemmPartFloats)
+      return fail(DGFS_ERR_INVALID_VALUE, "split-K workspace too small");
+    g.kchunk = kchunk;
+    g.part = ws->part;
+    if (bf16)
+      hipLaunchKernelGGL(k_gemm<true>, dim3(gx, gy, splits), dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL(k_gemm<false>, dim3(gx, gy, splits), dim3(256), 0, st, g);
+    DGFS_LAUNCH_CHECK("gemm(split)");
+    const long tot = static_cast<long>(M) * N;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, st,
+                       ws->part, splits, M, N, alpha, beta, C, ldc);
+    DGFS_LAUNCH_CHECK("gemm_reduce");
+    return DGFS_OK;
+  }
+  if (bf16)
+    hipLaunchKernelGGL(k_gemm<true>, dim3(gx, gy, 1), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_gemm<false>, dim3(gx, gy, 1), dim3(256), 0, st, g);
+  DGFS_LAUNCH_CHECK("gemm");
+  return DGFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// network elementwise kernels (activations Z[m][p][c], c fastest)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gelu(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+__device__ __forceinline__ float gelu_d(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float t = tanhf(k0 * (x + k1 * x * x * x));
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k0 * (1.f + 3.f * k1 * x * x);
+}
+
+__global__ __launch_bounds__(256) void k_lift_fwd(const float* __restrict__ th, const float* __restrict__ x,
+                                                  const float* __restrict__ lt, float* __restrict__ Z, int P) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long tot = static_cast<long>(kSurModes) * P * kSurWidth;
+  if (i >= tot) return;
+  const int c = static_cast<int>(i % kSurWidth);
+  const long mp = i / kSurWidth;
+  const int p = static_cast<int>(mp % P), m = static_cast<int>(mp / P);
+  Z[i] = th[kSurOffP0 + c] * x[static_cast<long>(p) * kSurModes + m] + th[kSurOffP1 + c] * lt[p] +
+         th[kSurOffE + m * kSurWidth + c];
+}
+
+// V += bias; Z = act(V) (act = GELU or identity); V keeps the pre-activation
+__global__ __launch_bounds__(256) void k_bias_act(float* __restrict__ V, const float* __restrict__ bias,
+                                                  float* __restrict__ Z, long tot, int act) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const float v = V[i] + bias[i % kSurWidth];
+  V[i] = v;
+  Z[i] = act ? gelu(v) : v;
+}
+
+__global__ __launch_bounds__(256) void k_head_fwd(const float* __restrict__ th, const float* __restrict__ Z,
+                                                  float* __restrict__ out, int P) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;  // over (m, p)
+  if (i >= static_cast<long>(kSurModes) * P) return;
+  const int p = static_cast<int>(i % P), m = static_cast<int>(i / P);
+  const float* z = Z + i * kSurWidth;
+  float s = th[kSurOffBout + m];
+#pragma unroll
+  for (int c = 0; c < kSurWidth; ++c) s += z[c] * th[kSurOffPout + c];
+  out[static_cast<long>(p) * kSurModes + m] = s;
+}
+
+__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ th, const float* __restrict__ dout,
+                                                  float* __restrict__ dZ, int P) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long tot = static_cast<long>(kSurModes) * P * kSurWidth;
+  if (i >= tot) return;
+  const int c = static_cast<int>(i % kSurWidth);
+  const long mp = i / kSurWidth;
+  const int p = static_cast<int>(mp % P), m = static_cast<int>(mp / P);
+  dZ[i] = dout[static_cast<long>(p) * kSurModes + m] * th[kSurOffPout + c];
+}
+
+__global__ __launch_bounds__(256) void k_act_bwd(const float* __restrict__ V, float* __restrict__ dZ, long tot) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  dZ[i] *= gelu_d(V[i]);
+}
+
+// Per-channel and per-mode reductions of the backward pass, two stages
+// (block partials, then one block) so the sums are deterministic.
+//   kind 0: out[c]     = sum_{m,p} X[m][p][c]                 (bias)
+//   kind 1: out[c]     = sum_{m,p} X[m][p][c] Y[m][p][c]      (Pout)
+//   kind 2: out[c]     = sum_{m,p} X[m][p][c] x[p][m]          (P0)
+//   kind 3: out[c]     = sum_{m,p} X[m][p][c] lt[p]            (P1)
+struct ChanRedArgs {
+  const float* X;
+  const float* Y;   // kind 1
+  const float* xin; // kind 2: [P][512]
+  const float* lt;  // kind 3: [P]
+  const float* dout; // unused
+  float* part;      // [gridDim.x][32]
+  int P, kind;
+};
+
+__global__ __launch_bounds__(256) void k_chan_reduce(ChanRedArgs a) {
+  __shared__ float red[8][kSurWidth];
+  const int c = threadIdx.x & 31, row = threadIdx.x >> 5;  // 8 rows of 32 channels
+  const long rows = static_cast<long>(kSurModes) * a.P;
+  float s = 0.f;
+  for (long r = static_cast<long>(blockIdx.x) * 8 + row; r < rows; r += static_cast<long>(gridDim.x) * 8) {
+    const long i = r * kSurWidth + c;
+    float v = a.X[i];
+    if (a.kind == 1) {
+      v *= a.Y[i];
+    } else if (a.kind == 2) {
+      const int p = static_cast<int>(r % a.P), m = static_cast<int>(r / a.P);
+      v *= a.xin[static_cast<long>(p) * kSurModes + m];
+    } else if (a.kind == 3) {
+      v *= a.lt[r % a.P];
+    }
+    s += v;
+  }
+  red[row][c] = s;
+  __syncthreads();
+  if (row == 0) {
+    float t = 0.f;
+    for (int k = 0; k < 8; ++k) t += red[k][c];
+    a.part[static_cast<long>(blockIdx.x) * kSurWidth + c] = t;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_sum_parts(const float* __restrict__ part, int nparts, int width,
+                                                 float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= width) return;
+  float s = 0.f;
+  for (int k = 0; k < nparts; ++k) s += part[static_cast<long>(k) * width + c];
+  out[c] = s;
+}
+
+// dE[m][c] = sum_p dZ0[m][p][c]
+__global__ __launch_bounds__(256) void k_mode_reduce(const float* __restrict__ dZ, float* __restrict__ out, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over (m, c)
+  if (i >= kSurModes * kSurWidth) return;
+  const int c = i % kSurWidth, m = i / kSurWidth;
+  float s = 0.f;
+  for (int p = 0; p < P; ++p) s += dZ[(static_cast<long>(m) * P + p) * kSurWidth + c];
+  out[i] = s;
+}
+
+// dbout[m] = sum_p dout[p][m]
+__global__ __launch_bounds__(256) void k_bout_reduce(const float* __restrict__ dout, float* __restrict__ out, int P) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= kSurModes) return;
+  float s = 0.f;
+  for (int p = 0; p < P; ++p) s += dout[static_cast<long>(p) * kSurModes + m];
+  out[m] = s;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ th, const float* __restrict__ g,
+                                              float* __restrict__ m1, float* __restrict__ m2, long n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  th[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Hermite encoder / decoder (FP64), one workgroup per point
+// ---------------------------------------------------------------------------
+struct EncArgs {
+  const double* f;  // [npts][n]
+  double* mom;      // [npts][8]: rho, ux, uy, uz, T, indicator, 0, 0
+  float* x;         // [npts][512] a_n (a_000 replaced by 0), or null
+  float* lt;        // [npts] log T, or null
+  double L, cw;
+  int npts;
+};
+
+__device__ __forceinline__ void herm_table(double* V, double c0, double h, double u, double s, int N) {
+  // V[i*8 + a] = He_a((c0 + i h - u) / s), probabilists' recurrence
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double x = (c0 + i * h - u) / s;
+    double p0 = 1.0, p1 = x;
+    V[i * kSurD + 0] = p0;
+    V[i * kSurD + 1] = p1;
+    for (int a = 1; a + 1 < kSurD; ++a) {
+      const double p2 = x * p1 - a * p0;
+      V[i * kSurD + a + 1] = p2;
+      p0 = p1;
+      p1 = p2;
+    }
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__constant__ double kInvFact[kSurD] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0, 1.0 / 5040.0};
+
+// the 8^3 Hermite moments of g in the frame (u, s), divided by n! and `scale`:
+// out[abc] = sum_ijk g[i][j][k] Vx[i][a] Vy[j][b] Vz[k][c] cw / (a! b! c! scale)
+template <int N>
+__device__ void herm_moments(const double* __restrict__ g, const double* Vx, const double* Vy, const double* Vz,
+                             double* t1, double* t2, double scale_cw, float* out, double* dout) {
+  constexpr int NN = N * N;
+  for (int e = threadIdx.x; e < kSurD * NN; e += blockDim.x) {  // t1[a][j][k]
+    const int a = e / NN, jk = e - a * NN;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += g[i * NN + jk] * Vx[i * kSurD + a];
+    t1[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSurD * kSurD * N; e += blockDim.x) {  // t2[a][b][k]
+    const int a = e / (kSurD * N), rem = e - a * kSurD * N, b = rem / N, k = rem - b * N;
+    double s = 0.0;
+    for (int j = 0; j < N; ++j) s += t1[a * NN + j * N + k] * Vy[j * kSurD + b];
+    t2[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int a = e / (kSurD * kSurD), b = (e / kSurD) % kSurD, c = e % kSurD;
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += t2[(a * kSurD + b) * N + k] * Vz[k * kSurD + c];
+    const double v = s * scale_cw * kInvFact[a] * kInvFact[b] * kInvFact[c];
+    if (out) out[e] = static_cast<float>(v);
+    if (dout) dout[e] = v;
+  }
+  __syncthreads();
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
+  constexpr int n = N * N * N, NN = N * N;
+  extern __shared__ double sm[];
+  double* t1 = sm;                      // kSurD * N * N
+  double* t2 = t1 + kSurD * NN;         // kSurD * kSurD * N
+  double* V = t2 + kSurD * kSurD * N;   // 3 * N * kSurD
+  double* red = V + 3 * N * kSurD;      // 256
+  const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
+  const double pi = 3.14159265358979323846;
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    const double* f = a.f + static_cast<size_t>(pt) * n;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const double cx = c0 + (r / NN) * hh, cy = c0 + ((r / N) % N) * hh, cz = c0 + (r % N) * hh, v = f[r];
+      s0 += v;
+      s1 += v * cx;
+      s2 += v * cy;
+      s3 += v * cz;
+      s4 += v * (cx * cx + cy * cy + cz * cz);
+    }
+    const double rho = block_sum(s0, red) * a.cw;
+    const double ux = block_sum(s1, red) * a.cw / rho;
+    const double uy = block_sum(s2, red) * a.cw / rho;
+    const double uz = block_sum(s3, red) * a.cw / rho;
+    const double T = (2.0 / 3.0) * (block_sum(s4, red) * a.cw / rho - (ux * ux + uy * uy + uz * uz));
+    const double Mn = rho / pow(pi * T, 1.5);
+    double d2 = 0, f2 = 0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const double cx = c0 + (r / NN) * hh - ux, cy = c0 + ((r / N) % N) * hh - uy, cz = c0 + (r % N) * hh - uz;
+      const double M = Mn * exp(-(cx * cx + cy * cy + cz * cz) / T), v = f[r];
+      d2 += (v - M) * (v - M);
+      f2 += v * v;
+    }
+    const double ind = sqrt(block_sum(d2, red) / block_sum(f2, red));
+    if (threadIdx.x == 0) {
+      double* mo = a.mom + static_cast<size_t>(pt) * 8;
+      mo[0] = rho; mo[1] = ux; mo[2] = uy; mo[3] = uz; mo[4] = T; mo[5] = ind; mo[6] = 0; mo[7] = 0;
+      if (a.lt) a.lt[pt] = static_cast<float>(log(T));
+    }
+    if (a.x) {
+      const double s = sqrt(0.5 * T);
+      herm_table(V, c0, hh, ux, s, N);
+      herm_table(V + N * kSurD, c0, hh, uy, s, N);
+      herm_table(V + 2 * N * kSurD, c0, hh, uz, s, N);
+      __syncthreads();
+      float* xo = a.x + static_cast<size_t>(pt) * kSurModes;
+      herm_moments<N>(f, V, V + N * kSurD, V + 2 * N * kSurD, t1, t2, a.cw / rho, xo, nullptr);
+      if (threadIdx.x == 0) xo[0] = 0.f;  // a_000 = 1 by construction
+    }
+    __syncthreads();
+  }
+}
+
+// targets: b^ = (Hermite moments of Q in the frame of mom) / (prefac rho T^{gamma/2})
+struct TgtArgs {
+  const double* Q;
+  const double* mom;
+  float* y;
+  double L, cw, gamma, prefac;
+  int npts;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_sur_targets(TgtArgs a) {
+  constexpr int n = N * N * N, NN = N * N;
+  extern __shared__ double sm[];
+  double* t1 = sm;
+  double* t2 = t1 + kSurD * NN;
+  double* V = t2 + kSurD * kSurD * N;
+  const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
+  for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+    const double* mo = a.mom + static_cast<size_t>(pt) * 8;
+    const double rho = mo[0], T = mo[4], s = sqrt(0.5 * T);
+    herm_table(V, c0, hh, mo[1], s, N);
+    herm_table(V + N * kSurD, c0, hh, mo[2], s, N);
+    herm_table(V + 2 * N * kSurD, c0, hh, mo[3], s, N);
+    __syncthreads();
+    const double scale = a.cw / (rho * a.prefac * rho * pow(T, 0.5 * a.gamma));
+    herm_moments<N>(a.Q + static_cast<size_t>(pt) * n, V, V + N * kSurD, V + 2 * N * kSurD, t1, t2, scale,
+                    a.y + static_cast<size_t>(pt) * kSurModes, nullptr);
+  }
+}
+
+// decode: Q[idx[i]] = M sum_n b_n He_n(c'), b = prefac rho T^{gamma/2} y with the
+// mass, momentum and energy modes removed (b_000 = b_100 = b_010 = b_001 = 0,
+// b_200 + b_020 + b_002 = 0)
+struct DecArgs {
+  const float* y;    // [nidx][512] network outputs, row i for point idx[i]
+  const double* mom; // [npts][8]
+  const int* idx;    // [nidx] point index of row i (null: identity)
+  double* Q;         // [npts][n]
+  double L, gamma, prefac;
+  int nidx;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_sur_decode(DecArgs a) {
+  constexpr int n = N * N * N, NN = N * N;
+  extern __shared__ double sm[];
+  double* s2 = sm;                      // kSurD * N * N   [a][j][k]
+  double* s1 = s2 + kSurD * NN;         // kSurD * kSurD * N [a][b][k]
+  double* V = s1 + kSurD * kSurD * N;   // 3 * N * kSurD
+  double* bb = V + 3 * N * kSurD;       // 512
+  const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
+  const double pi = 3.14159265358979323846;
+  for (int i = blockIdx.x; i < a.nidx; i += gridDim.x) {
+    const int pt = a.idx ? a.idx[i] : i;
+    const double* mo = a.mom + static_cast<size_t>(pt) * 8;
+    const double rho = mo[0], ux = mo[1], uy = mo[2], uz = mo[3], T = mo[4], s = sqrt(0.5 * T);
+    herm_table(V, c0, hh, ux, s, N);
+    herm_table(V + N * kSurD, c0, hh, uy, s, N);
+    herm_table(V + 2 * N * kSurD, c0, hh, uz, s, N);
+    const double scale = a.prefac * rho * pow(T, 0.5 * a.gamma);
+    const float* yi = a.y + static_cast<size_t>(i) * kSurModes;
+    for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) bb[e] = scale * static_cast<double>(yi[e]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      constexpr int e200 = 2 * kSurD * kSurD, e020 = 2 * kSurD, e002 = 2;
+      const double tr = (bb[e200] + bb[e020] + bb[e002]) / 3.0;
+      bb[e200] -= tr; bb[e020] -= tr; bb[e002] -= tr;
+      bb[0] = 0.0; bb[kSurD * kSurD] = 0.0; bb[kSurD] = 0.0; bb[1] = 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSurD * kSurD * N; e += blockDim.x) {  // s1[a][b][k]
+      const int ab = e / N, k = e - ab * N;
+      double t = 0.0;
+      for (int c = 0; c < kSurD; ++c) t += bb[ab * kSurD + c] * V[2 * N * kSurD + k * kSurD + c];
+      s1[e] = t;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSurD * NN; e += blockDim.x) {  // s2[a][j][k]
+      const int aa = e / NN, jk = e - aa * NN, j = jk / N, k = jk - j * N;
+      double t = 0.0;
+      for (int b = 0; b < kSurD; ++b) t += s1[(aa * kSurD + b) * N + k] * V[N * kSurD + j * kSurD + b];
+      s2[e] = t;
+    }
+    __syncthreads();
+    const double Mn = rho / pow(pi * T, 1.5);
+    double* Q = a.Q + static_cast<size_t>(pt) * n;
+    for (int jk = threadIdx.x; jk < NN; jk += blockDim.x) {
+      const int j = jk / N, k = jk - j * N;
+      const double cy = c0 + j * hh - uy, cz = c0 + k * hh - uz;
+      for (int ii = 0; ii < N; ++ii) {
+        const double cx = c0 + ii * hh - ux;
+        double t = 0.0;
+#pragma unroll
+        for (int aa = 0; aa < kSurD; ++aa) t += s2[aa * NN + jk] * V[ii * kSurD + aa];
+        Q[ii * NN + jk] = Mn * exp(-(cx * cx + cy * cy + cz * cz) / T) * t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// rows of src [*][n] -> dst [nidx][n] (gather) or back (scatter)
+__global__ __launch_bounds__(256) void k_rows_copy(const double* __restrict__ src, double* __restrict__ dst,
+                                                   const int* __restrict__ idx, int nidx, int n, int gather) {
+  const int i = blockIdx.y;
+  if (i >= nidx) return;
+  const long s = gather ? idx[i] : i, d = gather ? i : idx[i];
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    dst[d * n + r] = src[s * n + r];
+}
+
+__global__ __launch_bounds__(256) void k_rows_copy_f(const float* __restrict__ src, float* __restrict__ dst,
+                                                     const int* __restrict__ idx, int nidx, int n) {
+  const int i = blockIdx.y;
+  if (i >= nidx) return;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    dst[static_cast<long>(i) * n + r] = src[static_cast<long>(idx[i]) * n + r];
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct SurHandle {
+  int magic = 0x73757231;
+  int N = 0, n = 0, max_batch = 0;
+  double L = 0, cw = 0, gamma = 0, prefac = 0;
+  double Ainv[25] = {};
+  GemmWs ws;
+  // activations of the last forward (capacity max_batch points)
+  float* Z[kSurLayers + 1] = {};
+  float* U[kSurLayers] = {};
+  float* Vv[kSurLayers] = {};
+  float* dA = nullptr;
+  float* dB = nullptr;
+  float* dU = nullptr;
+  float* part = nullptr;  // channel-reduction partials
+  const float* x_last = nullptr;
+  const float* lt_last = nullptr;
+  int P_last = 0;
+  int bf16_last = 1;
+  // evaluation workspace
+  size_t cap_pts = 0;
+  double* mom = nullptr;
+  float* xin = nullptr;
+  float* ltin = nullptr;
+  float* yout = nullptr;
+  float* xg = nullptr;
+  float* ltg = nullptr;
+  int* idx_d = nullptr;
+  double* fb_f = nullptr;
+  double* fb_Q = nullptr;
+  size_t cap_fb = 0;
+  double* mom_h = nullptr;  // pinned copy of the moments
+  ~SurHandle() {
+    for (auto* p : Z) (void)hipFree(p);
+    for (auto* p : U) (void)hipFree(p);
+    for (auto* p : Vv) (void)hipFree(p);
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dU); (void)hipFree(part);
+    (void)hipFree(mom); (void)hipFree(xin); (void)hipFree(ltin); (void)hipFree(yout);
+    (void)hipFree(xg); (void)hipFree(ltg); (void)hipFree(idx_d); (void)hipFree(fb_f); (void)hipFree(fb_Q);
+    (void)hipHostFree(mom_h);
+  }
+};
+
+static SurHandle* as_sur(void* p) {
+  auto* h = static_cast<SurHandle*>(p);
+  if (!h || h->magic != 0x73757231) return nullptr;
+  return h;
+}
+
+static inline unsigned nblk(long tot) { return static_cast<unsigned>((tot + 255) / 256); }
+constexpr int kRedBlocks = 240;
+
+static int chan_reduce(SurHandle* h, int kind, const float* X, const float* Y, const float* xin, const float* lt,
+                       int P, float* out, hipStream_t st) {
+  ChanRedArgs ca{X, Y, xin, lt, nullptr, h->part, P, kind};
+  hipLaunchKernelGGL(k_chan_reduce, dim3(kRedBlocks), dim3(256), 0, st, ca);
+  DGFS_LAUNCH_CHECK("chan_reduce");
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, h->part, kRedBlocks, kSurWidth, out);
+  DGFS_LAUNCH_CHECK("sum_parts");
+  return DGFS_OK;
+}
+
+static int sur_forward(SurHandle* h, const float* th, const float* x, const float* lt, int P, float* out, bool bf16,
+                       hipStream_t st) {
+  const long act = static_cast<long>(kSurModes) * P * kSurWidth;
+  hipLaunchKernelGGL(k_lift_fwd, dim3(nblk(act)), dim3(256), 0, st, th, x, lt, h->Z[0], P);
+  DGFS_LAUNCH_CHECK("lift_fwd");
+  for (int l = 0; l < kSurLayers; ++l) {
+    const float* K = th + kSurOffLayers + l * kSurLayerSize;
+    const float* W = K + static_cast<long>(kSurModes) * kSurModes;
+    const float* R = W + kSurWidth * kSurWidth;
+    const float* bias = R + kSurWidth * kSurWidth;
+    int rc = gemm(&h->ws, 0, 0, kSurModes, kSurWidth * P, kSurModes, 1.f, K, kSurModes, h->Z[l],
+                  static_cast<long>(kSurWidth) * P, 0.f, h->U[l], static_cast<long>(kSurWidth) * P, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 0, 0, kSurModes * P, kSurWidth, kSurWidth, 1.f, h->U[l], kSurWidth, W, kSurWidth, 0.f,
+                       h->Vv[l], kSurWidth, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 0, 0, kSurModes * P, kSurWidth, kSurWidth, 1.f, h->Z[l], kSurWidth, R, kSurWidth, 1.f,
+                       h->Vv[l], kSurWidth, bf16, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bias_act, dim3(nblk(act)), dim3(256), 0, st, h->Vv[l], bias, h->Z[l + 1], act,
+                       l + 1 < kSurLayers ? 1 : 0);
+    DGFS_LAUNCH_CHECK("bias_act");
+  }
+  hipLaunchKernelGGL(k_head_fwd, dim3(nblk(static_cast<long>(kSurModes) * P)), dim3(256), 0, st, th,
+                     h->Z[kSurLayers], out, P);
+  DGFS_LAUNCH_CHECK("head_fwd");
+  h->x_last = x;
+  h->lt_last = lt;
+  h->P_last = P;
+  h->bf16_last = bf16 ? 1 : 0;
+  return DGFS_OK;
+}
+
+static int sur_backward(SurHandle* h, const float* th, const float* dout, float* grad, hipStream_t st) {
+  const int P = h->P_last;
+  const bool bf16 = h->bf16_last != 0;
+  const long act = static_cast<long>(kSurModes) * P * kSurWidth;
+  const long ld = static_cast<long>(kSurWidth) * P;
+  // head
+  hipLaunchKernelGGL(k_head_bwd, dim3(nblk(act)), dim3(256), 0, st, th, dout, h->dA, P);
+  DGFS_LAUNCH_CHECK("head_bwd");
+  {
+    // dPout[c] = sum_{m,p} Z4[m][p][c] dout[p][m] (kind 2 with xin = dout)
+    int rc = chan_reduce(h, 2, h->Z[kSurLayers], nullptr, dout, nullptr, P, grad + kSurOffPout, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bout_reduce, dim3(nblk(kSurModes)), dim3(256), 0, st, dout, grad + kSurOffBout, P);
+    DGFS_LAUNCH_CHECK("bout_reduce");
+  }
+  float* dZ = h->dA;    // gradient w.r.t. Z_{l+1}
+  float* dZn = h->dB;   // gradient w.r.t. Z_l
+  for (int l = kSurLayers - 1; l >= 0; --l) {
+    const float* K = th + kSurOffLayers + l * kSurLayerSize;
+    const float* W = K + static_cast<long>(kSurModes) * kSurModes;
+    const float* R = W + kSurWidth * kSurWidth;
+    float* gK = grad + kSurOffLayers + l * kSurLayerSize;
+    float* gW = gK + static_cast<long>(kSurModes) * kSurModes;
+    float* gR = gW + kSurWidth * kSurWidth;
+    float* gb = gR + kSurWidth * kSurWidth;
+    if (l + 1 < kSurLayers) {
+      hipLaunchKernelGGL(k_act_bwd, dim3(nblk(act)), dim3(256), 0, st, h->Vv[l], dZ, act);
+      DGFS_LAUNCH_CHECK("act_bwd");
+    }
+    // dZ now holds dV
+    int rc = chan_reduce(h, 0, dZ, nullptr, nullptr, nullptr, P, gb, st);
+    if (!rc) rc = gemm(&h->ws, 1, 0, kSurWidth, kSurWidth, kSurModes * P, 1.f, h->U[l], kSurWidth, dZ, kSurWidth, 0.f,
+                       gW, kSurWidth, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 1, 0, kSurWidth, kSurWidth, kSurModes * P, 1.f, h->Z[l], kSurWidth, dZ, kSurWidth, 0.f,
+                       gR, kSurWidth, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 0, 1, kSurModes * P, kSurWidth, kSurWidth, 1.f, dZ, kSurWidth, W, kSurWidth, 0.f,
+                       h->dU, kSurWidth, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 0, 1, kSurModes * P, kSurWidth, kSurWidth, 1.f, dZ, kSurWidth, R, kSurWidth, 0.f,
+                       dZn, kSurWidth, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 0, 1, kSurModes, kSurModes, kSurWidth * P, 1.f, h->dU, ld, h->Z[l], ld, 0.f, gK,
+                       kSurModes, bf16, st);
+    if (!rc) rc = gemm(&h->ws, 1, 0, kSurModes, kSurWidth * P, kSurModes, 1.f, K, kSurModes, h->dU, ld, 1.f, dZn, ld,
+                       bf16, st);
+    if (rc) return rc;
+    std::swap(dZ, dZn);
+  }
+  // lift
+  int rc = chan_reduce(h, 2, dZ, nullptr, h->x_last, nullptr, P, grad + kSurOffP0, st);
+  if (!rc) rc = chan_reduce(h, 3, dZ, nullptr, nullptr, h->lt_last, P, grad + kSurOffP1, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mode_reduce, dim3(nblk(kSurModes * kSurWidth)), dim3(256), 0, st, dZ, grad + kSurOffE, P);
+  DGFS_LAUNCH_CHECK("mode_reduce");
+  return DGFS_OK;
+}
+
+template <int N>
+static size_t enc_lds() {
+  return (static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + 256) * sizeof(double);
+}
+template <int N>
+static size_t dec_lds() {
+  return (static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + kSurModes) * sizeof(double);
+}
+
+template <int N>
+static int launch_encode(const EncArgs& ea, hipStream_t st) {
+  const size_t lds = enc_lds<N>();
+  DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sur_encode<N>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  hipLaunchKernelGGL(k_sur_encode<N>, dim3(std::min(ea.npts, 4 * cu_count())), dim3(256), lds, st, ea);
+  DGFS_LAUNCH_CHECK("sur_encode");
+  return DGFS_OK;
+}
+template <int N>
+static int launch_targets(const TgtArgs& ta, hipStream_t st) {
+  const size_t lds = enc_lds<N>();
+  DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sur_targets<N>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  hipLaunchKernelGGL(k_sur_targets<N>, dim3(std::min(ta.npts, 4 * cu_count())), dim3(256), lds, st, ta);
+  DGFS_LAUNCH_CHECK("sur_targets");
+  return DGFS_OK;
+}
+template <int N>
+static int launch_decode(const DecArgs& da, hipStream_t st) {
+  const size_t lds = dec_lds<N>();
+  DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sur_decode<N>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  hipLaunchKernelGGL(k_sur_decode<N>, dim3(std::min(da.nidx, 4 * cu_count())), dim3(256), lds, st, da);
+  DGFS_LAUNCH_CHECK("sur_decode");
+  return DGFS_OK;
+}
+
+static int do_encode(SurHandle* h, const EncArgs& ea, hipStream_t st) {
+  switch (h->N) {
+    case 8: return launch_encode<8>(ea, st);
+    case 12: return launch_encode<12>(ea, st);
+    case 16: return launch_encode<16>(ea, st);
+    case 24: return launch_encode<24>(ea, st);
+    case 32: return launch_encode<32>(ea, st);
+    default: return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d not built for the surrogate", h->N);
+  }
+}
+static int do_targets(SurHandle* h, const TgtArgs& ta, hipStream_t st) {
+  switch (h->N) {
+    case 8: return launch_targets<8>(ta, st);
+    case 12: return launch_targets<12>(ta, st);
+    case 16: return launch_targets<16>(ta, st);
+    case 24: return launch_targets<24>(ta, st);
+    case 32: return launch_targets<32>(ta, st);
+    default: return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d not built for the surrogate", h->N);
+  }
+}
+static int do_decode(SurHandle* h, const DecArgs& da, hipStream_t st) {
+  switch (h->N) {
+    case 8: return launch_decode<8>(da, st);
+    case 12: return launch_decode<12>(da, st);
+    case 16: return launch_decode<16>(da, st);
+    case 24: return launch_decode<24>(da, st);
+    case 32: return launch_decode<32>(da, st);
+    default: return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d not built for the surrogate", h->N);
+  }
+}
+
+static int ensure_eval_ws(SurHandle* h, int npts) {
+  if (static_cast<size_t>(npts) <= h->cap_pts) return DGFS_OK;
+  (void)hipFree(h->mom); (void)hipFree(h->xin); (void)hipFree(h->ltin); (void)hipFree(h->yout);
+  (void)hipFree(h->xg); (void)hipFree(h->ltg); (void)hipFree(h->idx_d); (void)hipHostFree(h->mom_h);
+  h->mom = nullptr; h->xin = h->ltin = h->yout = h->xg = h->ltg = nullptr; h->idx_d = nullptr; h->mom_h = nullptr;
+  h->cap_pts = 0;
+  DGFS_HIP_TRY(hipMalloc(&h->mom, static_cast<size_t>(npts) * 8 * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->xin, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
+  DGFS_HIP_TRY(hipMalloc(&h->ltin, static_cast<size_t>(npts) * sizeof(float)));
+  DGFS_HIP_TRY(hipMalloc(&h->yout, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
+  DGFS_HIP_TRY(hipMalloc(&h->xg, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
+  DGFS_HIP_TRY(hipMalloc(&h->ltg, static_cast<size_t>(npts) * sizeof(float)));
+  DGFS_HIP_TRY(hipMalloc(&h->idx_d, static_cast<size_t>(npts) * 2 * sizeof(int)));
+  DGFS_HIP_TRY(hipHostMalloc(&h->mom_h, static_cast<size_t>(npts) * 8 * sizeof(double)));
+  h->cap_pts = npts;
+  return DGFS_OK;
+}
+
+}  // namespace dgfs
+
+using namespace dgfs;
+
+extern "C" long dgfs_sur_param_count(void) { return kSurParams; }
+
+extern "C" int dgfs_nn_gemm(int transA, int transB, int M, int N, int K, float alpha, const float* A, long lda,
+                            const float* B, long ldb, float beta, float* C, long ldc, int bf16, void* stream) {
+  if (!A || !B || !C) return fail(DGFS_ERR_INVALID_VALUE, "null GEMM operand");
+  if (M < 0 || N < 0 || K < 1) return fail(DGFS_ERR_INVALID_VALUE, "bad GEMM sizes %d %d %d", M, N, K);
+  if (lda < (transA ? M : K) || ldb < (transB ? K : N) || ldc < N)
+    return fail(DGFS_ERR_INVALID_VALUE, "leading dimension too small");
+  if ((M + kGT - 1) / kGT > 65535) return fail(DGFS_ERR_INVALID_VALUE, "M = %d exceeds the grid (65535 row tiles)", M);
+  // process-lifetime workspace (released by the driver at exit)
+  static GemmWs* ws = new GemmWs();
+  return gemm(ws, transA ? 1 : 0, transB ? 1 : 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bf16 != 0,
+              static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgfs_sur_create(const dgfs_sur_desc* d, void** out) {
+  if (!d || !out) return fail(DGFS_ERR_INVALID_VALUE, "null descriptor or handle pointer");
+  *out = nullptr;
+  switch (d->Nv) {
+    case 8: case 12: case 16: case 24: case 32: break;
+    default: return fail(DGFS_ERR_UNSUPPORTED, "surrogate Nv=%d unsupported (built: 8 12 16 24 32)", d->Nv);
+  }
+  // the channel GEMMs have 512 * batch rows, one 64-row tile per grid row
+  if (d->max_batch < 1 || d->max_batch > 4096) return fail(DGFS_ERR_INVALID_VALUE, "max_batch must be 1..4096");
+  if (!(d->L > 0) || !(d->cw > 0) || !(d->prefac > 0)) return fail(DGFS_ERR_INVALID_VALUE, "L, cw, prefac must be positive");
+  auto h = std::make_unique<SurHandle>();
+  h->N = d->Nv;
+  h->n = d->Nv * d->Nv * d->Nv;
+  h->max_batch = d->max_batch;
+  h->L = d->L;
+  h->cw = d->cw;
+  h->gamma = d->gamma;
+  h->prefac = d->prefac;
+  int rc = invariant_gram_inverse(h->N, h->L, h->cw, h->Ainv);
+  if (rc) return rc;
+  const size_t act = static_cast<size_t>(kSurModes) * d->max_batch * kSurWidth * sizeof(float);
+  for (int l = 0; l <= kSurLayers; ++l) DGFS_HIP_TRY(hipMalloc(&h->Z[l], act));
+  for (int l = 0; l < kSurLayers; ++l) {
+    DGFS_HIP_TRY(hipMalloc(&h->U[l], act));
+    DGFS_HIP_TRY(hipMalloc(&h->Vv[l], act));
+  }
+  DGFS_HIP_TRY(hipMalloc(&h->dA, act));
+  DGFS_HIP_TRY(hipMalloc(&h->dB, act));
+  DGFS_HIP_TRY(hipMalloc(&h->dU, act));
+  DGFS_HIP_TRY(hipMalloc(&h->part, static_cast<size_t>(kRedBlocks) * kSurWidth * sizeof(float)));
+  *out = h.release();
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_sur_destroy(void* p) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  (void)hipDeviceSynchronize();
+  h->magic = 0;
+  delete h;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_sur_forward(void* p, const float* params, const float* x, const float* lt, int P, float* out,
+                                int bf16, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!params || !x || !lt || !out) return fail(DGFS_ERR_INVALID_VALUE, "null network operand");
+  if (P < 1 || P > h->max_batch) return fail(DGFS_ERR_INVALID_VALUE, "batch %d outside 1..%d", P, h->max_batch);
+  return sur_forward(h, params, x, lt, P, out, bf16 != 0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgfs_sur_backward(void* p, const float* params, const float* dout, float* grad, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!params || !dout || !grad) return fail(DGFS_ERR_INVALID_VALUE, "null network operand");
+  if (h->P_last < 1) return fail(DGFS_ERR_INVALID_VALUE, "backward needs a preceding forward");
+  return sur_backward(h, params, dout, grad, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgfs_sur_adam(long n, float* params, const float* grad, float* m1, float* m2, float lr, float b1,
+                             float b2, float eps, int step, void* stream) {
+  if (!params || !grad || !m1 || !m2) return fail(DGFS_ERR_INVALID_VALUE, "null Adam operand");
+  if (n < 1 || step < 1) return fail(DGFS_ERR_INVALID_VALUE, "Adam needs n >= 1 and step >= 1");
+  const float c1 = 1.f - std::pow(b1, static_cast<float>(step));
+  const float c2 = 1.f - std::pow(b2, static_cast<float>(step));
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, static_cast<hipStream_t>(stream), params, grad, m1, m2, n,
+                     lr, b1, b2, eps, c1, c2);
+  DGFS_LAUNCH_CHECK("adam");
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_sur_encode(void* p, const double* f, int npts, double* mom, float* x, float* lt, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!f || !mom) return fail(DGFS_ERR_INVALID_VALUE, "null encode operand");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  if (npts == 0) return DGFS_OK;
+  EncArgs ea{f, mom, x, lt, h->L, h->cw, npts};
+  return do_encode(h, ea, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgfs_sur_targets(void* p, const double* Q, const double* mom, int npts, float* y, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!Q || !mom || !y) return fail(DGFS_ERR_INVALID_VALUE, "null target operand");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  if (npts == 0) return DGFS_OK;
+  TgtArgs ta{Q, mom, y, h->L, h->cw, h->gamma, h->prefac, npts};
+  return do_targets(h, ta, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgfs_sur_decode(void* p, const float* y, const double* mom, const int* idx, int nidx, double* Q,
+                               int project, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!y || !mom || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null decode operand");
+  if (nidx < 0) return fail(DGFS_ERR_INVALID_VALUE, "nidx must be >= 0");
+  if (nidx == 0) return DGFS_OK;
+  if (project && idx) return fail(DGFS_ERR_INVALID_VALUE, "projection of an index subset: project the rows afterwards");
+  auto st = static_cast<hipStream_t>(stream);
+  DecArgs da{y, mom, idx, Q, h->L, h->gamma, h->prefac, nidx};
+  int rc = do_decode(h, da, st);
+  if (!rc && project) rc = launch_invariant_projection(Q, h->Ainv, h->L, h->cw, h->N, nidx, st);
+  return rc;
+}
+
+// Surrogate with fallback: encode every point, evaluate the network on the
+// points whose indicator is <= threshold, decode and project them; run the
+// exact operator (a dgfs_vhs handle) on the rest.  One host synchronisation
+// reads the indicators.  nfallback (optional) receives the fallback count.
+extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const double* f, double* Q, int npts,
+                             double threshold, int bf16, int* nfallback, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!vhs || !params || !f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null operand");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  if (npts == 0) {
+    if (nfallback) *nfallback = 0;
+    return DGFS_OK;
+  }
+  auto st = static_cast<hipStream_t>(stream);
+  int rc = ensure_eval_ws(h, npts);
+  if (rc) return rc;
+  EncArgs ea{f, h->mom, h->xin, h->ltin, h->L, h->cw, npts};
+  rc = do_encode(h, ea, st);
+  if (rc) return rc;
+  DGFS_HIP_TRY(hipMemcpyAsync(h->mom_h, h->mom, static_cast<size_t>(npts) * 8 * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+  DGFS_HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> sur, fb;
+  sur.reserve(npts);
+  for (int i = 0; i < npts; ++i) {
+    const double ind = h->mom_h[static_cast<size_t>(i) * 8 + 5];
+    if (ind <= threshold && std::isfinite(ind)) sur.push_back(i); else fb.push_back(i);
+  }
+  if (nfallback) *nfallback = static_cast<int>(fb.size());
+  const int ns = static_cast<int>(sur.size()), nf = static_cast<int>(fb.size());
+  int* idx_s = h->idx_d;
+  int* idx_f = h->idx_d + npts;
+  if (ns) DGFS_HIP_TRY(hipMemcpyAsync(idx_s, sur.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+  if (nf) DGFS_HIP_TRY(hipMemcpyAsync(idx_f, fb.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
+  // one row buffer for the fallback inputs/outputs and the projection of the
+  // surrogate rows, sized before any launch of this call
+  {
+    const size_t need = static_cast<size_t>(std::max(nf, ns)) * h->n;
+    if (need > h->cap_fb) {
+      DGFS_HIP_TRY(hipStreamSynchronize(st));
+      (void)hipFree(h->fb_f); (void)hipFree(h->fb_Q);
+      h->fb_f = h->fb_Q = nullptr;
+      h->cap_fb = 0;
+      DGFS_HIP_TRY(hipMalloc(&h->fb_f, need * sizeof(double)));
+      DGFS_HIP_TRY(hipMalloc(&h->fb_Q, need * sizeof(double)));
+      h->cap_fb = need;
+    }
+  }
+  // exact operator on the fallback rows
+  if (nf) {
+    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), nf), dim3(256), 0, st, f, h->fb_f,
+                       idx_f, nf, h->n, 1);
+    DGFS_LAUNCH_CHECK("rows_gather");
+    rc = dgfs_vhs_eval_contig(vhs, h->fb_f, h->fb_Q, nf, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), nf), dim3(256), 0, st, h->fb_Q, Q,
+                       idx_f, nf, h->n, 0);
+    DGFS_LAUNCH_CHECK("rows_scatter");
+  }
+  // network on the surrogate rows, in batches of max_batch
+  for (int b0 = 0; b0 < ns; b0 += h->max_batch) {
+    const int nb = std::min(h->max_batch, ns - b0);
+    hipLaunchKernelGGL(k_rows_copy_f, dim3(2, nb), dim3(256), 0, st, h->xin, h->xg, idx_s + b0, nb, kSurModes);
+    DGFS_LAUNCH_CHECK("x_gather");
+    hipLaunchKernelGGL(k_rows_copy_f, dim3(1, nb), dim3(256), 0, st, h->ltin, h->ltg, idx_s + b0, nb, 1);
+    DGFS_LAUNCH_CHECK("lt_gather");
+    rc = sur_forward(h, params, h->xg, h->ltg, nb, h->yout, bf16 != 0, st);
+    if (rc) return rc;
+    DecArgs da{h->yout, h->mom, idx_s + b0, Q, h->L, h->gamma, h->prefac, nb};
+    rc = do_decode(h, da, st);
+    if (rc) return rc;
+  }
+  // FP64 projection of the surrogate rows (gather, project, scatter)
+  if (ns) {
+    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), ns), dim3(256), 0, st, Q, h->fb_Q,
+                       idx_s, ns, h->n, 1);
+    DGFS_LAUNCH_CHECK("rows_gather");
+    rc = launch_invariant_projection(h->fb_Q, h->Ainv, h->L, h->cw, h->N, ns, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), ns), dim3(256), 0, st, h->fb_Q, Q,
+                       idx_s, ns, h->n, 0);
+    DGFS_LAUNCH_CHECK("rows_scatter");
+  }
+  return DGFS_OK;
+}

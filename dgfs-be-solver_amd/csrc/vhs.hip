@@ -672,6 +672,53 @@ int launch_aosoa_scatter(const double* in, double* Q, int nupts, int ldim, int v
   return DGFS_OK;
 }
 
+// Shared by the other operators of the library: (C C^T)^{-1} of the five
+// invariants {1, c, |c|^2} on the Nv^3 grid of [-L, L]^3 with weight cw, and
+// the projection launch.
+int invariant_gram_inverse(int N, double L, double cw, double* Ainv25) {
+  double A[25] = {};
+  const double hh = 2.0 * L / N, c0 = -L + 0.5 * hh;
+  for (int r = 0; r < N * N * N; ++r) {
+    const int I = r / (N * N), J = (r / N) % N, K = r % N;
+    const double cx = c0 + I * hh, cy = c0 + J * hh, cz = c0 + K * hh;
+    const double phi[5] = {1.0, cx, cy, cz, cx * cx + cy * cy + cz * cz};
+    for (int j = 0; j < 5; ++j)
+      for (int k = 0; k < 5; ++k) A[j * 5 + k] += phi[j] * phi[k] * cw * cw;
+  }
+  double m[5][10];
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 10; ++j) m[i][j] = (j < 5) ? A[i * 5 + j] : (j - 5 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 5; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 5; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (std::fabs(m[piv][c]) < 1e-300) return fail(DGFS_ERR_INVALID_VALUE, "singular invariant Gram matrix");
+    for (int j = 0; j < 10; ++j) std::swap(m[c][j], m[piv][j]);
+    const double d = m[c][c];
+    for (int j = 0; j < 10; ++j) m[c][j] /= d;
+    for (int r = 0; r < 5; ++r)
+      if (r != c) {
+        const double fct = m[r][c];
+        for (int j = 0; j < 10; ++j) m[r][j] -= fct * m[c][j];
+      }
+  }
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) Ainv25[i * 5 + j] = m[i][j + 5];
+  return DGFS_OK;
+}
+
+int launch_invariant_projection(double* Q, const double* Ainv25, double L, double cw, int N, int npts,
+                                hipStream_t st) {
+  if (npts <= 0) return DGFS_OK;
+  ProjArgs pr;
+  pr.Q = Q;
+  std::memcpy(pr.Ainv, Ainv25, sizeof(pr.Ainv));
+  pr.L = L; pr.cw = cw; pr.N = N; pr.npts = npts;
+  hipLaunchKernelGGL(k_vhs_project, dim3(std::min(npts, 4 * cu_count())), dim3(256), 0, st, pr);
+  DGFS_LAUNCH_CHECK("invariant_projection");
+  return DGFS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------

@@ -72,6 +72,13 @@ class dgfs_vss_desc(C.Structure):
     ]
 
 
+class dgfs_sur_desc(C.Structure):
+    _fields_ = [
+        ("Nv", C.c_int), ("max_batch", C.c_int), ("L", C.c_double), ("cw", C.c_double),
+        ("gamma", C.c_double), ("prefac", C.c_double),
+    ]
+
+
 _lib = None
 
 _P, _I, _D = C.c_void_p, C.c_int, C.c_double

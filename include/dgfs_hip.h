@@ -160,6 +160,63 @@ int dgfs_vss_set_timing(void* handle, int on);
 int dgfs_vss_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);
 
 /* ------------------------------------------------------------------------
+ * Learned collision surrogate (no reference counterpart: the reference has
+ * only the spectral operator).  A point's distribution is encoded by its 8^3
+ * Hermite modes in the frame of its own Maxwellian; a 4-layer spectral
+ * network (width 32, GEMMs on MFMA, BF16 operands, FP32 accumulation) maps
+ * them to the modes of Q scaled by prefac rho T^(gamma/2); the decoded Q is
+ * projected onto the collision invariants in FP64.  Points whose
+ * non-equilibrium indicator ||f - M[f]|| / ||f|| exceeds a threshold fall back
+ * to the exact operator.  Parameters are one FP32 vector of
+ * dgfs_sur_param_count() values owned by the caller:
+ *   P0[32], P1[32], E[512][32],
+ *   4 x { K[512][512], W[32][32], R[32][32], bias[32] },
+ *   Pout[32], bout[512].
+ * ---------------------------------------------------------------------- */
+typedef struct dgfs_sur_desc {
+  int Nv;          /* velocity points per axis: 8, 12, 16, 24 or 32 */
+  int max_batch;   /* largest network batch (activation workspace) */
+  double L;        /* half-width of the velocity box */
+  double cw;       /* (2L/Nv)^3 */
+  double gamma;    /* 2 (1 - omega) */
+  double prefac;   /* collision kernel prefactor */
+} dgfs_sur_desc;
+
+long dgfs_sur_param_count(void);
+int dgfs_sur_create(const dgfs_sur_desc* desc, void** handle);
+int dgfs_sur_destroy(void* handle);
+/* out[P][512] = network(x[P][512], lt[P] = log T); bf16 = 1: BF16 MFMA
+ * operands, 0: FP32 MFMA operands.  Keeps the activations for backward. */
+int dgfs_sur_forward(void* handle, const float* params, const float* x, const float* lt, int P,
+                     float* out, int bf16, void* stream);
+/* grad = d(loss)/d(params) given dout = d(loss)/d(out) of the last forward. */
+int dgfs_sur_backward(void* handle, const float* params, const float* dout, float* grad, void* stream);
+/* One Adam step (bias-corrected, step >= 1) over n parameters. */
+int dgfs_sur_adam(long n, float* params, const float* grad, float* m1, float* m2, float lr, float b1,
+                  float b2, float eps, int step, void* stream);
+/* f[npts][Nv^3] -> mom[npts][8] (rho, ux, uy, uz, T, indicator, 0, 0),
+ * x[npts][512] Hermite modes (a_000 stored as 0), lt[npts] = log T; x, lt may be null. */
+int dgfs_sur_encode(void* handle, const double* f, int npts, double* mom, float* x, float* lt,
+                    void* stream);
+/* Training targets: Hermite modes of Q[npts][Nv^3] in the frames of mom,
+ * divided by prefac rho T^(gamma/2). */
+int dgfs_sur_targets(void* handle, const double* Q, const double* mom, int npts, float* y,
+                     void* stream);
+/* Q[idx[i]] = decoded network output row i (idx null: row i -> point i);
+ * project = 1 (idx null only) applies the FP64 invariant projection. */
+int dgfs_sur_decode(void* handle, const float* y, const double* mom, const int* idx, int nidx,
+                    double* Q, int project, void* stream);
+/* Surrogate with fallback to the exact operator `vhs` (a dgfs_vhs handle of
+ * the same grid) for points whose indicator exceeds `threshold`; synchronises
+ * the stream once to read the indicators. */
+int dgfs_sur_eval(void* handle, void* vhs, const float* params, const double* f, double* Q, int npts,
+                  double threshold, int bf16, int* nfallback, void* stream);
+/* C = alpha op(A) op(B) + beta C (row-major FP32 storage; bf16 as above);
+ * the network's GEMM, exported for its tests. */
+int dgfs_nn_gemm(int transA, int transB, int M, int N, int K, float alpha, const float* A, long lda,
+                 const float* B, long ldb, float beta, float* C, long ldc, int bf16, void* stream);
+
+/* ------------------------------------------------------------------------
  * Flux-reconstruction advection of the velocity columns and RK stage kernels.
  * Matrices are the reference's AoSoA layout: element (row, var v, elem e) at
  * row*ld + ((e/32)*nvars + v)*32 + e%32; velocities are the variables.

@@ -36,6 +36,7 @@ def test_binding_covers_declared_symbols():
     from dgfs_hip import _lib
     from dgfs_hip import fr  # noqa: F401  registers the FR entry points
     from dgfs_hip.solvers.dgfsbi import scattering  # noqa: F401  registers the VSS entry points
+    from dgfs_hip import surrogate  # noqa: F401  registers the surrogate entry points
     assert declared_symbols() <= set(_lib.exported_symbols())
 
 

@@ -1,0 +1,239 @@
+"""GPU checks of the learned collision surrogate: the MFMA GEMM, the network's
+forward and backward kernels against the FP64 checker (oracle/surrogate.py),
+the Hermite encoder / decoder, training on solver-generated (f, Q) pairs, and
+the surrogate-with-fallback evaluation.
+
+Tolerances: GEMM 1e-5 relative (operands rounded to BF16 identically on both
+sides in BF16 mode); forward 1e-4 (FP32 operands) / 1e-2 (BF16 operands,
+whose rounding of FP32 intermediates can differ from the FP64 checker's by one
+BF16 ulp); gradients 2e-3 per parameter block with FP32 operands;
+encoder / decoder 1e-10 (FP64 kernels; modes stored in FP32: 1e-6)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+@pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 70, 45, 37), (1, 0, 64, 96, 50), (0, 1, 33, 130, 64),
+                                         (1, 1, 128, 64, 17), (1, 0, 32, 32, 20000), (0, 1, 100, 80, 9000)])
+def test_gemm_matches_numpy(ta, tb, M, N, K, bf16, device):
+    import torch
+    from dgfs_hip import surrogate as H
+    rng = np.random.default_rng(M + N + K)
+    A = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    At, Bt, Ct = (torch.as_tensor(x, device=device) for x in (A, B, C0))
+    H.gemm(ta, tb, M, N, K, 0.7, At, A.shape[1], Bt, B.shape[1], 0.3, Ct, N, bf16=bf16)
+    torch.cuda.synchronize()
+    r = S.bf16 if bf16 else (lambda a: a.astype(np.float64))
+    opA = r(A).T if ta else r(A)
+    opB = r(B).T if tb else r(B)
+    ref = 0.7 * opA @ opB + 0.3 * C0.astype(np.float64)
+    e = rel(Ct.cpu().numpy(), ref)
+    say("gemm", ta, tb, M, N, K, bf16, e)
+    assert e < 1e-5
+
+
+def _sur(device, Nv=16, L=6.0, max_batch=64, seed=0, scale=0.7):
+    from dgfs_hip.surrogate import SpectralSurrogate
+    return SpectralSurrogate(Nv, L, (2 * L / Nv) ** 3, 0.38, 1.0, max_batch=max_batch, device=device, seed=seed,
+                             init_scale=scale)
+
+
+def test_forward_backward_match_checker(device):
+    import torch
+    sur = _sur(device)
+    th = S.init_params(0, scale=0.7)
+    rng = np.random.default_rng(1)
+    P = 5
+    x = (rng.normal(size=(P, S.MODES)) * 0.3).astype(np.float32)
+    lt = (rng.normal(size=P) * 0.2).astype(np.float32)
+    dout = rng.normal(size=(P, S.MODES)).astype(np.float32)
+    xt, ltt, dt = (torch.as_tensor(a, device=device) for a in (x, lt, dout))
+    out = torch.empty((P, S.MODES), dtype=torch.float32, device=device)
+    # FP32 operands
+    sur.forward(xt, ltt, P, out, bf16=False)
+    sur.backward(dt)
+    torch.cuda.synchronize()
+    ref, cache = S.forward(sur.params.cpu().numpy().astype(np.float64), x, lt)
+    e = rel(out.cpu().numpy(), ref)
+    say("forward fp32", e)
+    assert e < 1e-4
+    g_ref = S.backward(sur.params.cpu().numpy().astype(np.float64), cache, dout.astype(np.float64))
+    g = sur.grad.cpu().numpy()
+    blocks = {"P0": (S.OFF_P0, S.WIDTH), "P1": (S.OFF_P1, S.WIDTH), "E": (S.OFF_E, S.MODES * S.WIDTH)}
+    for l in range(S.LAYERS):
+        b = S.OFF_LAYERS + l * S.LAYER_SIZE
+        blocks[f"K{l}"] = (b, S.MODES * S.MODES)
+        blocks[f"W{l}"] = (b + S.MODES * S.MODES, S.WIDTH * S.WIDTH)
+        blocks[f"R{l}"] = (b + S.MODES * S.MODES + S.WIDTH * S.WIDTH, S.WIDTH * S.WIDTH)
+        blocks[f"b{l}"] = (b + S.MODES * S.MODES + 2 * S.WIDTH * S.WIDTH, S.WIDTH)
+    blocks["Pout"] = (S.OFF_POUT, S.WIDTH)
+    blocks["bout"] = (S.OFF_BOUT, S.MODES)
+    worst = 0.0
+    for name, (o, sz) in blocks.items():
+        eb = rel(g[o:o + sz], g_ref[o:o + sz])
+        worst = max(worst, eb)
+        assert eb < 2e-3, (name, eb)
+    say("backward fp32 worst block", worst)
+    # BF16 operands against the checker with the same rounding points
+    sur.forward(xt, ltt, P, out, bf16=True)
+    torch.cuda.synchronize()
+    ref16, _ = S.forward(sur.params.cpu().numpy().astype(np.float64), x, lt, round_bf16=True)
+    e16 = rel(out.cpu().numpy(), ref16)
+    say("forward bf16", e16)
+    assert e16 < 1e-2
+
+
+def near_equilibrium_states(N, L, npts, seed, eps_max=0.15):
+    """Perturbed Maxwellians M (1 + phi): heat-flux and shear terms of the
+    Chapman-Enskog form with random amplitude <= eps_max, u, T per point."""
+    rng = np.random.default_rng(seed)
+    c0 = S.grid(N, L)
+    I, J, K = np.meshgrid(c0, c0, c0, indexing="ij")
+    cv = np.stack([I.ravel(), J.ravel(), K.ravel()])
+    out = np.empty((npts, N ** 3))
+    for i in range(npts):
+        u = rng.uniform(-0.4, 0.4, 3)
+        T = rng.uniform(0.7, 1.4)
+        rho = rng.uniform(0.5, 1.5)
+        c = (cv - u[:, None]) / math.sqrt(T)
+        c2 = (c ** 2).sum(0)
+        q = rng.normal(size=3)
+        Sm = rng.normal(size=(3, 3))
+        Sm = Sm + Sm.T
+        Sm -= np.trace(Sm) / 3 * np.eye(3)
+        eps = rng.uniform(0, eps_max)
+        phi = eps * (0.4 * (q @ c) * (c2 - 2.5) + 0.5 * np.einsum("ij,ir,jr->r", Sm, c, c))
+        out[i] = rho / (math.pi * T) ** 1.5 * np.exp(-c2) * (1 + phi)
+    return out
+
+
+def test_encode_targets_decode_match_checker(device):
+    import torch
+    N, L = 16, 6.0
+    sur = _sur(device, Nv=N, L=L)
+    f = near_equilibrium_states(N, L, 6, seed=3)
+    ft = torch.as_tensor(f, device=device)
+    mom = torch.empty((6, 8), dtype=torch.float64, device=device)
+    x = torch.empty((6, S.MODES), dtype=torch.float32, device=device)
+    lt = torch.empty((6,), dtype=torch.float32, device=device)
+    sur.encode(ft, 6, mom, x, lt)
+    y = torch.empty_like(x)
+    sur.targets(ft, mom, 6, y)  # modes of f itself as a stand-in field
+    rng = np.random.default_rng(0)
+    yr = torch.as_tensor(rng.normal(size=(6, S.MODES)).astype(np.float32), device=device)
+    Qd = torch.empty_like(ft)
+    sur.decode(yr, mom, 6, Qd, project=False)
+    torch.cuda.synchronize()
+    mom_h, x_h, lt_h, y_h, Qd_h = (t.cpu().numpy() for t in (mom, x, lt, y, Qd))
+    for i in range(6):
+        m_ref, a_ref, lt_ref = S.encode(f[i], N, L)
+        assert np.max(np.abs(mom_h[i, :6] - m_ref) / np.maximum(np.abs(m_ref), 1e-3)) < 1e-10
+        assert np.max(np.abs(x_h[i] - a_ref)) < 1e-6 * max(1.0, np.abs(a_ref).max())
+        assert abs(lt_h[i] - lt_ref) < 1e-6
+        t_ref = S.targets(f[i], m_ref, N, L, 0.38, 1.0)
+        assert np.max(np.abs(y_h[i] - t_ref)) < 1e-6 * max(1.0, np.abs(t_ref).max())
+        d_ref = S.decode(yr[i].cpu().numpy(), m_ref, N, L, 0.38, 1.0)
+        assert rel(Qd_h[i], d_ref) < 1e-10
+    say("encode/targets/decode ok")
+
+
+def _exact_op(N, L, Nrho, M, gamma, precision=64, project=True):
+    from dgfs_hip.solvers.dgfs.scattering import VHSOperator
+    from dgfs_hip.solvers.dgfs.velocitymesh import DGFSVelocityMesh
+    vm = DGFSVelocityMesh.from_params(None, N, L, Nrho, M)
+    return VHSOperator.from_mesh(vm, gamma, 1.0, precision=precision, project=project)
+
+
+def test_training_gains_over_untrained_and_bgk(device):
+    """The network trained on the exact operator's (f, Q) pairs predicts Q of
+    held-out near-equilibrium states better than the untrained network and
+    better than the BGK closure nu (M[f] - f) with the best single nu."""
+    import torch
+    N, L, Nrho, M, gamma = 16, 6.0, 8, 6, 0.38
+    op = _exact_op(N, L, Nrho, M, gamma)
+    ntr, nte = 3072, 128
+    f = torch.as_tensor(near_equilibrium_states(N, L, ntr + nte, seed=7), device=device)
+    Q = torch.empty_like(f)
+    op.eval_contig(f.data_ptr(), Q.data_ptr(), ntr + nte, torch.cuda.current_stream().cuda_stream)
+    sur = _sur(device, Nv=N, L=L, max_batch=256, scale=0.3)
+    x, lt, y, mom = sur.make_dataset(f, Q)
+
+    def test_err():
+        yp = sur.predict(x[ntr:], lt[ntr:])
+        Qs = torch.empty_like(f[ntr:])
+        sur.decode(yp, mom[ntr:].contiguous(), nte, Qs, project=True)
+        return float(torch.linalg.norm(Qs - Q[ntr:]) / torch.linalg.norm(Q[ntr:]))
+
+    e0 = test_err()
+    losses = sur.train(x[:ntr], lt[:ntr], y[:ntr], epochs=25, batch=128, lr=2e-3, log=say)
+    e1 = test_err()
+    # BGK with the least-squares optimal single frequency over the test set
+    c0 = torch.as_tensor(S.grid(N, L), device=device)
+    I, J, K = torch.meshgrid(c0, c0, c0, indexing="ij")
+    mo = mom[ntr:]
+    Mx = mo[:, 0:1] / (math.pi * mo[:, 4:5]) ** 1.5 * torch.exp(
+        -((I.reshape(1, -1) - mo[:, 1:2]) ** 2 + (J.reshape(1, -1) - mo[:, 2:3]) ** 2
+          + (K.reshape(1, -1) - mo[:, 3:4]) ** 2) / mo[:, 4:5])
+    d = Mx - f[ntr:]
+    nu = float((d * Q[ntr:]).sum() / (d * d).sum())
+    e_bgk = float(torch.linalg.norm(nu * d - Q[ntr:]) / torch.linalg.norm(Q[ntr:]))
+    say("surrogate test error: untrained", e0, "trained", e1, "bgk", e_bgk, "loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
+    assert e1 < 0.5 * e0
+    assert e1 < e_bgk
+
+
+def test_eval_with_fallback_routes_points(device):
+    import torch
+    N, L, Nrho, M, gamma = 16, 6.0, 8, 6, 0.38
+    op = _exact_op(N, L, Nrho, M, gamma)
+    sur = _sur(device, Nv=N, L=L, max_batch=16)
+    f = torch.as_tensor(near_equilibrium_states(N, L, 40, seed=11, eps_max=0.3), device=device)
+    st = torch.cuda.current_stream().cuda_stream
+    Qx = torch.empty_like(f)
+    op.eval_contig(f.data_ptr(), Qx.data_ptr(), 40, st)
+    Qa = torch.full_like(f, float("nan"))
+    nfb = sur.eval_with_fallback(op, f, Qa, 40, threshold=-1.0)
+    torch.cuda.synchronize()
+    assert nfb == 40
+    assert float((Qa - Qx).abs().max()) == 0.0
+    # all surrogate: equals encode -> network -> decode -> projection
+    Qb = torch.full_like(f, float("nan"))
+    nfb = sur.eval_with_fallback(op, f, Qb, 40, threshold=1e9)
+    x, lt, y, mom = sur.make_dataset(f, Qx)
+    yp = sur.predict(x, lt)
+    Qc = torch.empty_like(f)
+    sur.decode(yp, mom, 40, Qc, project=True)
+    torch.cuda.synchronize()
+    assert nfb == 0
+    assert rel(Qb.cpu().numpy(), Qc.cpu().numpy()) < 1e-12
+    # mixed: the points above the threshold are exact
+    ind = mom[:, 5].cpu().numpy()
+    thr = float(np.median(ind))
+    Qm = torch.full_like(f, float("nan"))
+    nfb = sur.eval_with_fallback(op, f, Qm, 40, threshold=thr)
+    torch.cuda.synchronize()
+    hi = ind > thr
+    assert nfb == int(hi.sum())
+    Qm_h, Qx_h, Qc_h = Qm.cpu().numpy(), Qx.cpu().numpy(), Qc.cpu().numpy()
+    assert np.array_equal(Qm_h[hi], Qx_h[hi])
+    assert rel(Qm_h[~hi], Qc_h[~hi]) < 1e-12
+    say("fallback routing ok", nfb)
