@@ -7,14 +7,14 @@
 // the frame of its own Maxwellian M[f] (density rho, velocity u, temperature
 // T from the discrete moments; DGFS convention M = rho (pi T)^-3/2
 // exp(-|c-u|^2/T), so the per-axis variance is s^2 = T/2).  With
-// c' = (c - u)/s and the probabilists' Hermite polynomials He_n,
-//   a_n = (1/(n! rho)) sum_r f(r) He_n(c'(r)) cw,   n = (nx, ny, nz), n_d < 8,
-// the 8^3 retained modes of f (Grad's expansion f ~ M sum_n a_n He_n).  By
-// construction a_000 = 1 and the momentum / energy modes vanish.  The
-// collision operator's modes b_n (same formula with Q) scale as
-// prefac rho T^{gamma/2} in the continuum, so the network maps the
-// dimensionless a to b^ = b / (prefac rho T^{gamma/2}).
-//
+// c' = (c - u)/s and the probabilists' Hermite polynomials He_n, the 8^3
+// retained modes a_n (n = (nx, ny, nz), n_d < 8) are the grid least-squares
+// coefficients of f in the basis M psi_n(c'), psi_n = prod_d He_{n_d}/sqrt(n_d!)
+// (Grad's expansion f ~ M sum_n a_n psi_n; a_000 = 1 for f = M).  The basis decays like M, so
+// round-off far in the tails never reaches the modes.  The collision
+// operator's modes b_n (same projection of Q) scale as prefac rho T^{gamma/2}
+// in the continuum, so the network maps the dimensionless a to
+// b^ = b / (prefac rho T^{gamma/2}).
 // Network (8^3 = 512 modes m, width 32 channels c, activations Z[m][p][c]):
 //   Z0[m][p][c] = P0[c] x[p][m] + P1[c] log T[p] + E[m][c]
 //   U_l = K_l Z_l            (mode mixing, K_l 512x512, one GEMM over all points)
@@ -384,16 +384,23 @@ struct EncArgs {
   int npts;
 };
 
+// 1/sqrt(a!): the basis uses He_a / sqrt(a!), which keeps the per-axis Gram
+// matrices well conditioned (about 1e3 instead of 4e4 at degree 7)
+__constant__ double kInvSqrtFact[kSurD] = {1.0, 1.0, 0.70710678118654752, 0.40824829046386302,
+                                           0.20412414523193151, 0.091287092917527686, 0.037267799624996495,
+                                           0.014085904245475275};
+
+// V[i*8 + a] = He_a(x_i) / sqrt(a!) exp(-x_i^2 / 2), x_i = (c0 + i h - u) / s:
+// the 1D factors of the basis functions (exp(-x^2/2) = exp(-(c-u)^2/T))
 __device__ __forceinline__ void herm_table(double* V, double c0, double h, double u, double s, int N) {
-  // V[i*8 + a] = He_a((c0 + i h - u) / s), probabilists' recurrence
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const double x = (c0 + i * h - u) / s;
+    const double x = (c0 + i * h - u) / s, w = exp(-0.5 * x * x);
     double p0 = 1.0, p1 = x;
-    V[i * kSurD + 0] = p0;
-    V[i * kSurD + 1] = p1;
+    V[i * kSurD + 0] = w * p0 * kInvSqrtFact[0];
+    V[i * kSurD + 1] = w * p1 * kInvSqrtFact[1];
     for (int a = 1; a + 1 < kSurD; ++a) {
       const double p2 = x * p1 - a * p0;
-      V[i * kSurD + a + 1] = p2;
+      V[i * kSurD + a + 1] = w * p2 * kInvSqrtFact[a + 1];
       p0 = p1;
       p1 = p2;
     }
@@ -413,14 +420,56 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return r;
 }
 
-__constant__ double kInvFact[kSurD] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0, 1.0 / 5040.0};
-
-// the 8^3 Hermite moments of g in the frame (u, s), divided by n! and `scale`:
-// out[abc] = sum_ijk g[i][j][k] Vx[i][a] Vy[j][b] Vz[k][c] cw / (a! b! c! scale)
+// Ginv[d] = (V_d^T V_d)^{-1}, 8 x 8 per axis (Gauss-Jordan with partial
+// pivoting, one thread per axis; work holds 3 x 8 x 16 doubles)
 template <int N>
-__device__ void herm_moments(const double* __restrict__ g, const double* Vx, const double* Vy, const double* Vz,
-                             double* t1, double* t2, double scale_cw, float* out, double* dout) {
+__device__ void herm_gram_inverse(const double* V, double* Ginv, double* work) {
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    const double* Vd = V + d * N * kSurD;
+    double* w = work + d * kSurD * 2 * kSurD;
+    for (int a = 0; a < kSurD; ++a)
+      for (int b = 0; b < kSurD; ++b) {
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s += Vd[i * kSurD + a] * Vd[i * kSurD + b];
+        w[a * 2 * kSurD + b] = s;
+        w[a * 2 * kSurD + kSurD + b] = (a == b) ? 1.0 : 0.0;
+      }
+    for (int c = 0; c < kSurD; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < kSurD; ++r)
+        if (fabs(w[r * 2 * kSurD + c]) > fabs(w[piv * 2 * kSurD + c])) piv = r;
+      if (piv != c)
+        for (int j = 0; j < 2 * kSurD; ++j) {
+          const double t = w[c * 2 * kSurD + j];
+          w[c * 2 * kSurD + j] = w[piv * 2 * kSurD + j];
+          w[piv * 2 * kSurD + j] = t;
+        }
+      const double dg = w[c * 2 * kSurD + c];
+      for (int j = 0; j < 2 * kSurD; ++j) w[c * 2 * kSurD + j] /= dg;
+      for (int r = 0; r < kSurD; ++r)
+        if (r != c) {
+          const double fct = w[r * 2 * kSurD + c];
+          for (int j = 0; j < 2 * kSurD; ++j) w[r * 2 * kSurD + j] -= fct * w[c * 2 * kSurD + j];
+        }
+    }
+    for (int a = 0; a < kSurD; ++a)
+      for (int b = 0; b < kSurD; ++b) Ginv[d * kSurD * kSurD + a * kSurD + b] = w[a * 2 * kSurD + kSurD + b];
+  }
+  __syncthreads();
+}
+
+// Least-squares (grid L2) coefficients of g in the basis M He_n, n_d < 8:
+// b = (Gx^-1 (x) Gy^-1 (x) Gz^-1) sum_r g(r) V_x V_y V_z / (C scale), with
+// M = C prod_d exp(-(c_d - u_d)^2 / T).  The basis functions decay like the
+// Maxwellian, so noise of g far in the tails does not reach the modes.
+template <int N>
+__device__ void herm_project(const double* __restrict__ g, const double* V, const double* Ginv, double* t1,
+                             double* t2, double* b3, double inv_scale, float* out, float out_shift0) {
   constexpr int NN = N * N;
+  const double* Vx = V;
+  const double* Vy = V + N * kSurD;
+  const double* Vz = V + 2 * N * kSurD;
   for (int e = threadIdx.x; e < kSurD * NN; e += blockDim.x) {  // t1[a][j][k]
     const int a = e / NN, jk = e - a * NN;
     double s = 0.0;
@@ -435,25 +484,56 @@ __device__ void herm_moments(const double* __restrict__ g, const double* Vx, con
     t2[e] = s;
   }
   __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {  // B[a][b][c] -> b3
+    const int ab = e / kSurD, c = e % kSurD;
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += t2[ab * N + k] * Vz[k * kSurD + c];
+    b3[e] = s;
+  }
+  __syncthreads();
+  // apply the inverse Gram matrices axis by axis (b3 -> t1 -> t2 -> out)
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int a = e / (kSurD * kSurD), bc = e % (kSurD * kSurD);
+    double s = 0.0;
+    for (int q = 0; q < kSurD; ++q) s += Ginv[a * kSurD + q] * b3[q * kSurD * kSurD + bc];
+    t1[e] = s;
+  }
+  __syncthreads();
   for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
     const int a = e / (kSurD * kSurD), b = (e / kSurD) % kSurD, c = e % kSurD;
     double s = 0.0;
-    for (int k = 0; k < N; ++k) s += t2[(a * kSurD + b) * N + k] * Vz[k * kSurD + c];
-    const double v = s * scale_cw * kInvFact[a] * kInvFact[b] * kInvFact[c];
-    if (out) out[e] = static_cast<float>(v);
-    if (dout) dout[e] = v;
+    for (int q = 0; q < kSurD; ++q) s += Ginv[kSurD * kSurD + b * kSurD + q] * t1[(a * kSurD + q) * kSurD + c];
+    t2[e] = s;
   }
   __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int ab = e / kSurD, c = e % kSurD;
+    double s = 0.0;
+    for (int q = 0; q < kSurD; ++q) s += Ginv[2 * kSurD * kSurD + c * kSurD + q] * t2[ab * kSurD + q];
+    out[e] = static_cast<float>(s * inv_scale - (e == 0 ? out_shift0 : 0.0));
+  }
+  __syncthreads();
+}
+
+// LDS of the encoder / target kernels: t1 (8 N^2), t2 (64 N), V (24 N),
+// Ginv (192), work (384), b3 (512), red (256) doubles
+template <int N>
+constexpr size_t enc_lds_doubles() {
+  return static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + 3 * kSurD * kSurD +
+         3 * kSurD * 2 * kSurD + kSurModes + 256;
 }
 
 template <int N>
 __global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
   constexpr int n = N * N * N, NN = N * N;
   extern __shared__ double sm[];
-  double* t1 = sm;                      // kSurD * N * N
-  double* t2 = t1 + kSurD * NN;         // kSurD * kSurD * N
-  double* V = t2 + kSurD * kSurD * N;   // 3 * N * kSurD
-  double* red = V + 3 * N * kSurD;      // 256
+  double* t1 = sm;                        // kSurD * N * N
+  double* t2 = t1 + kSurD * NN;           // kSurD * kSurD * N
+  double* V = t2 + kSurD * kSurD * N;     // 3 * N * kSurD
+  double* Ginv = V + 3 * N * kSurD;       // 3 * 64
+  double* work = Ginv + 3 * kSurD * kSurD;  // 3 * 128
+  double* b3 = work + 3 * kSurD * 2 * kSurD;  // 512
+  double* red = b3 + kSurModes;           // 256
   const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
   const double pi = 3.14159265358979323846;
   for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
@@ -492,15 +572,15 @@ __global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
       herm_table(V + N * kSurD, c0, hh, uy, s, N);
       herm_table(V + 2 * N * kSurD, c0, hh, uz, s, N);
       __syncthreads();
-      float* xo = a.x + static_cast<size_t>(pt) * kSurModes;
-      herm_moments<N>(f, V, V + N * kSurD, V + 2 * N * kSurD, t1, t2, a.cw / rho, xo, nullptr);
-      if (threadIdx.x == 0) xo[0] = 0.f;  // a_000 = 1 by construction
+      herm_gram_inverse<N>(V, Ginv, work);
+      // f ~ M sum a_n He_n; a_000 = 1 for f = M, stored as a_000 - 1
+      herm_project<N>(f, V, Ginv, t1, t2, b3, 1.0 / Mn, a.x + static_cast<size_t>(pt) * kSurModes, 1.0f);
     }
     __syncthreads();
   }
 }
 
-// targets: b^ = (Hermite moments of Q in the frame of mom) / (prefac rho T^{gamma/2})
+// targets: b^ = (L2 Hermite coefficients of Q in the frame of mom) / (prefac rho T^{gamma/2})
 struct TgtArgs {
   const double* Q;
   const double* mom;
@@ -516,7 +596,11 @@ __global__ __launch_bounds__(256) void k_sur_targets(TgtArgs a) {
   double* t1 = sm;
   double* t2 = t1 + kSurD * NN;
   double* V = t2 + kSurD * kSurD * N;
+  double* Ginv = V + 3 * N * kSurD;
+  double* work = Ginv + 3 * kSurD * kSurD;
+  double* b3 = work + 3 * kSurD * 2 * kSurD;
   const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
+  const double pi = 3.14159265358979323846;
   for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
     const double* mo = a.mom + static_cast<size_t>(pt) * 8;
     const double rho = mo[0], T = mo[4], s = sqrt(0.5 * T);
@@ -524,13 +608,15 @@ __global__ __launch_bounds__(256) void k_sur_targets(TgtArgs a) {
     herm_table(V + N * kSurD, c0, hh, mo[2], s, N);
     herm_table(V + 2 * N * kSurD, c0, hh, mo[3], s, N);
     __syncthreads();
-    const double scale = a.cw / (rho * a.prefac * rho * pow(T, 0.5 * a.gamma));
-    herm_moments<N>(a.Q + static_cast<size_t>(pt) * n, V, V + N * kSurD, V + 2 * N * kSurD, t1, t2, scale,
-                    a.y + static_cast<size_t>(pt) * kSurModes, nullptr);
+    herm_gram_inverse<N>(V, Ginv, work);
+    const double Mn = rho / pow(pi * T, 1.5);
+    const double inv = 1.0 / (Mn * a.prefac * rho * pow(T, 0.5 * a.gamma));
+    herm_project<N>(a.Q + static_cast<size_t>(pt) * n, V, Ginv, t1, t2, b3, inv,
+                    a.y + static_cast<size_t>(pt) * kSurModes, 0.0f);
   }
 }
 
-// decode: Q[idx[i]] = M sum_n b_n He_n(c'), b = prefac rho T^{gamma/2} y with the
+// decode: Q[idx[i]] = M sum_n b_n psi_n(c'), b = prefac rho T^{gamma/2} y with the
 // mass, momentum and energy modes removed (b_000 = b_100 = b_010 = b_001 = 0,
 // b_200 + b_020 + b_002 = 0)
 struct DecArgs {
@@ -584,17 +670,15 @@ __global__ __launch_bounds__(256) void k_sur_decode(DecArgs a) {
       s2[e] = t;
     }
     __syncthreads();
+    // the tables carry the per-axis Gaussian factors of M
     const double Mn = rho / pow(pi * T, 1.5);
     double* Q = a.Q + static_cast<size_t>(pt) * n;
     for (int jk = threadIdx.x; jk < NN; jk += blockDim.x) {
-      const int j = jk / N, k = jk - j * N;
-      const double cy = c0 + j * hh - uy, cz = c0 + k * hh - uz;
       for (int ii = 0; ii < N; ++ii) {
-        const double cx = c0 + ii * hh - ux;
         double t = 0.0;
 #pragma unroll
         for (int aa = 0; aa < kSurD; ++aa) t += s2[aa * NN + jk] * V[ii * kSurD + aa];
-        Q[ii * NN + jk] = Mn * exp(-(cx * cx + cy * cy + cz * cz) / T) * t;
+        Q[ii * NN + jk] = Mn * t;
       }
     }
     __syncthreads();
@@ -771,7 +855,7 @@ static int sur_backward(SurHandle* h, const float* th, const float* dout, float*
 
 template <int N>
 static size_t enc_lds() {
-  return (static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + 256) * sizeof(double);
+  return enc_lds_doubles<N>() * sizeof(double);
 }
 template <int N>
 static size_t dec_lds() {

@@ -41,7 +41,7 @@ class Queue(list):
 class HIPBackend:
     name = "hip"
 
-    def __init__(self, cfg=None, device=None):
+    def __init__(self, cfg=None, device=None, own_stream=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("HIPBackend needs a visible HIP device")
@@ -60,7 +60,13 @@ class HIPBackend:
             raise ValueError("the distribution registers are FP64 (precision = double)")
         self.fpdtype = np.float64
         self._providers = []
-        self._stream = torch.cuda.current_stream(self.device)
+        # kernels run on torch's current stream, or on a stream of their own
+        # ([backend-hip] stream = own), which HIP graph capture of the time
+        # steps needs (the legacy default stream cannot be captured)
+        if own_stream is None:
+            own_stream = cfg is not None and cfg.get("backend-hip", "stream", "default") == "own"
+        self._stream = torch.cuda.Stream(self.device) if own_stream else torch.cuda.current_stream(self.device)
+        self.own_stream = bool(own_stream)
 
     @property
     def stream_ptr(self):
@@ -70,7 +76,11 @@ class HIPBackend:
         import torch
         tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
                np.dtype(np.int32): torch.int32}[np.dtype(dtype)]
-        return torch.zeros(shape, dtype=tdt, device=self.device)
+        t = torch.zeros(shape, dtype=tdt, device=self.device)
+        if self.own_stream:
+            # the fill ran on torch's stream; the solver's stream must see it
+            torch.cuda.synchronize(self.device)
+        return t
 
     def matrix(self, ioshape, initval=None, extent=None, aliases=None, tags=set()):
         return Matrix(self, ioshape, initval, extent, aliases, tags)

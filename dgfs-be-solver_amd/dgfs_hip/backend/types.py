@@ -75,6 +75,8 @@ class MatrixBase:
         return ary
 
     def get(self):
+        # the solver may run on its own stream: wait for all of the device's work
+        self.backend.synchronize()
         host = self.data.detach().cpu().numpy()[:, : self.ncol]
         return self._unpack(np.ascontiguousarray(host))
 
@@ -84,6 +86,7 @@ class MatrixBase:
         buf = np.zeros((self.nrow, self.leaddim), dtype=self.dtype)
         buf[:, : self.ncol] = packed
         self.data.copy_(torch.from_numpy(buf))
+        self.backend.synchronize()
 
 
 class Matrix(MatrixBase):

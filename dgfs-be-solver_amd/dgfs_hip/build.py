@@ -19,7 +19,7 @@ LIB_NAME = "libdgfs_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("DGFS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["vhs.hip", "vss.hip", "surrogate.hip", "fr.hip"]
+SOURCES = ["vhs.hip", "vss.hip", "surrogate.hip", "fr.hip", "runtime.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -147,11 +147,16 @@ class SpectralSurrogate:
 
     @staticmethod
     def mode_weights(device):
-        """Loss weight of mode (a, b, c): a! b! c!, the squared norm of
-        M He_n in the weighted space Grad's expansion is orthogonal in."""
+        """Loss weight of mode n = (a, b, c): prod_d int exp(-x^2) psi_{n_d}(x)^2 dx
+        with psi_a = He_a / sqrt(a!), the diagonal of the continuum Gram matrix
+        of the basis, so the weighted error approximates the L2 error of the
+        decoded field (normalised to mean 1)."""
         import torch
-        fact = [math.factorial(k) for k in range(8)]
-        w = np.array([fact[a] * fact[b] * fact[c] for a in range(8) for b in range(8) for c in range(8)], float)
+        from numpy.polynomial.hermite_e import hermevander
+        x = np.linspace(-12.0, 12.0, 4001)
+        V = hermevander(x, 7) / np.sqrt([math.factorial(a) for a in range(8)])
+        g1 = (np.exp(-x * x)[:, None] * V * V).sum(0) * (x[1] - x[0])
+        w = np.einsum("a,b,c->abc", g1, g1, g1).ravel()
         return torch.as_tensor(w / w.mean(), dtype=torch.float32, device=device)
 
     def train(self, x, lt, y, epochs=10, batch=256, lr=1e-3, bf16=True, seed=0, log=None, lr_min=1e-5):

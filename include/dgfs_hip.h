@@ -160,6 +160,18 @@ int dgfs_vss_set_timing(void* handle, int on);
 int dgfs_vss_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);
 
 /* ------------------------------------------------------------------------
+ * HIP graphs of time steps.  The reference launches every kernel of a step
+ * from Python (integrators/std/steppers.py:122-225); here a step is captured
+ * once on the solver's (non-default) stream and replayed with one launch.
+ * No reference counterpart.
+ * ---------------------------------------------------------------------- */
+int dgfs_graph_begin(void* stream);
+int dgfs_graph_end(void* stream, void** graph);
+int dgfs_graph_launch(void* graph, void* stream);
+int dgfs_graph_nodes(void* graph, long* count);
+int dgfs_graph_destroy(void* graph);
+
+/* ------------------------------------------------------------------------
  * Learned collision surrogate (no reference counterpart: the reference has
  * only the spectral operator).  A point's distribution is encoded by its 8^3
  * Hermite modes in the frame of its own Maxwellian; a 4-layer spectral

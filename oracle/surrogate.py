@@ -8,19 +8,20 @@ restated here is the surrogate's own definition:
 
 * Hermite encoder: moments rho, u, T of f (DGFS Maxwellian convention of
   frfs/solvers/dgfs/initcond.py:55-58, per-axis variance T/2), indicator
-  ||f - M[f]|| / ||f||, and a_n = sum f He_n(c') cw / (n! rho) for the 8^3
-  modes n_d < 8, c' = (c - u)/sqrt(T/2);
-* targets b^ = (Hermite modes of Q) / (prefac rho T^{gamma/2});
-* decoder Q = M[f] sum_n b_n He_n(c') with the mass, momentum and energy
+  ||f - M[f]|| / ||f||, and the grid least-squares coefficients a_n of f in
+  the basis M psi_n(c'), psi_n = prod He_{n_d}/sqrt(n_d!), n_d < 8,
+  c' = (c - u)/sqrt(T/2) (a_000 stored minus 1);
+* targets b^ = (the same coefficients of Q) / (prefac rho T^{gamma/2});
+* decoder Q = M[f] sum_n b_n psi_n(c') with the mass, momentum and energy
   modes removed;
 * the 4-layer spectral network and its gradient (reverse mode written out by
   hand).
 
 Pins (tests/test_oracle_surrogate.py): the backward pass against central
 finite differences of the forward pass; the encoder against exact moments of
-Maxwellians and the Hermite orthogonality (decode(encode(M p)) = M p for
-polynomials p of degree < 8 per axis); the invariant removal against the
-discrete moments of the decoded field.
+Maxwellians; the projection reproduces any field of the basis span
+(coefficients of decode(y) are y); the invariant removal against the discrete
+moments of the decoded field.
 """
 from __future__ import annotations
 
@@ -39,7 +40,6 @@ OFF_LAYERS = OFF_E + MODES * WIDTH
 OFF_POUT = OFF_LAYERS + LAYERS * LAYER_SIZE
 OFF_BOUT = OFF_POUT + WIDTH
 NPARAMS = OFF_BOUT + MODES
-FACT = np.array([math.factorial(k) for k in range(D)], dtype=float)
 
 
 def unpack(th):
@@ -168,30 +168,42 @@ def moments(f, N, L):
     return rho, u, T, ind, M.ravel()
 
 
+SQRT_FACT = np.sqrt([math.factorial(a) for a in range(D)])
+
+
 def _vander(N, L, u, T):
+    """Per-axis tables He_a(x)/sqrt(a!) exp(-x^2/2), x = (c - u)/sqrt(T/2):
+    the 1D factors of the basis M psi_n, psi_n = prod_d He_{n_d}/sqrt(n_d!)."""
     c0 = grid(N, L)
     s = math.sqrt(0.5 * T)
-    return [hermevander((c0 - u[d]) / s, D - 1) for d in range(3)]
+    out = []
+    for d in range(3):
+        x = (c0 - u[d]) / s
+        out.append(hermevander(x, D - 1) / SQRT_FACT[None, :] * np.exp(-0.5 * x * x)[:, None])
+    return out
 
 
-def hermite_modes(g, N, L, u, T, scale):
-    """sum g He_a He_b He_c cw / (a! b! c! scale), shape (8, 8, 8)."""
-    cw = (2.0 * L / N) ** 3
+def hermite_project(g, N, L, u, T, C, scale):
+    """Grid least-squares coefficients b of g ~ C prod_d exp(-x_d^2/2) sum_n
+    b_n He_n, divided by scale; shape (8, 8, 8)."""
     V = _vander(N, L, u, T)
-    b = np.einsum("ijk,ia,jb,kc->abc", g.reshape(N, N, N), V[0], V[1], V[2]) * cw / scale
-    return b / (FACT[:, None, None] * FACT[None, :, None] * FACT[None, None, :])
+    Gi = [np.linalg.inv(v.T @ v) for v in V]
+    B = np.einsum("ijk,ia,jb,kc->abc", g.reshape(N, N, N), V[0], V[1], V[2])
+    return np.einsum("pa,qb,rc,abc->pqr", Gi[0], Gi[1], Gi[2], B) / (C * scale)
 
 
 def encode(f, N, L):
     rho, u, T, ind, _ = moments(f, N, L)
-    a = hermite_modes(f, N, L, u, T, rho).ravel()
-    a[0] = 0.0
+    C = rho / (math.pi * T) ** 1.5
+    a = hermite_project(f, N, L, u, T, C, 1.0).ravel()
+    a[0] -= 1.0
     return np.array([rho, *u, T, ind]), a, math.log(T)
 
 
 def targets(Q, mom, N, L, gamma, prefac):
     rho, u, T = mom[0], mom[1:4], mom[4]
-    return hermite_modes(Q, N, L, u, T, rho * prefac * rho * T ** (0.5 * gamma)).ravel()
+    C = rho / (math.pi * T) ** 1.5
+    return hermite_project(Q, N, L, u, T, C, prefac * rho * T ** (0.5 * gamma)).ravel()
 
 
 def decode(y, mom, N, L, gamma, prefac, remove_invariants=True):
@@ -204,10 +216,8 @@ def decode(y, mom, N, L, gamma, prefac, remove_invariants=True):
         b[0, 0, 2] -= tr
         b[0, 0, 0] = b[1, 0, 0] = b[0, 1, 0] = b[0, 0, 1] = 0.0
     V = _vander(N, L, u, T)
-    c0 = grid(N, L)
-    I, J, K = np.meshgrid(c0, c0, c0, indexing="ij")
-    M = rho / (math.pi * T) ** 1.5 * np.exp(-((I - u[0]) ** 2 + (J - u[1]) ** 2 + (K - u[2]) ** 2) / T)
-    return (M * np.einsum("abc,ia,jb,kc->ijk", b, V[0], V[1], V[2])).ravel()
+    C = rho / (math.pi * T) ** 1.5
+    return (C * np.einsum("abc,ia,jb,kc->ijk", b, V[0], V[1], V[2])).ravel()
 
 
 def project(Q, N, L):

@@ -53,25 +53,27 @@ def test_encoder_of_maxwellian(N, L):
     rho, u, T = 1.3, np.array([0.3, -0.2, 0.1]), 1.1
     M = rho / (math.pi * T) ** 1.5 * np.exp(-((I - u[0]) ** 2 + (J - u[1]) ** 2 + (K - u[2]) ** 2) / T)
     mom, a, lt = S.encode(M.ravel(), N, L)
-    # midpoint sums of a Gaussian converge spectrally in h: measured errors
-    # 4e-8 (moments) / 6e-7 (modes) at h = 0.75, 7e-13 / 1e-11 at h = 0.58
+    # midpoint sums of a Gaussian converge spectrally in h: measured moment
+    # errors 4e-8 at h = 0.75, 7e-13 at h = 0.58; the modes of a Maxwellian
+    # (in the basis span up to those moment errors) follow them
     tol = 1e-7 if N == 16 else 2e-12
     assert abs(mom[0] - rho) < tol and np.max(np.abs(mom[1:4] - u)) < tol and abs(mom[4] - T) < tol
     assert mom[5] < tol and abs(lt - math.log(T)) < tol
-    assert np.max(np.abs(a)) < 20 * tol
+    assert np.max(np.abs(a)) < 100 * tol
 
 
-def test_hermite_modes_invert_decoder():
-    """decode(modes(M p)) = M p for p a random combination of He_a He_b He_c,
-    a, b, c < 8 (Hermite orthogonality under the Gaussian weight; the grid
-    quadrature of degree-14 products is exact to 7e-10 at h = 0.44, T = 1.2)."""
-    N, L = 32, 7.0
+@pytest.mark.parametrize("N,L,T", [(16, 6.0, 1.0), (32, 7.0, 1.2), (24, 7.0, 2.0)])
+def test_projection_reproduces_the_basis_span(N, L, T):
+    """targets(decode(y)) = y for any coefficients y: the grid least-squares
+    projection is exact on the span of M psi_n, on coarse and fine grids (to
+    the conditioning of the per-axis Gram matrices, about 1e3-1e4)."""
     rng = np.random.default_rng(2)
-    mom = np.array([0.9, 0.2, 0.1, -0.3, 1.2, 0.0])
+    mom = np.array([0.9, 0.2, 0.1, -0.3, T, 0.0])
     y = rng.normal(size=S.MODES) * 0.1
     g = S.decode(y, mom, N, L, gamma=0.38, prefac=1.7, remove_invariants=False)
-    back = S.hermite_modes(g, N, L, mom[1:4], mom[4], mom[0] * 1.7 * mom[0] * mom[4] ** 0.19)
-    assert np.max(np.abs(back.ravel() - y)) < 1e-8
+    back = S.targets(g, mom, N, L, 0.38, 1.7)
+    # measured 1.6e-7 at h = 0.75 (per-axis Gram condition 9e3), 2e-9 at h <= 0.58
+    assert np.max(np.abs(back - y)) < (1e-6 if N == 16 else 1e-8)
 
 
 def test_decoder_removes_invariants():

@@ -182,6 +182,7 @@ def test_training_gains_over_untrained_and_bgk(device):
         sur.decode(yp, mom[ntr:].contiguous(), nte, Qs, project=True)
         return float(torch.linalg.norm(Qs - Q[ntr:]) / torch.linalg.norm(Q[ntr:]))
 
+    say("target modes: max |y|", float(y.abs().max()), "rms", float(y.pow(2).mean().sqrt()))
     e0 = test_err()
     losses = sur.train(x[:ntr], lt[:ntr], y[:ntr], epochs=25, batch=128, lr=2e-3, log=say)
     e1 = test_err()
