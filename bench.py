@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--rhs-steps", type=int, default=3,
+                    help="timed SSP-RK2 steps of the 16x16 p=3 cavity (two RHS evaluations each); 0 skips")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -68,6 +70,76 @@ def calibrate_copy(mib, device):
     print(f"# copy calibration: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
     del a, b
     torch.cuda.empty_cache()
+
+
+def bench_rhs(args, dev):
+    """RHS evaluations/s of the lid-driven cavity: 16x16 quads, p = 3 (4096
+    solution points), Nv = 32 on [-6, 6]^3, M = 12, Nrho = 16, omega = 0.81,
+    Kn = 0.05, lid speed 0.3; FP32 collision operator with FP64 projection.
+    One RHS evaluation = collide over all points + advection of all velocity
+    columns + boundary update; an SSP-RK2 step is two.  Timed as HIP-graph
+    replays of whole steps (one launch per step) and as eager steps."""
+    import numpy as np
+    import torch
+
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import cavity_cfg
+    from dgfs_hip.integrators.graphs import GraphStepper
+    from dgfs_hip.integrators.steppers import get_stepper
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+
+    cfg = cavity_cfg(Nv=args.nv, L=args.L, Nrho=args.nrho, M=args.M, Kn=0.05, omega=args.omega, order=3,
+                     ulid=0.3, precision="single", project=True)
+    be = HIPBackend(cfg, own_stream=True)
+    mesh = rect_quads(16, 16, 0.0, 1.0, 0.0, 1.0)
+    sysm = DGFSSystem(be, mesh, cfg, 3)
+    dt = 0.5 * sysm.unit_cfl_dt
+    st = get_stepper("dgfs-tvd-rk2", be, sysm, dt)
+    gs = GraphStepper(st)
+    out = {"workload": "cavity 16x16 quads p=3 (4096 points), Nv=32, M=12, Nrho=16, SSP-RK2, "
+                       "FP32 collision + FP64 projection",
+           "points": sysm.ele.nupts * sysm.ele.neles}
+    # eager step (also the graph warm-up), then capture + first replay
+    for _ in range(2):
+        st._idxcurr = gs.step(st.tcurr, dt)
+        st.tcurr += dt
+    be.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.rhs_steps):
+        st._idxcurr = gs.step(st.tcurr, dt)
+        st.tcurr += dt
+    be.synchronize()
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(args.rhs_steps):
+        st._idxcurr = st.step(st.tcurr, dt)
+        st.tcurr += dt
+    be.synchronize()
+    te = time.perf_counter() - t0
+    # share of the collision operator in one eager RHS evaluation
+    op = sysm.sm.op
+    op.set_timing(True)
+    st._idxcurr = st.step(st.tcurr, dt)
+    be.synchronize()
+    tim = op.get_timing()
+    op.set_timing(False)
+    coll_ms = sum(v[0] for v in tim.values()) / 2.0
+    out.update({
+        "rhs_evals_per_s": 2 * args.rhs_steps / tg,
+        "rhs_evals_per_s_eager": 2 * args.rhs_steps / te,
+        "ms_per_rhs_graph": 1e3 * tg / (2 * args.rhs_steps),
+        "ms_per_rhs_eager": 1e3 * te / (2 * args.rhs_steps),
+        "collide_ms_per_rhs": coll_ms,
+        "graph_nodes_per_step": gs.nodes,
+        "steps": args.rhs_steps,
+    })
+    f = sysm.eles_scal_upts_full[st.soln_bank].get()
+    out["finite"] = bool(np.isfinite(f).all())
+    gs.close()
+    del sysm, be
+    torch.cuda.empty_cache()
+    return out
 
 
 def make_states(vm, npts, seed, device, dtype=None):
@@ -288,6 +360,11 @@ def main():
         line["conservation_after_projection_max_rel"] = cons
         line["moment_error_vs_fp64_rel"] = mom
         del op64, Q64, op64p, Q64p
+
+    op32.close()
+    torch.cuda.empty_cache()
+    if args.rhs_steps > 0:
+        line["rhs"] = bench_rhs(args, dev)
 
     if not args.no_cpu:
         from oracle import vhs as O

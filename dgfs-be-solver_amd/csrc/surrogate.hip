@@ -8,10 +8,12 @@
 // T from the discrete moments; DGFS convention M = rho (pi T)^-3/2
 // exp(-|c-u|^2/T), so the per-axis variance is s^2 = T/2).  With
 // c' = (c - u)/s and the probabilists' Hermite polynomials He_n, the 8^3
-// retained modes a_n (n = (nx, ny, nz), n_d < 8) are the grid least-squares
-// coefficients of f in the basis M psi_n(c'), psi_n = prod_d He_{n_d}/sqrt(n_d!)
-// (Grad's expansion f ~ M sum_n a_n psi_n; a_000 = 1 for f = M).  The basis decays like M, so
-// round-off far in the tails never reaches the modes.  The collision
+// retained modes a_n (n = (nx, ny, nz), n_d < 8) are the coefficients of f/M0
+// (M0 the Gaussian factor of M) in the grid-orthonormal basis phi_n spanned by
+// psi_n = prod_d He_{n_d}(c'_d)/sqrt(n_d!) exp(-c'_d^2/2) (the span of Grad's
+// expansion f ~ M sum_n b_n psi_n; equilibrium maps to a = 0).  The basis
+// decays like M, so round-off far in the tails never reaches the modes, and
+// the modes' Euclidean norm is the grid L2 norm of the represented field.  The collision
 // operator's modes b_n (same projection of Q) scale as prefac rho T^{gamma/2}
 // in the continuum, so the network maps the dimensionless a to
 // b^ = b / (prefac rho T^{gamma/2}).
@@ -420,52 +422,80 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return r;
 }
 
-// Ginv[d] = (V_d^T V_d)^{-1}, 8 x 8 per axis (Gauss-Jordan with partial
-// pivoting, one thread per axis; work holds 3 x 8 x 16 doubles)
+// Linv[d] = L_d^{-1}, L_d the Cholesky factor of the per-axis Gram matrix
+// G_d = (h/s) V_d^T V_d (one thread per axis; work holds 3 x 8 x 8 doubles).
+// The 1D functions V_d L_d^{-T} are orthonormal on the grid with the measure
+// dx' = h/s, so their tensor products are orthonormal in 3D.
 template <int N>
-__device__ void herm_gram_inverse(const double* V, double* Ginv, double* work) {
+__device__ void herm_chol_inv(const double* V, double hs, double* Linv, double* work) {
   if (threadIdx.x < 3) {
     const int d = threadIdx.x;
     const double* Vd = V + d * N * kSurD;
-    double* w = work + d * kSurD * 2 * kSurD;
+    double* Lw = work + d * kSurD * kSurD;
+    double* Li = Linv + d * kSurD * kSurD;
     for (int a = 0; a < kSurD; ++a)
-      for (int b = 0; b < kSurD; ++b) {
+      for (int b = 0; b <= a; ++b) {
         double s = 0.0;
         for (int i = 0; i < N; ++i) s += Vd[i * kSurD + a] * Vd[i * kSurD + b];
-        w[a * 2 * kSurD + b] = s;
-        w[a * 2 * kSurD + kSurD + b] = (a == b) ? 1.0 : 0.0;
+        s *= hs;
+        for (int k = 0; k < b; ++k) s -= Lw[a * kSurD + k] * Lw[b * kSurD + k];
+        Lw[a * kSurD + b] = (a == b) ? sqrt(fmax(s, 1e-300)) : s / Lw[b * kSurD + b];
       }
-    for (int c = 0; c < kSurD; ++c) {
-      int piv = c;
-      for (int r = c + 1; r < kSurD; ++r)
-        if (fabs(w[r * 2 * kSurD + c]) > fabs(w[piv * 2 * kSurD + c])) piv = r;
-      if (piv != c)
-        for (int j = 0; j < 2 * kSurD; ++j) {
-          const double t = w[c * 2 * kSurD + j];
-          w[c * 2 * kSurD + j] = w[piv * 2 * kSurD + j];
-          w[piv * 2 * kSurD + j] = t;
+    // forward substitution for the inverse of the lower triangle
+    for (int c = 0; c < kSurD; ++c)
+      for (int a = 0; a < kSurD; ++a) {
+        if (a < c) {
+          Li[a * kSurD + c] = 0.0;
+          continue;
         }
-      const double dg = w[c * 2 * kSurD + c];
-      for (int j = 0; j < 2 * kSurD; ++j) w[c * 2 * kSurD + j] /= dg;
-      for (int r = 0; r < kSurD; ++r)
-        if (r != c) {
-          const double fct = w[r * 2 * kSurD + c];
-          for (int j = 0; j < 2 * kSurD; ++j) w[r * 2 * kSurD + j] -= fct * w[c * 2 * kSurD + j];
-        }
-    }
-    for (int a = 0; a < kSurD; ++a)
-      for (int b = 0; b < kSurD; ++b) Ginv[d * kSurD * kSurD + a * kSurD + b] = w[a * 2 * kSurD + kSurD + b];
+        double s = (a == c) ? 1.0 : 0.0;
+        for (int k = c; k < a; ++k) s -= Lw[a * kSurD + k] * Li[k * kSurD + c];
+        Li[a * kSurD + c] = s / Lw[a * kSurD + a];
+      }
   }
   __syncthreads();
 }
 
-// Least-squares (grid L2) coefficients of g in the basis M He_n, n_d < 8:
-// b = (Gx^-1 (x) Gy^-1 (x) Gz^-1) sum_r g(r) V_x V_y V_z / (C scale), with
-// M = C prod_d exp(-(c_d - u_d)^2 / T).  The basis functions decay like the
-// Maxwellian, so noise of g far in the tails does not reach the modes.
+// Apply A_x (x) A_y (x) A_z to the 8^3 block `in` (A_d = Linv_d, or its
+// transpose when `transpose`), through `tmp`; result in `out` (may alias in).
+__device__ void kron3(const double* Linv, bool transpose, double* in, double* tmp, double* out) {
+  auto A = [&](int d, int r, int c) {
+    return transpose ? Linv[d * kSurD * kSurD + c * kSurD + r] : Linv[d * kSurD * kSurD + r * kSurD + c];
+  };
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int a = e / (kSurD * kSurD), bc = e % (kSurD * kSurD);
+    double s = 0.0;
+    for (int q = 0; q < kSurD; ++q) s += A(0, a, q) * in[q * kSurD * kSurD + bc];
+    tmp[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int a = e / (kSurD * kSurD), b = (e / kSurD) % kSurD, c = e % kSurD;
+    double s = 0.0;
+    for (int q = 0; q < kSurD; ++q) s += A(1, b, q) * tmp[(a * kSurD + q) * kSurD + c];
+    in[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+    const int ab = e / kSurD, c = e % kSurD;
+    double s = 0.0;
+    for (int q = 0; q < kSurD; ++q) s += A(2, c, q) * in[ab * kSurD + q];
+    tmp[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) out[e] = tmp[e];
+  __syncthreads();
+}
+
+// Coefficients of g / C in the grid-orthonormal basis phi_n (tensor products
+// of V_d L_d^{-T}), divided by `scale`:
+//   c = (Lx^-1 (x) Ly^-1 (x) Lz^-1) (h/s)^3 sum_r g(r) V_x V_y V_z / (C scale).
+// ||c|| is the grid L2 norm (measure dx'^3) of the part of g / (C scale) in
+// the span, so the modes stay bounded and a mean-square error of the modes is
+// the L2 error of the decoded field.
 template <int N>
-__device__ void herm_project(const double* __restrict__ g, const double* V, const double* Ginv, double* t1,
-                             double* t2, double* b3, double inv_scale, float* out, float out_shift0) {
+__device__ void herm_project(const double* __restrict__ g, const double* V, const double* Linv, double* t1,
+                             double* t2, double* b3, double factor, float* out, float shift0) {
   constexpr int NN = N * N;
   const double* Vx = V;
   const double* Vy = V + N * kSurD;
@@ -488,52 +518,34 @@ __device__ void herm_project(const double* __restrict__ g, const double* V, cons
     const int ab = e / kSurD, c = e % kSurD;
     double s = 0.0;
     for (int k = 0; k < N; ++k) s += t2[ab * N + k] * Vz[k * kSurD + c];
-    b3[e] = s;
+    b3[e] = s * factor;
   }
   __syncthreads();
-  // apply the inverse Gram matrices axis by axis (b3 -> t1 -> t2 -> out)
-  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
-    const int a = e / (kSurD * kSurD), bc = e % (kSurD * kSurD);
-    double s = 0.0;
-    for (int q = 0; q < kSurD; ++q) s += Ginv[a * kSurD + q] * b3[q * kSurD * kSurD + bc];
-    t1[e] = s;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
-    const int a = e / (kSurD * kSurD), b = (e / kSurD) % kSurD, c = e % kSurD;
-    double s = 0.0;
-    for (int q = 0; q < kSurD; ++q) s += Ginv[kSurD * kSurD + b * kSurD + q] * t1[(a * kSurD + q) * kSurD + c];
-    t2[e] = s;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
-    const int ab = e / kSurD, c = e % kSurD;
-    double s = 0.0;
-    for (int q = 0; q < kSurD; ++q) s += Ginv[2 * kSurD * kSurD + c * kSurD + q] * t2[ab * kSurD + q];
-    out[e] = static_cast<float>(s * inv_scale - (e == 0 ? out_shift0 : 0.0));
-  }
+  kron3(Linv, false, b3, t1, b3);
+  for (int e = threadIdx.x; e < kSurModes; e += blockDim.x)
+    out[e] = static_cast<float>(b3[e] - (e == 0 ? static_cast<double>(shift0) : 0.0));
   __syncthreads();
 }
 
 // LDS of the encoder / target kernels: t1 (8 N^2), t2 (64 N), V (24 N),
-// Ginv (192), work (384), b3 (512), red (256) doubles
+// Linv (192), work (192), b3 (512), red (256) doubles
 template <int N>
 constexpr size_t enc_lds_doubles() {
   return static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + 3 * kSurD * kSurD +
-         3 * kSurD * 2 * kSurD + kSurModes + 256;
+         3 * kSurD * kSurD + kSurModes + 256;
 }
 
 template <int N>
 __global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
   constexpr int n = N * N * N, NN = N * N;
   extern __shared__ double sm[];
-  double* t1 = sm;                        // kSurD * N * N
-  double* t2 = t1 + kSurD * NN;           // kSurD * kSurD * N
-  double* V = t2 + kSurD * kSurD * N;     // 3 * N * kSurD
-  double* Ginv = V + 3 * N * kSurD;       // 3 * 64
-  double* work = Ginv + 3 * kSurD * kSurD;  // 3 * 128
-  double* b3 = work + 3 * kSurD * 2 * kSurD;  // 512
-  double* red = b3 + kSurModes;           // 256
+  double* t1 = sm;                          // kSurD * N * N
+  double* t2 = t1 + kSurD * NN;             // kSurD * kSurD * N
+  double* V = t2 + kSurD * kSurD * N;       // 3 * N * kSurD
+  double* Linv = V + 3 * N * kSurD;         // 3 * 64
+  double* work = Linv + 3 * kSurD * kSurD;  // 3 * 64
+  double* b3 = work + 3 * kSurD * kSurD;    // 512
+  double* red = b3 + kSurModes;             // 256
   const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
   const double pi = 3.14159265358979323846;
   for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
@@ -567,20 +579,23 @@ __global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
       if (a.lt) a.lt[pt] = static_cast<float>(log(T));
     }
     if (a.x) {
-      const double s = sqrt(0.5 * T);
+      const double s = sqrt(0.5 * T), hs = hh / s;
       herm_table(V, c0, hh, ux, s, N);
       herm_table(V + N * kSurD, c0, hh, uy, s, N);
       herm_table(V + 2 * N * kSurD, c0, hh, uz, s, N);
       __syncthreads();
-      herm_gram_inverse<N>(V, Ginv, work);
-      // f ~ M sum a_n He_n; a_000 = 1 for f = M, stored as a_000 - 1
-      herm_project<N>(f, V, Ginv, t1, t2, b3, 1.0 / Mn, a.x + static_cast<size_t>(pt) * kSurModes, 1.0f);
+      herm_chol_inv<N>(V, hs, Linv, work);
+      // f = M is the 000 basis function: its coefficient is prod_d L_d[0][0],
+      // subtracted so that equilibrium maps to zero
+      const double c000 = 1.0 / (Linv[0] * Linv[kSurD * kSurD] * Linv[2 * kSurD * kSurD]);
+      herm_project<N>(f, V, Linv, t1, t2, b3, hs * hs * hs / Mn, a.x + static_cast<size_t>(pt) * kSurModes,
+                      static_cast<float>(c000));
     }
     __syncthreads();
   }
 }
 
-// targets: b^ = (L2 Hermite coefficients of Q in the frame of mom) / (prefac rho T^{gamma/2})
+// targets: b^ = (orthonormal-basis coefficients of Q in the frame of mom) / (prefac rho T^{gamma/2})
 struct TgtArgs {
   const double* Q;
   const double* mom;
@@ -596,29 +611,31 @@ __global__ __launch_bounds__(256) void k_sur_targets(TgtArgs a) {
   double* t1 = sm;
   double* t2 = t1 + kSurD * NN;
   double* V = t2 + kSurD * kSurD * N;
-  double* Ginv = V + 3 * N * kSurD;
-  double* work = Ginv + 3 * kSurD * kSurD;
-  double* b3 = work + 3 * kSurD * 2 * kSurD;
+  double* Linv = V + 3 * N * kSurD;
+  double* work = Linv + 3 * kSurD * kSurD;
+  double* b3 = work + 3 * kSurD * kSurD;
   const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
   const double pi = 3.14159265358979323846;
   for (int pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
     const double* mo = a.mom + static_cast<size_t>(pt) * 8;
-    const double rho = mo[0], T = mo[4], s = sqrt(0.5 * T);
+    const double rho = mo[0], T = mo[4], s = sqrt(0.5 * T), hs = hh / s;
     herm_table(V, c0, hh, mo[1], s, N);
     herm_table(V + N * kSurD, c0, hh, mo[2], s, N);
     herm_table(V + 2 * N * kSurD, c0, hh, mo[3], s, N);
     __syncthreads();
-    herm_gram_inverse<N>(V, Ginv, work);
+    herm_chol_inv<N>(V, hs, Linv, work);
     const double Mn = rho / pow(pi * T, 1.5);
-    const double inv = 1.0 / (Mn * a.prefac * rho * pow(T, 0.5 * a.gamma));
-    herm_project<N>(a.Q + static_cast<size_t>(pt) * n, V, Ginv, t1, t2, b3, inv,
+    const double factor = hs * hs * hs / (Mn * a.prefac * rho * pow(T, 0.5 * a.gamma));
+    herm_project<N>(a.Q + static_cast<size_t>(pt) * n, V, Linv, t1, t2, b3, factor,
                     a.y + static_cast<size_t>(pt) * kSurModes, 0.0f);
   }
 }
 
-// decode: Q[idx[i]] = M sum_n b_n psi_n(c'), b = prefac rho T^{gamma/2} y with the
-// mass, momentum and energy modes removed (b_000 = b_100 = b_010 = b_001 = 0,
-// b_200 + b_020 + b_002 = 0)
+// decode: the network output y holds orthonormal-basis coefficients; the
+// Hermite-basis coefficients are b = prefac rho T^{gamma/2} (Lx^-T (x) Ly^-T
+// (x) Lz^-T) y, the mass, momentum and energy modes are removed
+// (b_000 = b_100 = b_010 = b_001 = 0, b_200 + b_020 + b_002 = 0) and
+// Q[idx[i]] = M sum_n b_n psi_n(c')
 struct DecArgs {
   const float* y;    // [nidx][512] network outputs, row i for point idx[i]
   const double* mom; // [npts][8]
@@ -633,9 +650,11 @@ __global__ __launch_bounds__(256) void k_sur_decode(DecArgs a) {
   constexpr int n = N * N * N, NN = N * N;
   extern __shared__ double sm[];
   double* s2 = sm;                      // kSurD * N * N   [a][j][k]
-  double* s1 = s2 + kSurD * NN;         // kSurD * kSurD * N [a][b][k]
+  double* s1 = s2 + kSurD * NN;         // kSurD * kSurD * N [a][b][k] (>= 512)
   double* V = s1 + kSurD * kSurD * N;   // 3 * N * kSurD
   double* bb = V + 3 * N * kSurD;       // 512
+  double* Linv = bb + kSurModes;        // 3 * 64
+  double* work = Linv + 3 * kSurD * kSurD;  // 3 * 64
   const double hh = 2.0 * a.L / N, c0 = -a.L + 0.5 * hh;
   const double pi = 3.14159265358979323846;
   for (int i = blockIdx.x; i < a.nidx; i += gridDim.x) {
@@ -649,6 +668,8 @@ __global__ __launch_bounds__(256) void k_sur_decode(DecArgs a) {
     const float* yi = a.y + static_cast<size_t>(i) * kSurModes;
     for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) bb[e] = scale * static_cast<double>(yi[e]);
     __syncthreads();
+    herm_chol_inv<N>(V, hh / s, Linv, work);
+    kron3(Linv, true, bb, s1, bb);
     if (threadIdx.x == 0) {
       constexpr int e200 = 2 * kSurD * kSurD, e020 = 2 * kSurD, e002 = 2;
       const double tr = (bb[e200] + bb[e020] + bb[e002]) / 3.0;
@@ -859,7 +880,8 @@ static size_t enc_lds() {
 }
 template <int N>
 static size_t dec_lds() {
-  return (static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + kSurModes) * sizeof(double);
+  return (static_cast<size_t>(kSurD) * N * N + kSurD * kSurD * N + 3 * N * kSurD + kSurModes +
+          6 * kSurD * kSurD) * sizeof(double);
 }
 
 template <int N>

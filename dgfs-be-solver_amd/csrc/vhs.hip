@@ -67,17 +67,16 @@ struct GainArgs {
   int nitems, ngroups, ppg, Nrho, M;
 };
 
+// One (point, radial group) work item of the multi-pass gain: LDS and phase
+// tables come from the calling kernel, Z / S / W are this workgroup's scratch.
 template <typename T, int N>
-__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_gain(GainArgs<T> a) {
+__device__ __forceinline__ void vhs_gain_item(const GainArgs<T>& a, int item, cpx<T>* lds, cpx<T> (*eph)[N]) {
   constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
-  __shared__ cpx<T> lds[PB * N * (N + 1)];
-  __shared__ cpx<T> eph[3][N];
   const int tid = threadIdx.x;
   cpx<T>* Z = a.Z + static_cast<size_t>(blockIdx.x) * n;
   T* S = a.S + static_cast<size_t>(blockIdx.x) * n;
   cpx<T>* W = a.W + static_cast<size_t>(blockIdx.x) * 3 * NN;
-
-  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+  {
     const int pt = item / a.ngroups, g = item - pt * a.ngroups;
     const int p0 = g * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
     const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
@@ -169,6 +168,33 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_gain(GainArgs<T> a) {
             });
       }
     }
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_gain(GainArgs<T> a) {
+  __shared__ cpx<T> lds[Shape<N>::PB * N * (N + 1)];
+  __shared__ cpx<T> eph[3][N];
+  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) vhs_gain_item<T, N>(a, item, lds, eph);
+}
+
+// Two velocity grids in one persistent launch: the work items of grid NA
+// come first, then those of grid NB; each workgroup runs either kind with the
+// scratch of the grid's own handle.  Both grids use Shape<N>::NT threads.
+template <typename T, int NA, int NB>
+__global__ __launch_bounds__(Shape<NA>::NT) void k_vhs_gain_pair(GainArgs<T> a, GainArgs<T> b) {
+  static_assert(Shape<NA>::NT == Shape<NB>::NT, "paired grids need the same workgroup size");
+  constexpr int LA = Shape<NA>::PB * NA * (NA + 1), LB = Shape<NB>::PB * NB * (NB + 1);
+  __shared__ cpx<T> lds[LA > LB ? LA : LB];
+  __shared__ cpx<T> eph[3][NA > NB ? NA : NB];
+  auto* ea = reinterpret_cast<cpx<T>(*)[NA]>(&eph[0][0]);
+  auto* eb = reinterpret_cast<cpx<T>(*)[NB]>(&eph[0][0]);
+  const int total = a.nitems + b.nitems;
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    if (it < a.nitems)
+      vhs_gain_item<T, NA>(a, it, lds, ea);
+    else
+      vhs_gain_item<T, NB>(b, it - a.nitems, lds, eb);
   }
 }
 
@@ -869,68 +895,65 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
   }
 }
 
-template <typename T, int N>
-static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipStream_t st) {
-  constexpr int n = N * N * N;
-  const int slots = use_rr(h) ? cu_count() : h->cap_slots;
+// radial groups per point: enough (point, group) items for two per slot
+static int pick_groups(const VhsHandle* h, int npts, int slots) {
   int groups = h->groups_override;
-  if (groups <= 0) {
-    groups = std::max(1, std::min(h->Nrho, (2 * slots + npts - 1) / std::max(npts, 1)));
-  }
+  if (groups <= 0) groups = std::max(1, std::min(h->Nrho, (2 * slots + npts - 1) / std::max(npts, 1)));
   const int ppg = (h->Nrho + groups - 1) / groups;
-  groups = (h->Nrho + ppg - 1) / ppg;
+  return (h->Nrho + ppg - 1) / ppg;
+}
+
+// spectra of the chunk (and the transposed copy for the register-resident
+// gain), gain partial buffer; returns the gain kernel's arguments
+template <typename T, int N>
+static int stage_prep(VhsHandle* h, const double* fc, int npts, int groups, int slots, bool rr, GainArgs<T>* ga,
+                      hipStream_t st) {
+  constexpr int n = N * N * N;
+  const int ppg = (h->Nrho + groups - 1) / groups;
   const size_t items = static_cast<size_t>(npts) * groups;
   if (items > h->cap_items) {
     (void)hipFree(h->Ghat);
     h->Ghat = nullptr;
+    h->cap_items = 0;
     DGFS_HIP_TRY(hipMalloc(&h->Ghat, items * n * sizeof(cpx<T>)));
     h->cap_items = items;
   }
   h->last_groups = groups;
   h->last_slots = slots;
   h->threads = Shape<N>::NT;
-
-  if (use_rr(h)) {
+  if (rr) {
     const size_t fbytes = static_cast<size_t>(npts) * n * sizeof(cpx<T>);
     if (fbytes > h->cap_ft) {
       (void)hipFree(h->Ft);
       h->Ft = nullptr;
+      h->cap_ft = 0;
       DGFS_HIP_TRY(hipMalloc(&h->Ft, fbytes));
       h->cap_ft = fbytes;
     }
   }
-  PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), use_rr(h) ? static_cast<cpx<T>*>(h->Ft) : nullptr,
+  PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), rr ? static_cast<cpx<T>*>(h->Ft) : nullptr,
                  static_cast<cpx<T>*>(h->Z), npts};
   KTimerT<VhsHandle> t0(h, 0, st);
-  hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, pa);
+  hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, pa);
   DGFS_LAUNCH_CHECK("vhs_prep");
   t0.stop();
+  *ga = GainArgs<T>{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Ghat),
+                    static_cast<cpx<T>*>(h->Z), static_cast<T*>(h->S), static_cast<cpx<T>*>(h->W),
+                    static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
+                    static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+  return DGFS_OK;
+}
 
-  GainArgs<T> ga{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Ghat),
-                 static_cast<cpx<T>*>(h->Z), static_cast<T*>(h->S), static_cast<cpx<T>*>(h->W),
-                 static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
-                 static_cast<int>(items), groups, ppg, h->Nrho, h->M};
-  if (use_rr(h)) {
-    h->last_algorithm = 2;
-    const int rc = run_gain_rr<T, N>(h, npts, groups, ppg, items, st);
-    if (rc) return rc;
-  } else {
-    h->last_algorithm = 1;
-    KTimerT<VhsHandle> t1(h, 1, st);
-    hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
-                       dim3(Shape<N>::NT), 0, st, ga);
-    DGFS_LAUNCH_CHECK("vhs_gain");
-    t1.stop();
-  }
-
+// gain + loss assembly and the optional FP64 projection
+template <typename T, int N>
+static int stage_finish(VhsHandle* h, const double* fc, double* Qc, int npts, int groups, hipStream_t st) {
   FinArgs<T> fa{static_cast<const cpx<T>*>(h->Ghat), static_cast<const cpx<T>*>(h->F),
                 static_cast<const T*>(h->b2), fc, Qc, static_cast<cpx<T>*>(h->Z),
                 h->prefac, npts, groups};
   KTimerT<VhsHandle> t2(h, 2, st);
-  hipLaunchKernelGGL((k_vhs_finalize<T, N>), dim3(std::min(npts, slots)), dim3(Shape<N>::NT), 0, st, fa);
+  hipLaunchKernelGGL((k_vhs_finalize<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, fa);
   DGFS_LAUNCH_CHECK("vhs_finalize");
   t2.stop();
-
   if (h->project) {
     ProjArgs pr;
     pr.Q = Qc;
@@ -942,6 +965,28 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
     t3.stop();
   }
   return DGFS_OK;
+}
+
+template <typename T, int N>
+static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipStream_t st) {
+  const bool rr = use_rr(h);
+  const int slots = rr ? cu_count() : h->cap_slots;
+  const int groups = pick_groups(h, npts, slots);
+  GainArgs<T> ga;
+  int rc = stage_prep<T, N>(h, fc, npts, groups, slots, rr, &ga, st);
+  if (rc) return rc;
+  if (rr) {
+    h->last_algorithm = 2;
+    rc = run_gain_rr<T, N>(h, npts, groups, ga.ppg, static_cast<size_t>(ga.nitems), st);
+    if (rc) return rc;
+  } else {
+    h->last_algorithm = 1;
+    KTimerT<VhsHandle> t1(h, 1, st);
+    hipLaunchKernelGGL((k_vhs_gain<T, N>), dim3(std::min(ga.nitems, slots)), dim3(Shape<N>::NT), 0, st, ga);
+    DGFS_LAUNCH_CHECK("vhs_gain");
+    t1.stop();
+  }
+  return stage_finish<T, N>(h, fc, Qc, npts, groups, st);
 }
 
 template <typename T, int N>
@@ -1022,6 +1067,46 @@ static int dispatch(VhsHandle* h, const double* f, double* Q, bool aosoa, int nu
 #undef DGFS_CASE
     default:
       return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d is not built", h->N);
+  }
+}
+
+// two handles (velocity grids NA > NB, same precision) in one persistent gain
+// launch; both take the multi-pass gain
+template <typename T, int NA, int NB>
+static int run_pair(VhsHandle* ha, const double* fa, double* Qa, int na, VhsHandle* hb, const double* fb, double* Qb,
+                    int nb, hipStream_t st) {
+  int rc = ensure_ws<T, NA>(ha, na);
+  if (!rc) rc = ensure_ws<T, NB>(hb, nb);
+  if (rc) return rc;
+  const int slots = std::min(ha->cap_slots, hb->cap_slots);
+  const int gA = pick_groups(ha, na + nb, slots), gB = pick_groups(hb, na + nb, slots);
+  GainArgs<T> ga, gb;
+  rc = stage_prep<T, NA>(ha, fa, na, gA, slots, false, &ga, st);
+  if (!rc) rc = stage_prep<T, NB>(hb, fb, nb, gB, slots, false, &gb, st);
+  if (rc) return rc;
+  ha->last_algorithm = hb->last_algorithm = 1;
+  {
+    KTimerT<VhsHandle> t1(ha, 1, st);
+    hipLaunchKernelGGL((k_vhs_gain_pair<T, NA, NB>), dim3(std::min(ga.nitems + gb.nitems, slots)),
+                       dim3(Shape<NA>::NT), 0, st, ga, gb);
+    DGFS_LAUNCH_CHECK("vhs_gain_pair");
+    t1.stop();
+  }
+  rc = stage_finish<T, NA>(ha, fa, Qa, na, gA, st);
+  if (!rc) rc = stage_finish<T, NB>(hb, fb, Qb, nb, gB, st);
+  return rc;
+}
+
+template <typename T>
+static int dispatch_pair(VhsHandle* ha, const double* fa, double* Qa, int na, VhsHandle* hb, const double* fb,
+                         double* Qb, int nb, hipStream_t st) {
+  const int key = ha->N * 100 + hb->N;
+  switch (key) {
+    case 3224: return run_pair<T, 32, 24>(ha, fa, Qa, na, hb, fb, Qb, nb, st);
+    case 3216: return run_pair<T, 32, 16>(ha, fa, Qa, na, hb, fb, Qb, nb, st);
+    case 2416: return run_pair<T, 24, 16>(ha, fa, Qa, na, hb, fb, Qb, nb, st);
+    default: return fail(DGFS_ERR_UNSUPPORTED, "paired grids %d and %d are not built (32+24, 32+16, 24+16)",
+                         ha->N, hb->N);
   }
 }
 
@@ -1219,4 +1304,31 @@ extern "C" int dgfs_vhs_get_algorithm(void* p, int* algorithm) {
   if (!algorithm) return fail(DGFS_ERR_INVALID_VALUE, "null output");
   *algorithm = h->last_algorithm;
   return DGFS_OK;
+}
+
+// Two independent batches on two velocity grids (for instance 32^3 and 24^3
+// points of a mixed run) in one persistent gain launch.  Batches larger than
+// one workspace chunk are evaluated as separate launches.
+extern "C" int dgfs_vhs_eval_pair(void* pa, const double* fa, double* Qa, int na, void* pb, const double* fb,
+                                  double* Qb, int nb, void* stream) {
+  VhsHandle* ha = as_handle(pa);
+  VhsHandle* hb = as_handle(pb);
+  if (!ha || !hb) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (ha == hb) return fail(DGFS_ERR_INVALID_VALUE, "the two batches need two handles");
+  if (!fa || !Qa || !fb || !Qb || na < 0 || nb < 0) return fail(DGFS_ERR_INVALID_VALUE, "bad batch arguments");
+  if (fa == Qa || fb == Qb) return fail(DGFS_ERR_INVALID_VALUE, "f and Q may not alias");
+  if (ha->prec != hb->prec) return fail(DGFS_ERR_INVALID_VALUE, "paired handles need the same precision");
+  auto st = static_cast<hipStream_t>(stream);
+  if (na == 0 || nb == 0 || na > chunk_points(ha, na) || nb > chunk_points(hb, nb)) {
+    int rc = dgfs_vhs_eval_contig(pa, fa, Qa, na, stream);
+    return rc ? rc : dgfs_vhs_eval_contig(pb, fb, Qb, nb, stream);
+  }
+  if (ha->N < hb->N) {
+    std::swap(ha, hb);
+    std::swap(fa, fb);
+    std::swap(Qa, Qb);
+    std::swap(na, nb);
+  }
+  return (ha->prec == 64) ? dispatch_pair<double>(ha, fa, Qa, na, hb, fb, Qb, nb, st)
+                          : dispatch_pair<float>(ha, fa, Qa, na, hb, fb, Qb, nb, st);
 }

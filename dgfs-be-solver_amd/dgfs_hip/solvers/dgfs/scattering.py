@@ -73,6 +73,11 @@ class VHSOperator:
         _lib.call("dgfs_vhs_eval_contig", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)),
                   int(npts), C.c_void_p(stream or 0))
 
+    def eval_pair(self, f, Q, npts, other, f_other, Q_other, npts_other, stream=None):
+        """This handle's batch and another grid's batch in one persistent gain launch."""
+        _lib.call("dgfs_vhs_eval_pair", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)), int(npts), other._h,
+                  C.c_void_p(int(f_other)), C.c_void_p(int(Q_other)), int(npts_other), C.c_void_p(stream or 0))
+
     def eval_aosoa(self, f, Q, nupts, ldim, vsize, neles, e0, ne, stream=None):
         _lib.call("dgfs_vhs_eval", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)),
                   int(nupts), int(ldim), int(vsize), int(neles), int(e0), int(ne),

@@ -147,17 +147,11 @@ class SpectralSurrogate:
 
     @staticmethod
     def mode_weights(device):
-        """Loss weight of mode n = (a, b, c): prod_d int exp(-x^2) psi_{n_d}(x)^2 dx
-        with psi_a = He_a / sqrt(a!), the diagonal of the continuum Gram matrix
-        of the basis, so the weighted error approximates the L2 error of the
-        decoded field (normalised to mean 1)."""
+        """Loss weights of the modes: uniform, because the modes are
+        coefficients in a grid-orthonormal basis, so their mean-square error
+        is the L2 error of the decoded field (per unit prefac rho T^{gamma/2})."""
         import torch
-        from numpy.polynomial.hermite_e import hermevander
-        x = np.linspace(-12.0, 12.0, 4001)
-        V = hermevander(x, 7) / np.sqrt([math.factorial(a) for a in range(8)])
-        g1 = (np.exp(-x * x)[:, None] * V * V).sum(0) * (x[1] - x[0])
-        w = np.einsum("a,b,c->abc", g1, g1, g1).ravel()
-        return torch.as_tensor(w / w.mean(), dtype=torch.float32, device=device)
+        return torch.ones(MODES, dtype=torch.float32, device=device)
 
     def train(self, x, lt, y, epochs=10, batch=256, lr=1e-3, bf16=True, seed=0, log=None, lr_min=1e-5):
         """Adam on the weighted mean-square error of the modes; returns the

@@ -8,9 +8,10 @@ restated here is the surrogate's own definition:
 
 * Hermite encoder: moments rho, u, T of f (DGFS Maxwellian convention of
   frfs/solvers/dgfs/initcond.py:55-58, per-axis variance T/2), indicator
-  ||f - M[f]|| / ||f||, and the grid least-squares coefficients a_n of f in
-  the basis M psi_n(c'), psi_n = prod He_{n_d}/sqrt(n_d!), n_d < 8,
-  c' = (c - u)/sqrt(T/2) (a_000 stored minus 1);
+  ||f - M[f]|| / ||f||, and the coefficients a_n of f / C (C the Maxwellian's
+  prefactor) in the grid-orthonormal basis spanned by M psi_n(c'),
+  psi_n = prod He_{n_d}/sqrt(n_d!), n_d < 8, c' = (c - u)/sqrt(T/2) (the
+  Maxwellian's own coefficient subtracted);
 * targets b^ = (the same coefficients of Q) / (prefac rho T^{gamma/2});
 * decoder Q = M[f] sum_n b_n psi_n(c') with the mass, momentum and energy
   modes removed;
@@ -183,20 +184,28 @@ def _vander(N, L, u, T):
     return out
 
 
+def _linv(V, hs):
+    """Inverse Cholesky factors of the per-axis Gram matrices (h/s) V^T V."""
+    return [np.linalg.inv(np.linalg.cholesky(hs * v.T @ v)) for v in V]
+
+
 def hermite_project(g, N, L, u, T, C, scale):
-    """Grid least-squares coefficients b of g ~ C prod_d exp(-x_d^2/2) sum_n
-    b_n He_n, divided by scale; shape (8, 8, 8)."""
+    """Coefficients of g / (C scale) in the grid-orthonormal basis
+    phi = psi L^-T (per axis; measure dx' = h/s), shape (8, 8, 8)."""
     V = _vander(N, L, u, T)
-    Gi = [np.linalg.inv(v.T @ v) for v in V]
-    B = np.einsum("ijk,ia,jb,kc->abc", g.reshape(N, N, N), V[0], V[1], V[2])
-    return np.einsum("pa,qb,rc,abc->pqr", Gi[0], Gi[1], Gi[2], B) / (C * scale)
+    hs = (2.0 * L / N) / math.sqrt(0.5 * T)
+    Li = _linv(V, hs)
+    B = np.einsum("ijk,ia,jb,kc->abc", g.reshape(N, N, N), V[0], V[1], V[2]) * hs ** 3
+    return np.einsum("pa,qb,rc,abc->pqr", Li[0], Li[1], Li[2], B) / (C * scale)
 
 
 def encode(f, N, L):
     rho, u, T, ind, _ = moments(f, N, L)
     C = rho / (math.pi * T) ** 1.5
     a = hermite_project(f, N, L, u, T, C, 1.0).ravel()
-    a[0] -= 1.0
+    V = _vander(N, L, u, T)
+    Li = _linv(V, (2.0 * L / N) / math.sqrt(0.5 * T))
+    a[0] -= 1.0 / (Li[0][0, 0] * Li[1][0, 0] * Li[2][0, 0])   # the Maxwellian's own coefficient
     return np.array([rho, *u, T, ind]), a, math.log(T)
 
 
@@ -208,14 +217,16 @@ def targets(Q, mom, N, L, gamma, prefac):
 
 def decode(y, mom, N, L, gamma, prefac, remove_invariants=True):
     rho, u, T = mom[0], mom[1:4], mom[4]
-    b = (prefac * rho * T ** (0.5 * gamma)) * np.asarray(y, dtype=np.float64).reshape(D, D, D).copy()
+    V = _vander(N, L, u, T)
+    Li = _linv(V, (2.0 * L / N) / math.sqrt(0.5 * T))
+    c = (prefac * rho * T ** (0.5 * gamma)) * np.asarray(y, dtype=np.float64).reshape(D, D, D)
+    b = np.einsum("ap,bq,cr,abc->pqr", Li[0], Li[1], Li[2], c)   # L^-T per axis
     if remove_invariants:
         tr = (b[2, 0, 0] + b[0, 2, 0] + b[0, 0, 2]) / 3.0
         b[2, 0, 0] -= tr
         b[0, 2, 0] -= tr
         b[0, 0, 2] -= tr
         b[0, 0, 0] = b[1, 0, 0] = b[0, 1, 0] = b[0, 0, 1] = 0.0
-    V = _vander(N, L, u, T)
     C = rho / (math.pi * T) ** 1.5
     return (C * np.einsum("abc,ia,jb,kc->ijk", b, V[0], V[1], V[2])).ravel()
 

@@ -64,16 +64,15 @@ def test_encoder_of_maxwellian(N, L):
 
 @pytest.mark.parametrize("N,L,T", [(16, 6.0, 1.0), (32, 7.0, 1.2), (24, 7.0, 2.0)])
 def test_projection_reproduces_the_basis_span(N, L, T):
-    """targets(decode(y)) = y for any coefficients y: the grid least-squares
-    projection is exact on the span of M psi_n, on coarse and fine grids (to
-    the conditioning of the per-axis Gram matrices, about 1e3-1e4)."""
+    """targets(decode(y)) = y for any coefficients y: the projection onto the
+    grid-orthonormal basis is exact on its span, on coarse and fine grids."""
     rng = np.random.default_rng(2)
     mom = np.array([0.9, 0.2, 0.1, -0.3, T, 0.0])
     y = rng.normal(size=S.MODES) * 0.1
     g = S.decode(y, mom, N, L, gamma=0.38, prefac=1.7, remove_invariants=False)
     back = S.targets(g, mom, N, L, 0.38, 1.7)
-    # measured 1.6e-7 at h = 0.75 (per-axis Gram condition 9e3), 2e-9 at h <= 0.58
-    assert np.max(np.abs(back - y)) < (1e-6 if N == 16 else 1e-8)
+    # measured 3e-12 (h = 0.75) and 4e-13 (h = 0.44)
+    assert np.max(np.abs(back - y)) < 1e-10
 
 
 def test_decoder_removes_invariants():

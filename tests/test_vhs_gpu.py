@@ -208,3 +208,28 @@ def test_register_resident_matches(N, L, Nrho, M, device):
         say("  ", key, "vs oracle", e, "vs multi-pass", rel(Q, outs[(1, 0)]))
         assert e < max(3 * e_mp, 1e-4)
         assert rel(Q, outs[(1, 0)]) < 1e-4
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_pair_launch_matches_separate(precision, device):
+    """Batches on 32^3 and 24^3 grids in one persistent gain launch give the
+    separate (multi-pass) evaluations' results, to summation order."""
+    opa, vma = make_op(32, 6.0, 4, 6, 0.38, 1.0, precision=precision)
+    opb, vmb = make_op(24, 7.0, 4, 6, 0.38, 1.0, precision=precision)
+    if precision == 32:
+        opa.set_algorithm(1)
+    fa = smooth_states(O.VelocityMesh(32, 6.0, 4, 6), 6, seed=1)
+    fb = smooth_states(O.VelocityMesh(24, 7.0, 4, 6), 5, seed=2)
+    Qa = run_contig(opa, fa, device)
+    Qb = run_contig(opb, fb, device)
+    import torch
+    fa_t, fb_t = (torch.as_tensor(x, device=device) for x in (fa, fb))
+    Qa_t, Qb_t = torch.full_like(fa_t, float("nan")), torch.full_like(fb_t, float("nan"))
+    # the smaller grid first: the library orders the pair itself
+    opb.eval_pair(fb_t.data_ptr(), Qb_t.data_ptr(), 5, opa, fa_t.data_ptr(), Qa_t.data_ptr(), 6,
+                  torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ea, eb = rel(Qa_t.cpu().numpy(), Qa), rel(Qb_t.cpu().numpy(), Qb)
+    say("pair vs separate", precision, ea, eb)
+    tol = 1e-13 if precision == 64 else 1e-5
+    assert ea < tol and eb < tol
