@@ -770,6 +770,7 @@ struct VhsHandle {
   void* S = nullptr;
   void* W = nullptr;
   int groups_override = 0;
+  int slots_override = 0;  // cap on the persistent multi-pass grid (0: occupancy)
   int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident
   int last_algorithm = 0;
   void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
@@ -970,7 +971,8 @@ static int stage_finish(VhsHandle* h, const double* fc, double* Qc, int npts, in
 template <typename T, int N>
 static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipStream_t st) {
   const bool rr = use_rr(h);
-  const int slots = rr ? cu_count() : h->cap_slots;
+  int slots = rr ? cu_count() : h->cap_slots;
+  if (!rr && h->slots_override > 0) slots = std::min(slots, h->slots_override);
   const int groups = pick_groups(h, npts, slots);
   GainArgs<T> ga;
   int rc = stage_prep<T, N>(h, fc, npts, groups, slots, rr, &ga, st);
@@ -1261,6 +1263,14 @@ extern "C" int dgfs_vhs_set_groups(void* p, int ngroups) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
   if (ngroups < 0) return fail(DGFS_ERR_INVALID_VALUE, "ngroups < 0");
   h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vhs_set_slots(void* p, int nslots) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (nslots < 0) return fail(DGFS_ERR_INVALID_VALUE, "nslots < 0");
+  h->slots_override = nslots;
   return DGFS_OK;
 }
 

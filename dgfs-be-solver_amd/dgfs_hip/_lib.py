@@ -92,6 +92,7 @@ _SIGNATURES = {
     "dgfs_vhs_destroy": [_P],
     "dgfs_vhs_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "dgfs_vhs_set_groups": [_P, _I],
+    "dgfs_vhs_set_slots": [_P, _I],
     "dgfs_vhs_set_timing": [_P, _I],
     "dgfs_vhs_set_algorithm": [_P, _I],
     "dgfs_vhs_get_algorithm": [_P, C.POINTER(_I)],

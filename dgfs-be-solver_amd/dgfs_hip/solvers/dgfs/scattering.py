@@ -63,6 +63,7 @@ class VHSOperator:
         _lib.call("dgfs_vhs_create", C.byref(desc), C.byref(h))
         self._h = h
         self.Nv, self.vsize, self.precision, self.project = Nv, Nv ** 3, precision, project
+        self.Nrho, self.M, self.K = Nrho, M, Nrho * M
 
     @classmethod
     def from_mesh(cls, vm, gamma, prefac, precision=64, project=False):
@@ -85,6 +86,10 @@ class VHSOperator:
 
     def set_groups(self, ngroups):
         _lib.call("dgfs_vhs_set_groups", self._h, int(ngroups))
+
+    def set_slots(self, nslots):
+        """Cap the multi-pass persistent grid (0: occupancy limit)."""
+        _lib.call("dgfs_vhs_set_slots", self._h, int(nslots))
 
     def get_config(self):
         g, s, t = C.c_int(), C.c_int(), C.c_int()

@@ -94,6 +94,10 @@ int dgfs_vhs_destroy(void* handle);
 int dgfs_vhs_get_config(void* handle, int* ngroups, int* nslots, int* threads);
 /* Override the radial-group split (0 = automatic). */
 int dgfs_vhs_set_groups(void* handle, int ngroups);
+/* Cap the persistent grid of the multi-pass kernels at nslots workgroups
+ * (0 = the occupancy limit).  With set_groups and set_algorithm, the launch
+ * shape the autotuner (dgfs_hip/autotune.py) searches. */
+int dgfs_vhs_set_slots(void* handle, int nslots);
 
 /* Gain-kernel algorithm: 0 automatic, 1 multi-pass (slab + pencil passes
  * through a per-workgroup scratch, any N and precision), 2 register-resident

@@ -233,3 +233,23 @@ def test_pair_launch_matches_separate(precision, device):
     say("pair vs separate", precision, ea, eb)
     tol = 1e-13 if precision == 64 else 1e-5
     assert ea < tol and eb < tol
+
+
+def test_autotuner_picks_a_valid_shape(tmp_path, device):
+    """The tuner sweeps algorithm x radial groups x persistent grid on a
+    sample batch, applies the fastest, caches it, and Q is unchanged."""
+    import torch
+    from dgfs_hip.autotune import Autotuner
+    op, vm = make_op(16, 6.0, 8, 6, 0.38, 1.0, precision=32)
+    f = smooth_states(O.VelocityMesh(16, 6.0, 8, 6), 48, seed=4)
+    Q0 = run_contig(op, f, device)
+    ft = torch.as_tensor(f, device=device)
+    Qt = torch.empty_like(ft)
+    tuner = Autotuner(cache_path=str(tmp_path / "tune.json"), repeats=1)
+    best, ms = tuner.tune(op, ft.data_ptr(), Qt.data_ptr(), 48)
+    say("autotune best", best, "of", len(ms), "min ms", min(ms.values()))
+    assert len(ms) >= 10 and best["algorithm"] in (1, 2)
+    Q1 = run_contig(op, f, device)
+    assert rel(Q1, Q0) < 1e-5
+    best2, _ = Autotuner(cache_path=str(tmp_path / "tune.json")).tune(op, ft.data_ptr(), Qt.data_ptr(), 48)
+    assert best2 == best
