@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--rhs-steps", type=int, default=3,
                     help="timed SSP-RK2 steps of the 16x16 p=3 cavity (two RHS evaluations each); 0 skips")
+    ap.add_argument("--sur-train", type=int, default=32768,
+                    help="training states of the learned surrogate (0 skips the surrogate section)")
+    ap.add_argument("--sur-epochs", type=int, default=12)
+    ap.add_argument("--sur-threshold", type=float, default=0.05)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -139,6 +143,133 @@ def bench_rhs(args, dev):
     gs.close()
     del sysm, be
     torch.cuda.empty_cache()
+    return out
+
+
+def near_equilibrium_states(vm, npts, seed, device, eps_max=0.25, frac_far=0.1):
+    """Seeded states of a low-Knudsen flow field: Maxwellians with random
+    density, velocity and temperature carrying a Chapman-Enskog-like heat-flux
+    and shear perturbation of amplitude eps = eps_max * U(0,1)^2 (most points
+    near equilibrium), plus a fraction frac_far of bimodal / anisotropic
+    states far from it (the make_states recipe)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    cv = torch.as_tensor(vm.cv(), device=device)
+    out = torch.empty((npts, vm.vsize()), dtype=torch.float64, device=device)
+    nfar = int(round(frac_far * npts))
+    if nfar:
+        out[:nfar] = make_states(vm, nfar, seed + 1, device)
+    for i in range(nfar, npts):
+        u = rng.uniform(-0.4, 0.4, 3)
+        T = rng.uniform(0.7, 1.4)
+        rho = rng.uniform(0.6, 1.4)
+        q = rng.normal(size=3)
+        S = rng.normal(size=(3, 3))
+        S = S + S.T
+        S -= np.trace(S) / 3 * np.eye(3)
+        eps = eps_max * rng.uniform() ** 2
+        c = (cv - torch.as_tensor(u, device=device).view(3, 1)) / math.sqrt(T)
+        c2 = (c * c).sum(0)
+        phi = eps * (0.4 * (torch.as_tensor(q, device=device) @ c) * (c2 - 2.5)
+                     + 0.5 * torch.einsum("ij,ir,jr->r", torch.as_tensor(S, device=device), c, c))
+        out[i] = rho / (math.pi * T) ** 1.5 * torch.exp(-c2) * (1 + phi)
+    perm = torch.as_tensor(rng.permutation(npts), device=device)
+    return out[perm].contiguous()
+
+
+def bench_surrogate(args, vm, gamma, dev, stream):
+    """Learned surrogate with fallback on 4096 points of a low-Knudsen field:
+    trained on the GPU (HIP forward/backward/Adam) on (f, Q) pairs the exact
+    FP32 operator produces, then timed against the exact operators."""
+    import torch
+
+    from dgfs_hip.solvers.dgfs.scattering import VHSOperator
+    from dgfs_hip.surrogate import SpectralSurrogate
+
+    P = args.points
+    exact32 = VHSOperator.from_mesh(vm, gamma, 1.0, precision=32, project=True)
+    sur = SpectralSurrogate(vm.Nv(), vm.L(), vm.cw(), gamma, 1.0, max_batch=2048, device=dev, seed=0,
+                            init_scale=0.3)
+    # training data, 4096 states at a time
+    t0 = time.perf_counter()
+    xs, lts, ys = [], [], []
+    for c0 in range(0, args.sur_train, 4096):
+        nb = min(4096, args.sur_train - c0)
+        f = near_equilibrium_states(vm, nb, seed=1000 + c0, device=dev)
+        Q = torch.empty_like(f)
+        exact32.eval_contig(f.data_ptr(), Q.data_ptr(), nb, stream)
+        x, lt, y, _ = sur.make_dataset(f, Q)
+        xs.append(x), lts.append(lt), ys.append(y)
+        del f, Q
+    x, lt, y = torch.cat(xs), torch.cat(lts), torch.cat(ys)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = sur.train(x, lt, y, epochs=args.sur_epochs, batch=256, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del x, lt, y, xs, lts, ys
+    # evaluation batch (unseen seed)
+    f = near_equilibrium_states(vm, P, seed=7, device=dev)
+    Qs = torch.empty_like(f)
+    nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=args.sur_threshold, stream=stream)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=args.sur_threshold, stream=stream)
+    torch.cuda.synchronize()
+    t_sur = (time.perf_counter() - t0) / reps
+    Q32 = torch.empty_like(f)
+    exact32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
+    torch.cuda.synchronize()
+    t_ex32 = time.perf_counter() - t0
+    exact32.close()
+    op64 = VHSOperator.from_mesh(vm, gamma, 1.0, precision=64, project=True)
+    Q64 = torch.empty_like(f)
+    op64.eval_contig(f.data_ptr(), Q64.data_ptr(), P, stream)
+    torch.cuda.synchronize()
+    op64.close()
+    # errors against FP64 (projected), overall and on the surrogate points
+    mom = torch.empty((P, 8), dtype=torch.float64, device=dev)
+    sur.encode(f, P, mom)
+    torch.cuda.synchronize()
+    ind = mom[:, 5]
+    mask = ind <= args.sur_threshold
+    err = float(torch.linalg.norm(Qs - Q64) / torch.linalg.norm(Q64))
+    err_sur_pts = float(torch.linalg.norm((Qs - Q64)[mask]) / torch.linalg.norm(Q64[mask])) if mask.any() else 0.0
+    # a closure without the network at the same cost: BGK, nu (M[f] - f), with
+    # the single nu that fits Q64 best over the surrogate points
+    cvt = torch.as_tensor(vm.cv(), device=dev)
+    Mf = mom[:, 0:1] / (math.pi * mom[:, 4:5]) ** 1.5 * torch.exp(
+        -((cvt[0][None] - mom[:, 1:2]) ** 2 + (cvt[1][None] - mom[:, 2:3]) ** 2
+          + (cvt[2][None] - mom[:, 3:4]) ** 2) / mom[:, 4:5])
+    d = (Mf - f)[mask]
+    nu = float((d * Q64[mask]).sum() / (d * d).sum()) if mask.any() else 0.0
+    err_zero = float(torch.linalg.norm(nu * d - Q64[mask]) / torch.linalg.norm(Q64[mask])) if mask.any() else 0.0
+    del Mf, d
+    cv = torch.as_tensor(vm.cv(), device=dev)
+    phi = torch.stack([torch.ones_like(cv[0]), cv[0], cv[1], cv[2], (cv ** 2).sum(0)])
+    cons = float((((Qs @ phi.T) * vm.cw()).abs() / (Qs.abs().sum(1, keepdim=True) * vm.cw())).max())
+    out = {
+        "workload": f"{P} points of a low-Knudsen field (seeded Maxwellians with Chapman-Enskog "
+                    f"perturbations, 10% bimodal/anisotropic), Nv={vm.Nv()}",
+        "value": P / t_sur, "unit": "Q(f) evals/s",
+        "fallback_threshold": args.sur_threshold,
+        "fallback_points": int(nfb), "fallback_rate": nfb / P,
+        "rel_l2_vs_fp64": err, "rel_l2_vs_fp64_on_surrogate_points": err_sur_pts,
+        "rel_l2_bgk_best_nu_on_surrogate_points": err_zero,
+        "conservation_max_rel": cons,
+        "exact_fp32_evals_per_s_same_batch": P / t_ex32,
+        "train_states": args.sur_train, "train_epochs": args.sur_epochs,
+        "train_loss_first_last": [losses[0], losses[-1]],
+        "data_gen_s": t_data, "train_s": t_train,
+    }
+    sur.close()
     return out
 
 
@@ -267,6 +398,8 @@ def main():
     achieved = bq32 * pts_per_launch / (gain_ms / max(gain_n, 1) / 1e3)
     traffic = None
     tr = read_traffic(f"vhs_gain_rr_fp32_n{nv}_traffic.json" if algo == 2 else "")
+    if algo == 3:
+        tr = read_traffic(f"vhs_gain_rr_inl_fp32_n{nv}_traffic.json")
     if tr and tr.get("points_per_dispatch"):
         traffic = tr["bytes_per_dispatch"] * pts_per_launch / tr["points_per_dispatch"]
 
@@ -298,7 +431,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": (f"k_vhs_gain_rr<float,{nv}>" if algo == 2 else f"k_vhs_gain<float,{nv}>"),
+            "kernel": (f"k_vhs_gain_rr<float,{nv},{'true' if algo == 3 else 'false'}>" if algo in (2, 3)
+                       else f"k_vhs_gain<float,{nv}>"),
             "algorithmic_bytes_per_eval": bq32,
             "points_per_launch": pts_per_launch,
             "avg_launch_ms": gain_ms / max(gain_n, 1),
@@ -363,6 +497,11 @@ def main():
 
     op32.close()
     torch.cuda.empty_cache()
+    if args.sur_train > 0:
+        line["surrogate"] = bench_surrogate(args, vm, gamma, dev, stream)
+        if "fp64_plain" in line:
+            line["surrogate"]["speedup_vs_fp64"] = line["surrogate"]["value"] / line["fp64_plain"]["value"]
+        torch.cuda.empty_cache()
     if args.rhs_steps > 0:
         line["rhs"] = bench_rhs(args, dev)
 

@@ -471,7 +471,10 @@ __device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx
   }
 }
 
-template <typename T, int N>
+// INL: the three Nyquist-plane transforms of every (p, q) are computed in the
+// kernel (from the transposed spectrum, into LDS) instead of being read from
+// the buffer k_vhs_nyq writes
+template <typename T, int N, bool INL>
 __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a) {
   constexpr int NT = RRShape<N>::NT, G = RRShape<N>::G, NN = N * N, n = N * N * N, h = N / 2;
   extern __shared__ char smem_raw[];
@@ -507,10 +510,61 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
           sincos(alpha * a.sz[q * 3 + d] * lv, &sn, &cs);
           eph[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
         }
-        // this (p, q)'s Nyquist planes into LDS (read after later barriers)
-        const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
-        for (int i = tid; i < 3 * NN; i += NT) Wl[i] = Wq[i];
-        __syncthreads();
+        if constexpr (INL) {
+          // B = (F/2)(conj D - i D) on the planes kx = h, ky = h (kx != h),
+          // kz = h (kx, ky != h), D = E - E~ (as k_vhs_nyq), then 2D IFFTs
+          __syncthreads();
+          for (int i = tid; i < 3 * NN; i += NT) {
+            const int j = i / NN, rem = i - j * NN, r = rem / N, c = rem - r * N;
+            int kx, ky, kz;
+            bool zero = false;
+            if (j == 0) {
+              kx = h; ky = r; kz = c;
+            } else if (j == 1) {
+              kx = r; ky = h; kz = c;
+              zero = (kx == h);
+            } else {
+              kx = r; ky = c; kz = h;
+              zero = (kx == h || ky == h);
+            }
+            cpx<T> v{T(0), T(0)};
+            if (!zero) {
+              const cpx<T> ex = eph[0][kx], ey = eph[1][ky], ez = eph[2][kz];
+              const cpx<T> E = ex * ey * ez;
+              const cpx<T> Et = (kx == h ? conj(ex) : ex) * (ky == h ? conj(ey) : ey) * (kz == h ? conj(ez) : ez);
+              const cpx<T> D = E - Et;
+              v = T(0.5) * (Ft[kx * NN + kz * N + ky] * (conj(D) - mul_i(D)));
+            }
+            Wl[i] = v;
+          }
+          __syncthreads();
+          if (tid < 3 * N) {  // along c
+            cpx<T> w[N];
+            cpx<T>* row = Wl + tid * N;
+#pragma unroll
+            for (int k = 0; k < N; ++k) w[k] = row[k];
+            FFT<T, N, +1>::run(w);
+#pragma unroll
+            for (int k = 0; k < N; ++k) row[k] = w[k];
+          }
+          __syncthreads();
+          if (tid < 3 * N) {  // along r
+            const int j = tid / N, c = tid - j * N;
+            cpx<T> w[N];
+            cpx<T>* col = Wl + j * NN + c;
+#pragma unroll
+            for (int k = 0; k < N; ++k) w[k] = col[k * N];
+            FFT<T, N, +1>::run(w);
+#pragma unroll
+            for (int k = 0; k < N; ++k) col[k * N] = w[k];
+          }
+          __syncthreads();
+        } else {
+          // this (p, q)'s Nyquist planes into LDS (read after later barriers)
+          const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
+          for (int i = tid; i < 3 * NN; i += NT) Wl[i] = Wq[i];
+          __syncthreads();
+        }
         rr_zy_inverse<T, N, G>(v0, Ft, Xb, eph, g, g, l);
         rr_zy_inverse<T, N, G>(v1, Ft, Xb, eph, g + G, g, l);
         T val[N];
@@ -771,7 +825,8 @@ struct VhsHandle {
   void* W = nullptr;
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent multi-pass grid (0: occupancy)
-  int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident
+  int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident (Nyquist planes
+                         // from k_vhs_nyq), 3 register-resident with in-kernel Nyquist planes
   int last_algorithm = 0;
   void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
   void* Ft = nullptr;    // spectrum transposed to [kx][kz][ky] [cap_pts][n]
@@ -851,7 +906,7 @@ constexpr bool rr_available() {
 static bool use_rr(const VhsHandle* h) {
   const bool avail = (h->prec == 32) && (h->N == 16 || h->N == 32);
   if (h->algorithm == 1) return false;
-  if (h->algorithm == 2) return avail;
+  if (h->algorithm == 2 || h->algorithm == 3) return avail;
   return avail;
 }
 
@@ -860,13 +915,16 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
   if constexpr (rr_available<T, N>()) {
     constexpr int NN = N * N;
     const int K = h->Nrho * h->M;
+    const bool inl = (h->algorithm == 3);
+    h->last_algorithm = inl ? 3 : 2;
     const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
-    if (wbytes > h->cap_wny) {
+    if (!inl && wbytes > h->cap_wny) {
       (void)hipFree(h->Wny);
       h->Wny = nullptr;
       DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
       h->cap_wny = wbytes;
     }
+    if (!inl) {
     NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Wny), h->alpha, h->szd, npts,
                   h->Nrho, h->M};
     const long nyq_items = static_cast<long>(npts) * K;
@@ -875,16 +933,22 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
                        dim3(Shape<N>::NT), 0, st, na);
     DGFS_LAUNCH_CHECK("vhs_nyq");
     t4.stop();
+    }
     KTimerT<VhsHandle> t1(h, 1, st);
     const size_t lds = (static_cast<size_t>(RRLds<N>::xsize) + RRLds<N>::wl) * sizeof(cpx<T>) +
                        static_cast<size_t>(RRLds<N>::s0) * sizeof(T);
-    DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vhs_gain_rr<T, N>),
+    const void* kfn = inl ? reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, true>)
+                          : reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, false>);
+    DGFS_HIP_TRY(hipFuncSetAttribute(kfn,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cu_count())));
     GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
                      static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
                      static_cast<int>(items), groups, ppg, h->Nrho, h->M};
-    hipLaunchKernelGGL((k_vhs_gain_rr<T, N>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
+    if (inl)
+      hipLaunchKernelGGL((k_vhs_gain_rr<T, N, true>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
+    else
+      hipLaunchKernelGGL((k_vhs_gain_rr<T, N, false>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
     DGFS_LAUNCH_CHECK("vhs_gain_rr");
     t1.stop();
     h->threads = RRShape<N>::NT;
@@ -978,7 +1042,6 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   int rc = stage_prep<T, N>(h, fc, npts, groups, slots, rr, &ga, st);
   if (rc) return rc;
   if (rr) {
-    h->last_algorithm = 2;
     rc = run_gain_rr<T, N>(h, npts, groups, ga.ppg, static_cast<size_t>(ga.nitems), st);
     if (rc) return rc;
   } else {
@@ -1301,8 +1364,8 @@ extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
 extern "C" int dgfs_vhs_set_algorithm(void* p, int algorithm) {
   VhsHandle* h = as_handle(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
-  if (algorithm < 0 || algorithm > 2) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1 or 2");
-  if (algorithm == 2 && !(h->prec == 32 && (h->N == 16 || h->N == 32)))
+  if (algorithm < 0 || algorithm > 3) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1, 2 or 3");
+  if (algorithm >= 2 && !(h->prec == 32 && (h->N == 16 || h->N == 32)))
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel: FP32 and Nv in {16, 32} only");
   h->algorithm = algorithm;
   return DGFS_OK;

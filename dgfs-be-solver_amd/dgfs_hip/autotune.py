@@ -3,7 +3,7 @@
 For a given operator (velocity grid, precision, K = M Nrho) and batch size it
 times every candidate of
   * the gain algorithm: multi-pass (any grid) or register-resident (FP32,
-    Nv 16 / 32);
+    Nv 16 / 32; Nyquist planes from a separate kernel or inside the kernel);
   * the radial-group split (work items per point: 1, 2, 4, ..., Nrho);
   * the persistent grid of the multi-pass kernels (1, 2, 4 workgroups per CU,
     or the occupancy limit),
@@ -50,7 +50,7 @@ class Autotuner:
     def candidates(op, cus):
         algos = [1]
         if op.precision == 32 and op.Nv in (16, 32):
-            algos.insert(0, 2)
+            algos[:0] = [2, 3]
         groups = [0]
         g = 1
         while g <= op.Nrho:
@@ -59,7 +59,7 @@ class Autotuner:
         out = []
         for a in algos:
             for gr in groups:
-                slots = [0] if a == 2 else [0, cus, 2 * cus, 4 * cus]
+                slots = [0] if a >= 2 else [0, cus, 2 * cus, 4 * cus]
                 for sl in slots:
                     out.append({"algorithm": a, "groups": gr, "slots": sl})
         return out

@@ -97,7 +97,8 @@ class VHSOperator:
         return {"ngroups": g.value, "nslots": s.value, "threads": t.value}
 
     def set_algorithm(self, algorithm):
-        """0 automatic, 1 multi-pass, 2 register-resident (FP32, Nv 16/32)."""
+        """0 automatic, 1 multi-pass, 2 register-resident (FP32, Nv 16/32),
+        3 register-resident with in-kernel Nyquist planes."""
         _lib.call("dgfs_vhs_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):

@@ -102,7 +102,8 @@ int dgfs_vhs_set_slots(void* handle, int nslots);
 /* Gain-kernel algorithm: 0 automatic, 1 multi-pass (slab + pencil passes
  * through a per-workgroup scratch, any N and precision), 2 register-resident
  * (the whole N^3 field in registers, FP32 with Nv 16 or 32; Nyquist-plane
- * corrections precomputed).  get_algorithm reports the one the last eval ran.
+ * corrections precomputed by a separate kernel), 3 register-resident with the
+ * Nyquist-plane corrections computed inside the gain kernel.  get_algorithm reports the one the last eval ran.
  * No reference counterpart (the reference has one cuFFT dataflow). */
 int dgfs_vhs_set_algorithm(void* handle, int algorithm);
 int dgfs_vhs_get_algorithm(void* handle, int* algorithm);
