@@ -173,3 +173,60 @@ def test_moments_kernel(device):
     qx = (f * c[0] * c2).sum(1) * cw
     assert np.allclose(mom[:, 5], qx, rtol=1e-10, atol=1e-13)
     assert np.allclose(mom[:, 14], rho * T, rtol=1e-12)
+
+
+def test_triangle_mesh_equilibrium_and_mass(device):
+    """Triangle elements (p = 2, Williams-Shunn points): a wall-equilibrium
+    state is a steady state of the advection (rhs = 0), and a perturbed state
+    keeps its total mass over SSP-RK2 steps (x-periodic, diffuse walls)."""
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import couette_cfg
+    from dgfs_hip.integrators.steppers import get_stepper
+    from dgfs_hip.mesh import rect_tris
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    cfg = couette_cfg(Nv=16, L=5.0, Nrho=4, M=6, Kn=1.0, omega=0.5, order=2, uwall=0.0)
+    cfg.set("solver-elements-tri", "soln-pts", "williams-shunn")
+    be = HIPBackend(cfg)
+    mesh = rect_tris(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True)
+    sysm = DGFSSystem(be, mesh, cfg, 3)
+    vm = sysm.vm
+    m = OF.maxwellian(vm.cv(), 1.0, [0.0, 0.0, 0.0], 1.0)
+    nupts, neles = sysm.ele.nupts, sysm.ele.neles
+    f = np.broadcast_to(m[None, :, None], (nupts, vm.vsize(), neles)).copy()
+    sysm.ele.scal_upts[0].set(f)
+    sysm.updatebc(0.0, 0, 1, 0, 0)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    r = sysm.ele.scal_upts[1].get()
+    say("tri equilibrium max |rhs|", np.abs(r).max())
+    assert np.abs(r).max() < 1e-11 * np.abs(f).max() * 10
+    # perturbed state: total mass (sum over points weighted by the
+    # element-local nodal quadrature) is conserved by the scheme
+    rng = np.random.default_rng(8)
+    f1 = f * (1 + 0.05 * rng.random((nupts, 1, neles)))
+    sysm.eles_scal_upts_full[0].set(f1)
+    st = get_stepper("dgfs-tvd-rk2", be, sysm, 0.002)
+    djac = sysm.ele.djac_at_np("upts")          # (nupts, neles)
+    W = _tri_nodal_weights(sysm.ele)            # (nupts,)
+
+    def mass(a):
+        return float(np.einsum("p,pe,pve->", W, djac, a) * vm.cw())
+
+    m0 = mass(f1)
+    st.advance_to(0.01)
+    m1 = mass(sysm.eles_scal_upts_full[st.soln_bank].get())
+    say("tri mass drift", abs(m1 - m0) / m0)
+    assert abs(m1 - m0) / m0 < 1e-10
+
+
+def _tri_nodal_weights(ele):
+    """W_p = integral over the reference triangle of the nodal basis function
+    l_p (collapsed Gauss-Legendre rule of 12 x 12 points, exact for the
+    degree-2 basis)."""
+    a, wa = np.polynomial.legendre.leggauss(12)
+    A, B = np.meshgrid(a, a, indexing="ij")
+    WA = np.outer(wa, wa) * (1 - B) / 2
+    xi = (1 + A) * (1 - B) / 2 - 1
+    pts = np.stack([xi.ravel(), B.ravel()], axis=1)
+    L = ele.basis.ubasis.nodal_basis_at(pts)     # (npts, nupts)
+    return WA.ravel() @ L
