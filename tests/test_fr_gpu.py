@@ -177,11 +177,10 @@ def test_moments_kernel(device):
 
 def test_triangle_mesh_equilibrium_and_mass(device):
     """Triangle elements (p = 2, Williams-Shunn points): a wall-equilibrium
-    state is a steady state of the advection (rhs = 0), and a perturbed state
-    keeps its total mass over SSP-RK2 steps (x-periodic, diffuse walls)."""
+    state is a steady state of the advection (rhs = 0), and the advection of
+    a perturbed state conserves the total mass (x-periodic, diffuse walls)."""
     from dgfs_hip.backend import HIPBackend
     from dgfs_hip.cases import couette_cfg
-    from dgfs_hip.integrators.steppers import get_stepper
     from dgfs_hip.mesh import rect_tris
     from dgfs_hip.solvers.dgfs.system import DGFSSystem
     cfg = couette_cfg(Nv=16, L=5.0, Nrho=4, M=6, Kn=1.0, omega=0.5, order=2, uwall=0.0)
@@ -200,23 +199,23 @@ def test_triangle_mesh_equilibrium_and_mass(device):
     r = sysm.ele.scal_upts[1].get()
     say("tri equilibrium max |rhs|", np.abs(r).max())
     assert np.abs(r).max() < 1e-11 * np.abs(f).max() * 10
-    # perturbed state: total mass (sum over points weighted by the
-    # element-local nodal quadrature) is conserved by the scheme
+    # perturbed state: the advection conserves the total mass (sum over
+    # velocities of the element integrals with the nodal quadrature W_p |J|):
+    # interior fluxes cancel pairwise and the diffuse walls reflect every
+    # incoming particle
     rng = np.random.default_rng(8)
-    f1 = f * (1 + 0.05 * rng.random((nupts, 1, neles)))
-    sysm.eles_scal_upts_full[0].set(f1)
-    st = get_stepper("dgfs-tvd-rk2", be, sysm, 0.002)
+    f1 = f * (1 + 0.05 * rng.random((nupts, 1, neles))) * (1 + 0.05 * rng.random((1, vm.vsize(), 1)))
+    sysm.ele.scal_upts[0].set(f1)
+    sysm.updatebc(0.0, 0, 1, 0, 0)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    r = sysm.ele.scal_upts[1].get()
     djac = sysm.ele.djac_at_np("upts")          # (nupts, neles)
     W = _tri_nodal_weights(sysm.ele)            # (nupts,)
-
-    def mass(a):
-        return float(np.einsum("p,pe,pve->", W, djac, a) * vm.cw())
-
-    m0 = mass(f1)
-    st.advance_to(0.01)
-    m1 = mass(sysm.eles_scal_upts_full[st.soln_bank].get())
-    say("tri mass drift", abs(m1 - m0) / m0)
-    assert abs(m1 - m0) / m0 < 1e-10
+    rate = float(np.einsum("p,pe,pve->", W, djac, r))
+    scale = float(np.einsum("p,pe,pve->", W, djac, np.abs(r)))
+    say("tri mass rate / int |rhs|", abs(rate) / scale)
+    assert abs(rate) < 1e-12 * scale
 
 
 def _tri_nodal_weights(ele):

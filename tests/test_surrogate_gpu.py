@@ -164,28 +164,31 @@ def _exact_op(N, L, Nrho, M, gamma, precision=64, project=True):
 
 def test_training_gains_over_untrained_and_bgk(device):
     """The network trained on the exact operator's (f, Q) pairs predicts Q of
-    held-out near-equilibrium states better than the untrained network and
-    better than the BGK closure nu (M[f] - f) with the best single nu."""
+    held-out near-equilibrium states far better than the untrained network and
+    better than the BGK closure nu (M[f] - f) with the best single nu.  At
+    24^3 (h = 0.5) the 8^3-mode representation itself is exact to 0.7% and a
+    least-squares linear map of the modes reaches 4% (measured with the
+    checker on 1200 states)."""
     import torch
-    N, L, Nrho, M, gamma = 16, 6.0, 8, 6, 0.38
+    N, L, Nrho, M, gamma = 24, 6.0, 8, 6, 0.38
     op = _exact_op(N, L, Nrho, M, gamma)
-    ntr, nte = 3072, 128
+    ntr, nte = 4096, 256
     f = torch.as_tensor(near_equilibrium_states(N, L, ntr + nte, seed=7), device=device)
     Q = torch.empty_like(f)
     op.eval_contig(f.data_ptr(), Q.data_ptr(), ntr + nte, torch.cuda.current_stream().cuda_stream)
     sur = _sur(device, Nv=N, L=L, max_batch=256, scale=0.3)
     x, lt, y, mom = sur.make_dataset(f, Q)
+    say("target modes: max |y|", float(y.abs().max()), "rms", float(y.pow(2).mean().sqrt()))
 
-    def test_err():
-        yp = sur.predict(x[ntr:], lt[ntr:])
+    def field_err(yp):
         Qs = torch.empty_like(f[ntr:])
-        sur.decode(yp, mom[ntr:].contiguous(), nte, Qs, project=True)
+        sur.decode(yp.contiguous(), mom[ntr:].contiguous(), nte, Qs, project=True)
         return float(torch.linalg.norm(Qs - Q[ntr:]) / torch.linalg.norm(Q[ntr:]))
 
-    say("target modes: max |y|", float(y.abs().max()), "rms", float(y.pow(2).mean().sqrt()))
-    e0 = test_err()
-    losses = sur.train(x[:ntr], lt[:ntr], y[:ntr], epochs=25, batch=128, lr=2e-3, log=say)
-    e1 = test_err()
+    floor = field_err(y[ntr:])
+    e0 = field_err(sur.predict(x[ntr:], lt[ntr:]))
+    losses = sur.train(x[:ntr], lt[:ntr], y[:ntr], epochs=30, batch=128, lr=1e-3, log=say)
+    e1 = field_err(sur.predict(x[ntr:], lt[ntr:]))
     # BGK with the least-squares optimal single frequency over the test set
     c0 = torch.as_tensor(S.grid(N, L), device=device)
     I, J, K = torch.meshgrid(c0, c0, c0, indexing="ij")
@@ -196,10 +199,12 @@ def test_training_gains_over_untrained_and_bgk(device):
     d = Mx - f[ntr:]
     nu = float((d * Q[ntr:]).sum() / (d * d).sum())
     e_bgk = float(torch.linalg.norm(nu * d - Q[ntr:]) / torch.linalg.norm(Q[ntr:]))
-    say("surrogate test error: untrained", e0, "trained", e1, "bgk", e_bgk, "loss", losses[0], "->", losses[-1])
-    assert losses[-1] < 0.1 * losses[0]
-    assert e1 < 0.5 * e0
-    assert e1 < e_bgk
+    say("surrogate test error: floor", floor, "untrained", e0, "trained", e1, "bgk", e_bgk,
+        "loss", losses[0], "->", losses[-1])
+    assert floor < 0.02
+    assert losses[-1] < 0.2 * losses[0]
+    assert e1 < 0.2 * e0
+    assert e1 < 0.5 * e_bgk
 
 
 def test_eval_with_fallback_routes_points(device):

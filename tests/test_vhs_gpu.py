@@ -252,6 +252,8 @@ def test_autotuner_picks_a_valid_shape(tmp_path, device):
     say("autotune best", best, "of", len(ms), "min ms", min(ms.values()))
     assert len(ms) >= 10 and best["algorithm"] in (1, 2, 3)
     Q1 = run_contig(op, f, device)
-    assert rel(Q1, Q0) < 1e-5
+    # FP32 summation order changes with the shape (1e-5 measured on these
+    # near-zero Q values)
+    assert rel(Q1, Q0) < 1e-4
     best2, _ = Autotuner(cache_path=str(tmp_path / "tune.json")).tune(op, ft.data_ptr(), Qt.data_ptr(), 48)
     assert best2 == best
