@@ -50,8 +50,11 @@ def parse():
                     help="timed SSP-RK2 steps of the 16x16 p=3 cavity (two RHS evaluations each); 0 skips")
     ap.add_argument("--sur-train", type=int, default=32768,
                     help="training states of the learned surrogate (0 skips the surrogate section)")
-    ap.add_argument("--sur-epochs", type=int, default=12)
+    ap.add_argument("--sur-epochs", type=int, default=20)
     ap.add_argument("--sur-threshold", type=float, default=0.05)
+    ap.add_argument("--algorithm", type=int, default=0,
+                    help="gain kernel of the FP32 path: 0 automatic, 1 multi-pass, 2 register-resident, "
+                         "3 register-resident with in-kernel Nyquist planes")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -146,12 +149,12 @@ def bench_rhs(args, dev):
     return out
 
 
-def near_equilibrium_states(vm, npts, seed, device, eps_max=0.25, frac_far=0.1):
+def near_equilibrium_states(vm, npts, seed, device, eps_range=(1e-3, 0.3), frac_far=0.05):
     """Seeded states of a low-Knudsen flow field: Maxwellians with random
     density, velocity and temperature carrying a Chapman-Enskog-like heat-flux
-    and shear perturbation of amplitude eps = eps_max * U(0,1)^2 (most points
-    near equilibrium), plus a fraction frac_far of bimodal / anisotropic
-    states far from it (the make_states recipe)."""
+    and shear perturbation whose amplitude eps is log-uniform in eps_range
+    (most points near equilibrium, a tail far from it), plus a fraction
+    frac_far of bimodal / anisotropic states (the make_states recipe)."""
     import numpy as np
     import torch
     rng = np.random.default_rng(seed)
@@ -168,7 +171,7 @@ def near_equilibrium_states(vm, npts, seed, device, eps_max=0.25, frac_far=0.1):
         S = rng.normal(size=(3, 3))
         S = S + S.T
         S -= np.trace(S) / 3 * np.eye(3)
-        eps = eps_max * rng.uniform() ** 2
+        eps = math.exp(rng.uniform(math.log(eps_range[0]), math.log(eps_range[1])))
         c = (cv - torch.as_tensor(u, device=device).view(3, 1)) / math.sqrt(T)
         c2 = (c * c).sum(0)
         phi = eps * (0.4 * (torch.as_tensor(q, device=device) @ c) * (c2 - 2.5)
@@ -199,8 +202,9 @@ def bench_surrogate(args, vm, gamma, dev, stream):
         f = near_equilibrium_states(vm, nb, seed=1000 + c0, device=dev)
         Q = torch.empty_like(f)
         exact32.eval_contig(f.data_ptr(), Q.data_ptr(), nb, stream)
-        x, lt, y, _ = sur.make_dataset(f, Q)
-        xs.append(x), lts.append(lt), ys.append(y)
+        x, lt, y, mo = sur.make_dataset(f, Q)
+        keep = mo[:, 5] <= 0.2  # the network only sees points near equilibrium
+        xs.append(x[keep]), lts.append(lt[keep]), ys.append(y[keep])
         del f, Q
     x, lt, y = torch.cat(xs), torch.cat(lts), torch.cat(ys)
     torch.cuda.synchronize()
@@ -210,17 +214,8 @@ def bench_surrogate(args, vm, gamma, dev, stream):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del x, lt, y, xs, lts, ys
-    # evaluation batch (unseen seed)
+    # evaluation batch (unseen seed) and the exact references
     f = near_equilibrium_states(vm, P, seed=7, device=dev)
-    Qs = torch.empty_like(f)
-    nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=args.sur_threshold, stream=stream)
-    torch.cuda.synchronize()
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=args.sur_threshold, stream=stream)
-    torch.cuda.synchronize()
-    t_sur = (time.perf_counter() - t0) / reps
     Q32 = torch.empty_like(f)
     exact32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
     torch.cuda.synchronize()
@@ -228,42 +223,64 @@ def bench_surrogate(args, vm, gamma, dev, stream):
     exact32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
     torch.cuda.synchronize()
     t_ex32 = time.perf_counter() - t0
-    exact32.close()
     op64 = VHSOperator.from_mesh(vm, gamma, 1.0, precision=64, project=True)
     Q64 = torch.empty_like(f)
     op64.eval_contig(f.data_ptr(), Q64.data_ptr(), P, stream)
     torch.cuda.synchronize()
     op64.close()
-    # errors against FP64 (projected), overall and on the surrogate points
     mom = torch.empty((P, 8), dtype=torch.float64, device=dev)
     sur.encode(f, P, mom)
     torch.cuda.synchronize()
     ind = mom[:, 5]
-    mask = ind <= args.sur_threshold
-    err = float(torch.linalg.norm(Qs - Q64) / torch.linalg.norm(Q64))
-    err_sur_pts = float(torch.linalg.norm((Qs - Q64)[mask]) / torch.linalg.norm(Q64[mask])) if mask.any() else 0.0
-    # a closure without the network at the same cost: BGK, nu (M[f] - f), with
-    # the single nu that fits Q64 best over the surrogate points
+    # a closure without the network at the same cost: BGK, nu (M[f] - f),
+    # with the single nu that fits Q64 best over the points it would replace
     cvt = torch.as_tensor(vm.cv(), device=dev)
     Mf = mom[:, 0:1] / (math.pi * mom[:, 4:5]) ** 1.5 * torch.exp(
         -((cvt[0][None] - mom[:, 1:2]) ** 2 + (cvt[1][None] - mom[:, 2:3]) ** 2
           + (cvt[2][None] - mom[:, 3:4]) ** 2) / mom[:, 4:5])
-    d = (Mf - f)[mask]
-    nu = float((d * Q64[mask]).sum() / (d * d).sum()) if mask.any() else 0.0
-    err_zero = float(torch.linalg.norm(nu * d - Q64[mask]) / torch.linalg.norm(Q64[mask])) if mask.any() else 0.0
-    del Mf, d
     cv = torch.as_tensor(vm.cv(), device=dev)
     phi = torch.stack([torch.ones_like(cv[0]), cv[0], cv[1], cv[2], (cv ** 2).sum(0)])
-    cons = float((((Qs @ phi.T) * vm.cw()).abs() / (Qs.abs().sum(1, keepdim=True) * vm.cw())).max())
+    Qs = torch.empty_like(f)
+    per_thr = {}
+    for thr in sorted({args.sur_threshold, 0.02, 0.1}):
+        nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=thr, stream=stream)
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nfb = sur.eval_with_fallback(exact32, f, Qs, P, threshold=thr, stream=stream)
+        torch.cuda.synchronize()
+        t_sur = (time.perf_counter() - t0) / reps
+        mask = ind <= thr
+        err = float(torch.linalg.norm(Qs - Q64) / torch.linalg.norm(Q64))
+        err_pts = float(torch.linalg.norm((Qs - Q64)[mask]) / torch.linalg.norm(Q64[mask])) if mask.any() else 0.0
+        d = (Mf - f)[mask]
+        if mask.any():
+            nu = float((d * Q64[mask]).sum() / (d * d).sum())
+            err_bgk = float(torch.linalg.norm(nu * d - Q64[mask]) / torch.linalg.norm(Q64[mask]))
+            Qb = Q64.clone()
+            Qb[mask] = nu * d
+            err_bgk_all = float(torch.linalg.norm(Qb - Q64) / torch.linalg.norm(Q64))
+            del Qb
+        else:
+            err_bgk = err_bgk_all = 0.0
+        cons = float((((Qs @ phi.T) * vm.cw()).abs() / (Qs.abs().sum(1, keepdim=True) * vm.cw())).max())
+        per_thr[thr] = {
+            "value": P / t_sur, "fallback_points": int(nfb), "fallback_rate": nfb / P,
+            "rel_l2_vs_fp64": err, "rel_l2_vs_fp64_on_surrogate_points": err_pts,
+            "rel_l2_bgk_best_nu_in_place_of_network": err_bgk_all,
+            "rel_l2_bgk_best_nu_on_surrogate_points": err_bgk,
+            "conservation_max_rel": cons,
+        }
+    del Mf
+    exact32.close()
+    head = per_thr[args.sur_threshold]
     out = {
         "workload": f"{P} points of a low-Knudsen field (seeded Maxwellians with Chapman-Enskog "
-                    f"perturbations, 10% bimodal/anisotropic), Nv={vm.Nv()}",
-        "value": P / t_sur, "unit": "Q(f) evals/s",
-        "fallback_threshold": args.sur_threshold,
-        "fallback_points": int(nfb), "fallback_rate": nfb / P,
-        "rel_l2_vs_fp64": err, "rel_l2_vs_fp64_on_surrogate_points": err_sur_pts,
-        "rel_l2_bgk_best_nu_on_surrogate_points": err_zero,
-        "conservation_max_rel": cons,
+                    f"perturbations of log-uniform amplitude 1e-3..0.3, 5% bimodal/anisotropic), "
+                    f"Nv={vm.Nv()}",
+        "unit": "Q(f) evals/s", "fallback_threshold": args.sur_threshold, **head,
+        "by_threshold": {str(k): v for k, v in per_thr.items()},
         "exact_fp32_evals_per_s_same_batch": P / t_ex32,
         "train_states": args.sur_train, "train_epochs": args.sur_epochs,
         "train_loss_first_last": [losses[0], losses[-1]],
@@ -371,6 +388,8 @@ def main():
     torch.cuda.synchronize()
 
     op32 = VHSOperator.from_mesh(vm, gamma, 1.0, precision=32, project=True)
+    if args.algorithm:
+        op32.set_algorithm(args.algorithm)
     for _ in range(args.warmup):
         op32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
     torch.cuda.synchronize()
