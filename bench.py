@@ -50,7 +50,7 @@ def parse():
                     help="timed SSP-RK2 steps of the 16x16 p=3 cavity (two RHS evaluations each); 0 skips")
     ap.add_argument("--sur-train", type=int, default=32768,
                     help="training states of the learned surrogate (0 skips the surrogate section)")
-    ap.add_argument("--sur-epochs", type=int, default=20)
+    ap.add_argument("--sur-epochs", type=int, default=60)
     ap.add_argument("--sur-threshold", type=float, default=0.05)
     ap.add_argument("--algorithm", type=int, default=0,
                     help="gain kernel of the FP32 path: 0 automatic, 1 multi-pass, 2 register-resident, "

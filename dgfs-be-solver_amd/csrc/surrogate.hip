@@ -164,6 +164,82 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
       }
 }
 
+// Large-tile BF16 variant: 128x128 block tile, 4 waves of 64x64 (4 x 4 MFMA
+// tiles each), K staged 32 at a time as BF16 in K-contiguous LDS rows
+// (As[m][k], Bs[n][k]) so every MFMA operand is one 8-byte LDS read.  Used
+// when both output dimensions are >= 128.
+constexpr int kBT = 128, kBK = 32, kBKP = kBK + 4;
+
+template <int UNUSED>
+__global__ __launch_bounds__(256) void k_gemm_bf16_big(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) short As[kBT][kBKP];
+  __shared__ __attribute__((aligned(16))) short Bs[kBT][kBKP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int m0 = blockIdx.y * kBT, n0 = blockIdx.x * kBT;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int kbeg = blockIdx.z * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
+  fl4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = fl4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = tid + 256 * i;  // 4096 elements of each tile
+      int mm, kk;
+      if (g.transA) { mm = e & 127; kk = e >> 7; } else { kk = e & 31; mm = e >> 5; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      float v = 0.f;
+      if (gm < g.M && gk < kend)
+        v = g.transA ? g.A[static_cast<long>(gk) * g.lda + gm] : g.A[static_cast<long>(gm) * g.lda + gk];
+      As[mm][kk] = to_bf16(v);
+      int nn, kb;
+      if (g.transB) { kb = e & 31; nn = e >> 5; } else { nn = e & 127; kb = e >> 7; }
+      const int gn = n0 + nn, gkb = k0 + kb;
+      float w = 0.f;
+      if (gn < g.N && gkb < kend)
+        w = g.transB ? g.B[static_cast<long>(gn) * g.ldb + gkb] : g.B[static_cast<long>(gkb) * g.ldb + gn];
+      Bs[nn][kb] = to_bf16(w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kBK / 16; ++s) {
+      sh4 av[4], bv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        av[t] = *reinterpret_cast<const sh4*>(&As[wm + t * 16 + li][s * 16 + 4 * lg]);
+        bv[t] = *reinterpret_cast<const sh4*>(&Bs[wn + t * 16 + li][s * 16 + 4 * lg]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + a * 16 + 4 * lg + r, col = n0 + wn + b * 16 + li;
+        if (row < g.M && col < g.N) {
+          if (g.part) {
+            g.part[(static_cast<long>(blockIdx.z) * g.M + row) * g.N + col] = acc[a][b][r];
+          } else {
+            float* cp = g.C + static_cast<long>(row) * g.ldc + col;
+            const float v = g.alpha * acc[a][b][r];
+            *cp = (g.beta == 0.f) ? v : v + g.beta * *cp;
+          }
+        }
+      }
+}
+
 __global__ __launch_bounds__(256) void k_gemm_reduce(const float* __restrict__ part, int splits, int M, int N,
                                                      float alpha, float beta, float* C, long ldc) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -191,6 +267,35 @@ struct GemmWs {
 static int gemm(GemmWs* ws, int transA, int transB, int M, int N, int K, float alpha, const float* A, long lda,
                 const float* B, long ldb, float beta, float* C, long ldc, bool bf16, hipStream_t st) {
   if (M <= 0 || N <= 0) return DGFS_OK;
+  if (bf16 && M >= kBT && N >= kBT) {
+    // large outputs: 128x128 tiles; split K when the output has few tiles
+    GemmArgs g{A, B, C, nullptr, lda, ldb, ldc, transA, transB, M, N, K, K, alpha, beta};
+    const int gx = (N + kBT - 1) / kBT, gy = (M + kBT - 1) / kBT, tiles = gx * gy;
+    int splits = 1;
+    if (tiles < 256 && K > 4096) splits = std::min(256 / tiles, (K + 2047) / 2048);
+    if (splits > 1 && ws) {
+      int kchunk = (K + splits - 1) / splits;
+      kchunk = (kchunk + kBK - 1) / kBK * kBK;
+      splits = (K + kchunk - 1) / kchunk;
+      int rc = ws->init();
+      if (rc) return rc;
+      if (static_cast<size_t>(splits) * M * N > kGemmPartFloats) splits = 1;
+      else {
+        g.kchunk = kchunk;
+        g.part = ws->part;
+        hipLaunchKernelGGL(k_gemm_bf16_big<0>, dim3(gx, gy, splits), dim3(256), 0, st, g);
+        DGFS_LAUNCH_CHECK("gemm_big(split)");
+        const long tot = static_cast<long>(M) * N;
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, st,
+                           ws->part, splits, M, N, alpha, beta, C, ldc);
+        DGFS_LAUNCH_CHECK("gemm_reduce");
+        return DGFS_OK;
+      }
+    }
+    hipLaunchKernelGGL(k_gemm_bf16_big<0>, dim3(gx, gy, 1), dim3(256), 0, st, g);
+    DGFS_LAUNCH_CHECK("gemm_big");
+    return DGFS_OK;
+  }
   GemmArgs g{A, B, C, nullptr, lda, ldb, ldc, transA, transB, M, N, K, K, alpha, beta};
   const int gx = (N + kGT - 1) / kGT, gy = (M + kGT - 1) / kGT;
   // split the reduction when the output has few tiles and K is long

@@ -30,7 +30,9 @@ def rel(a, b):
 
 @pytest.mark.parametrize("bf16", [False, True])
 @pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 70, 45, 37), (1, 0, 64, 96, 50), (0, 1, 33, 130, 64),
-                                         (1, 1, 128, 64, 17), (1, 0, 32, 32, 20000), (0, 1, 100, 80, 9000)])
+                                         (1, 1, 128, 64, 17), (1, 0, 32, 32, 20000), (0, 1, 100, 80, 9000),
+                                         (0, 0, 512, 1000, 512), (1, 0, 300, 260, 77), (0, 1, 512, 512, 8192),
+                                         (1, 1, 129, 200, 40)])
 def test_gemm_matches_numpy(ta, tb, M, N, K, bf16, device):
     import torch
     from dgfs_hip import surrogate as H
