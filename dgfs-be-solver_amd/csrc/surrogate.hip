@@ -251,10 +251,11 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float* __restrict__ p
   *cp = (beta == 0.f) ? alpha * s : alpha * s + beta * *cp;
 }
 
-// split-K partials: at most 256 / tiles splits of an M x N output covering
-// `tiles` 64x64 tiles, so splits * M * N <= 256 * 64 * 64 floats; allocated
-// once at that size and never resized while work may be in flight
-constexpr size_t kGemmPartFloats = 256u * kGT * kGT;
+// split-K partials: up to 256 / tiles splits of an M x N output; the weight
+// gradients (512 x 512 from K = 32 * batch, 32 x 32 from K = 512 * batch)
+// need at most 16 M floats.  Allocated once at that size and never resized
+// while work may be in flight; a split that would not fit runs unsplit.
+constexpr size_t kGemmPartFloats = size_t{1} << 24;
 struct GemmWs {
   float* part = nullptr;
   int init() {
