@@ -480,12 +480,21 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ th, const floa
   th[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// dst = BF16 rounding (nearest even) of src, kept in FP32 storage: the
+// network's BF16 weights, used with FP32 activations on the FP32 MFMA
+__global__ __launch_bounds__(256) void k_round_bf16(const float* __restrict__ src, float* __restrict__ dst, long n) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned u = static_cast<unsigned>(static_cast<unsigned short>(to_bf16(src[i]))) << 16;
+  dst[i] = __uint_as_float(u);
+}
+
 // ---------------------------------------------------------------------------
 // Hermite encoder / decoder (FP64), one workgroup per point
 // ---------------------------------------------------------------------------
 struct EncArgs {
   const double* f;  // [npts][n]
-  double* mom;      // [npts][8]: rho, ux, uy, uz, T, indicator, 0, 0
+  double* mom;      // [npts][8]: rho, ux, uy, uz, T, indicator, high-mode indicator, 0
   float* x;         // [npts][512] a_n (a_000 replaced by 0), or null
   float* lt;        // [npts] log T, or null
   double L, cw;
@@ -696,6 +705,19 @@ __global__ __launch_bounds__(256) void k_sur_encode(EncArgs a) {
       const double c000 = 1.0 / (Linv[0] * Linv[kSurD * kSurD] * Linv[2 * kSurD * kSurD]);
       herm_project<N>(f, V, Linv, t1, t2, b3, hs * hs * hs / Mn, a.x + static_cast<size_t>(pt) * kSurModes,
                       static_cast<float>(c000));
+      // second indicator: the share of f's norm in the modes of total degree
+      // >= 4 (zero for Maxwellians carrying the heat-flux and stress modes of
+      // Grad's 13-moment description)
+      double hi = 0.0, all = 0.0;
+      for (int e = threadIdx.x; e < kSurModes; e += blockDim.x) {
+        const int deg = e / (kSurD * kSurD) + (e / kSurD) % kSurD + e % kSurD;
+        const double v = b3[e];
+        all += v * v;
+        if (deg >= 4) hi += v * v;
+      }
+      hi = block_sum(hi, red);
+      all = block_sum(all, red);
+      if (threadIdx.x == 0) a.mom[static_cast<size_t>(pt) * 8 + 6] = sqrt(hi / all);
     }
     __syncthreads();
   }
@@ -1194,11 +1216,12 @@ extern "C" int dgfs_sur_decode(void* p, const float* y, const double* mom, const
 }
 
 // Surrogate with fallback: encode every point, evaluate the network on the
-// points whose indicator is <= threshold, decode and project them; run the
+// points whose indicator is <= threshold (and, if threshold_hi > 0, whose
+// high-mode indicator is <= threshold_hi), decode and project them; run the
 // exact operator (a dgfs_vhs handle) on the rest.  One host synchronisation
 // reads the indicators.  nfallback (optional) receives the fallback count.
 extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const double* f, double* Q, int npts,
-                             double threshold, int bf16, int* nfallback, void* stream) {
+                             double threshold, double threshold_hi, int bf16, int* nfallback, void* stream) {
   SurHandle* h = as_sur(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
   if (!vhs || !params || !f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null operand");
@@ -1220,7 +1243,9 @@ extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const doub
   sur.reserve(npts);
   for (int i = 0; i < npts; ++i) {
     const double ind = h->mom_h[static_cast<size_t>(i) * 8 + 5];
-    if (ind <= threshold && std::isfinite(ind)) sur.push_back(i); else fb.push_back(i);
+    const double ind_hi = h->mom_h[static_cast<size_t>(i) * 8 + 6];
+    const bool ok = ind <= threshold && std::isfinite(ind) && (threshold_hi <= 0.0 || ind_hi <= threshold_hi);
+    if (ok) sur.push_back(i); else fb.push_back(i);
   }
   if (nfallback) *nfallback = static_cast<int>(fb.size());
   const int ns = static_cast<int>(sur.size()), nf = static_cast<int>(fb.size());
@@ -1277,5 +1302,13 @@ extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const doub
                        idx_s, ns, h->n, 0);
     DGFS_LAUNCH_CHECK("rows_scatter");
   }
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_sur_round_bf16(long n, const float* src, float* dst, void* stream) {
+  if (!src || !dst || n < 0) return fail(DGFS_ERR_INVALID_VALUE, "bad rounding operands");
+  if (n == 0) return DGFS_OK;
+  hipLaunchKernelGGL(k_round_bf16, dim3(nblk(n)), dim3(256), 0, static_cast<hipStream_t>(stream), src, dst, n);
+  DGFS_LAUNCH_CHECK("round_bf16");
   return DGFS_OK;
 }

@@ -25,9 +25,10 @@ _lib.register({
     "dgfs_sur_encode": [_P, _P, _I, _P, _P, _P, _P],
     "dgfs_sur_targets": [_P, _P, _P, _I, _P, _P],
     "dgfs_sur_decode": [_P, _P, _P, _P, _I, _P, _I, _P],
-    "dgfs_sur_eval": [_P, _P, _P, _P, _P, _I, _D, _I, C.POINTER(_I), _P],
+    "dgfs_sur_eval": [_P, _P, _P, _P, _P, _I, _D, _D, _I, C.POINTER(_I), _P],
     "dgfs_nn_gemm": [_I, _I, _I, _I, _I, _F, _P, _L, _P, _L, _F, _P, _L, _I, _P],
     "dgfs_sur_param_count": [],
+    "dgfs_sur_round_bf16": [_L, _P, _P, _P],
 })
 
 MODES = 512
@@ -95,6 +96,10 @@ class SpectralSurrogate:
         th = init_params(seed, init_scale)
         assert th.size == self.nparams
         self.params = torch.as_tensor(th, dtype=torch.float32, device=self.device).contiguous()
+        # BF16 copy of the weights (FP32 storage) that forward passes use; the
+        # optimiser updates the FP32 master copy
+        self.params_bf16 = torch.empty_like(self.params)
+        self._round()
         self.grad = torch.zeros_like(self.params)
         self.m1 = torch.zeros_like(self.params)
         self.m2 = torch.zeros_like(self.params)
@@ -103,6 +108,10 @@ class SpectralSurrogate:
     @property
     def handle(self):
         return self._h
+
+    def _round(self):
+        _lib.call("dgfs_sur_round_bf16", int(self.nparams), _ptr(self.params), _ptr(self.params_bf16),
+                  self._stream())
 
     def _stream(self):
         import torch
@@ -119,18 +128,22 @@ class SpectralSurrogate:
         _lib.call("dgfs_sur_decode", self._h, _ptr(y), _ptr(mom), _ptr(idx), int(nidx), _ptr(Q),
                   int(bool(project)), self._stream())
 
-    def forward(self, x, lt, P, out, bf16=True, params=None):
-        _lib.call("dgfs_sur_forward", self._h, _ptr(self.params if params is None else params), _ptr(x), _ptr(lt),
-                  int(P), _ptr(out), int(bool(bf16)), self._stream())
+    def forward(self, x, lt, P, out, bf16=False, params=None):
+        """Network outputs.  Default: the BF16 weights with FP32 activations on
+        the FP32 MFMA (bf16=True rounds the activations to BF16 as well and
+        runs the BF16 MFMA)."""
+        _lib.call("dgfs_sur_forward", self._h, _ptr(self.params_bf16 if params is None else params), _ptr(x),
+                  _ptr(lt), int(P), _ptr(out), int(bool(bf16)), self._stream())
 
     def backward(self, dout, grad=None, params=None):
-        _lib.call("dgfs_sur_backward", self._h, _ptr(self.params if params is None else params), _ptr(dout),
+        _lib.call("dgfs_sur_backward", self._h, _ptr(self.params_bf16 if params is None else params), _ptr(dout),
                   _ptr(self.grad if grad is None else grad), self._stream())
 
     def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
         self.step += 1
         _lib.call("dgfs_sur_adam", int(self.nparams), _ptr(self.params), _ptr(self.grad), _ptr(self.m1),
                   _ptr(self.m2), float(lr), float(b1), float(b2), float(eps), int(self.step), self._stream())
+        self._round()
 
     # -- training ------------------------------------------------------------
     def make_dataset(self, f, Q):
@@ -153,14 +166,23 @@ class SpectralSurrogate:
         import torch
         return torch.ones(MODES, dtype=torch.float32, device=device)
 
-    def train(self, x, lt, y, epochs=10, batch=256, lr=1e-3, bf16=True, seed=0, log=None, lr_min=1e-5):
-        """Adam on the weighted mean-square error of the modes; returns the
-        per-epoch mean losses.  Gradient of the loss w.r.t. the outputs is
-        formed elementwise; everything else runs in the HIP kernels."""
+    def train(self, x, lt, y, epochs=10, batch=256, lr=1e-3, bf16=False, seed=0, log=None, lr_min=1e-5,
+              relative=False):
+        """Adam on the mean-square error of the modes (relative=True: each
+        sample's squared error divided by its target's mean square, so small
+        and large Q weigh alike); returns the per-epoch mean losses.  The
+        gradient of the loss w.r.t. the outputs is formed elementwise;
+        everything else runs in the HIP kernels."""
         import torch
         npts = x.shape[0]
         batch = min(batch, self.max_batch)
         w = self.mode_weights(self.device)
+        if relative:
+            ms = y.pow(2).mean(1)
+            sw = 1.0 / (ms + 1e-3 * float(ms.mean()))
+            sw = (sw / sw.mean()).view(-1, 1)
+        else:
+            sw = None
         out = torch.empty((batch, MODES), dtype=torch.float32, device=self.device)
         g = torch.Generator(device="cpu")
         g.manual_seed(seed)
@@ -169,25 +191,30 @@ class SpectralSurrogate:
         k = 0
         for ep in range(epochs):
             perm = torch.randperm(npts, generator=g).to(self.device)
-            tot, nb = 0.0, 0
+            # the epoch's loss is accumulated on the device: no host
+            # synchronisation inside the step loop
+            tot = torch.zeros((), dtype=torch.float64, device=self.device)
+            nb = 0
             for b0 in range(0, npts - batch + 1, batch):
                 idx = perm[b0:b0 + batch]
                 xb, ltb, yb = x[idx].contiguous(), lt[idx].contiguous(), y[idx].contiguous()
                 self.forward(xb, ltb, batch, out, bf16=bf16)
                 diff = out - yb
-                dout = (2.0 / (batch * MODES)) * w * diff
+                if sw is not None:
+                    diff = diff * sw[idx].sqrt()
+                dout = (2.0 / (batch * MODES)) * w * (diff * sw[idx].sqrt() if sw is not None else diff)
                 self.backward(dout.contiguous())
                 lr_k = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(1, steps_total)))
                 self.adam(lr_k)
                 k += 1
-                tot += float((w * diff * diff).mean())
+                tot += (w * diff * diff).mean()
                 nb += 1
-            losses.append(tot / max(nb, 1))
+            losses.append(float(tot) / max(nb, 1))
             if log:
                 log(f"epoch {ep} loss {losses[-1]:.4e}")
         return losses
 
-    def predict(self, x, lt, bf16=True):
+    def predict(self, x, lt, bf16=False):
         import torch
         npts = x.shape[0]
         y = torch.empty((npts, MODES), dtype=torch.float32, device=self.device)
@@ -197,13 +224,15 @@ class SpectralSurrogate:
         return y
 
     # -- evaluation with fallback ----------------------------------------------
-    def eval_with_fallback(self, vhs_op, f, Q, npts, threshold=0.05, bf16=True, stream=None):
+    def eval_with_fallback(self, vhs_op, f, Q, npts, threshold=0.05, threshold_hi=1e-3, bf16=False, stream=None):
         """Q for npts contiguous points: surrogate where the indicator is
-        <= threshold, the exact operator `vhs_op` (a VHSOperator of the same
-        grid) elsewhere.  Returns the number of fallback points."""
+        <= threshold (and the high-mode indicator <= threshold_hi when that is
+        > 0), the exact operator `vhs_op` (a VHSOperator of the same grid)
+        elsewhere.  Returns the number of fallback points."""
         nfb = C.c_int()
-        _lib.call("dgfs_sur_eval", self._h, vhs_op._h, _ptr(self.params), _ptr(f), _ptr(Q), int(npts),
-                  float(threshold), int(bool(bf16)), C.byref(nfb), stream if stream is not None else self._stream())
+        _lib.call("dgfs_sur_eval", self._h, vhs_op._h, _ptr(self.params_bf16), _ptr(f), _ptr(Q), int(npts),
+                  float(threshold), float(threshold_hi), int(bool(bf16)), C.byref(nfb),
+                  stream if stream is not None else self._stream())
         return nfb.value
 
     def close(self):

@@ -219,7 +219,8 @@ int dgfs_sur_backward(void* handle, const float* params, const float* dout, floa
 /* One Adam step (bias-corrected, step >= 1) over n parameters. */
 int dgfs_sur_adam(long n, float* params, const float* grad, float* m1, float* m2, float lr, float b1,
                   float b2, float eps, int step, void* stream);
-/* f[npts][Nv^3] -> mom[npts][8] (rho, ux, uy, uz, T, indicator, 0, 0),
+/* f[npts][Nv^3] -> mom[npts][8] (rho, ux, uy, uz, T, indicator ||f - M[f]|| / ||f||,
+ * high-mode indicator = share of f's norm in modes of total degree >= 4, 0),
  * x[npts][512] Hermite modes (a_000 stored as 0), lt[npts] = log T; x, lt may be null. */
 int dgfs_sur_encode(void* handle, const double* f, int npts, double* mom, float* x, float* lt,
                     void* stream);
@@ -232,10 +233,14 @@ int dgfs_sur_targets(void* handle, const double* Q, const double* mom, int npts,
 int dgfs_sur_decode(void* handle, const float* y, const double* mom, const int* idx, int nidx,
                     double* Q, int project, void* stream);
 /* Surrogate with fallback to the exact operator `vhs` (a dgfs_vhs handle of
- * the same grid) for points whose indicator exceeds `threshold`; synchronises
- * the stream once to read the indicators. */
+ * the same grid) for points whose indicator exceeds `threshold` or (when
+ * threshold_hi > 0) whose high-mode indicator exceeds `threshold_hi`;
+ * synchronises the stream once to read the indicators. */
 int dgfs_sur_eval(void* handle, void* vhs, const float* params, const double* f, double* Q, int npts,
-                  double threshold, int bf16, int* nfallback, void* stream);
+                  double threshold, double threshold_hi, int bf16, int* nfallback, void* stream);
+/* dst[i] = src[i] rounded to BF16 (nearest even), stored as FP32: the BF16
+ * weights the network runs with in its FP32-operand mode. */
+int dgfs_sur_round_bf16(long n, const float* src, float* dst, void* stream);
 /* C = alpha op(A) op(B) + beta C (row-major FP32 storage; bf16 as above);
  * the network's GEMM, exported for its tests. */
 int dgfs_nn_gemm(int transA, int transB, int M, int N, int K, float alpha, const float* A, long lda,

@@ -7,8 +7,8 @@ has the spectral operator, frfs/solvers/dgfs/scattering.py:227-286); what is
 restated here is the surrogate's own definition:
 
 * Hermite encoder: moments rho, u, T of f (DGFS Maxwellian convention of
-  frfs/solvers/dgfs/initcond.py:55-58, per-axis variance T/2), indicator
-  ||f - M[f]|| / ||f||, and the coefficients a_n of f / C (C the Maxwellian's
+  frfs/solvers/dgfs/initcond.py:55-58, per-axis variance T/2), indicators
+  ||f - M[f]|| / ||f|| and the share of the modes of total degree >= 4, and the coefficients a_n of f / C (C the Maxwellian's
   prefactor) in the grid-orthonormal basis spanned by M psi_n(c'),
   psi_n = prod He_{n_d}/sqrt(n_d!), n_d < 8, c' = (c - u)/sqrt(T/2) (the
   Maxwellian's own coefficient subtracted);
@@ -203,10 +203,12 @@ def encode(f, N, L):
     rho, u, T, ind, _ = moments(f, N, L)
     C = rho / (math.pi * T) ** 1.5
     a = hermite_project(f, N, L, u, T, C, 1.0).ravel()
+    deg = np.add.outer(np.add.outer(np.arange(D), np.arange(D)), np.arange(D)).ravel()
+    ind_hi = math.sqrt(np.sum(a[deg >= 4] ** 2) / np.sum(a ** 2))
     V = _vander(N, L, u, T)
     Li = _linv(V, (2.0 * L / N) / math.sqrt(0.5 * T))
     a[0] -= 1.0 / (Li[0][0, 0] * Li[1][0, 0] * Li[2][0, 0])   # the Maxwellian's own coefficient
-    return np.array([rho, *u, T, ind]), a, math.log(T)
+    return np.array([rho, *u, T, ind, ind_hi]), a, math.log(T)
 
 
 def targets(Q, mom, N, L, gamma, prefac):

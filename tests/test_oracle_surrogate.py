@@ -58,7 +58,7 @@ def test_encoder_of_maxwellian(N, L):
     # (in the basis span up to those moment errors) follow them
     tol = 1e-7 if N == 16 else 2e-12
     assert abs(mom[0] - rho) < tol and np.max(np.abs(mom[1:4] - u)) < tol and abs(mom[4] - T) < tol
-    assert mom[5] < tol and abs(lt - math.log(T)) < tol
+    assert mom[5] < tol and abs(lt - math.log(T)) < tol and mom[6] < 100 * tol
     assert np.max(np.abs(a)) < 100 * tol
 
 

@@ -69,15 +69,18 @@ def test_forward_backward_match_checker(device):
     dout = rng.normal(size=(P, S.MODES)).astype(np.float32)
     xt, ltt, dt = (torch.as_tensor(a, device=device) for a in (x, lt, dout))
     out = torch.empty((P, S.MODES), dtype=torch.float32, device=device)
-    # FP32 operands
+    # the BF16 weights the network runs with
+    w16 = sur.params_bf16.cpu().numpy().astype(np.float64)
+    assert np.array_equal(w16, S.bf16(sur.params.cpu().numpy()))
+    # FP32 activations
     sur.forward(xt, ltt, P, out, bf16=False)
     sur.backward(dt)
     torch.cuda.synchronize()
-    ref, cache = S.forward(sur.params.cpu().numpy().astype(np.float64), x, lt)
+    ref, cache = S.forward(w16, x, lt)
     e = rel(out.cpu().numpy(), ref)
     say("forward fp32", e)
     assert e < 1e-4
-    g_ref = S.backward(sur.params.cpu().numpy().astype(np.float64), cache, dout.astype(np.float64))
+    g_ref = S.backward(w16, cache, dout.astype(np.float64))
     g = sur.grad.cpu().numpy()
     blocks = {"P0": (S.OFF_P0, S.WIDTH), "P1": (S.OFF_P1, S.WIDTH), "E": (S.OFF_E, S.MODES * S.WIDTH)}
     for l in range(S.LAYERS):
@@ -97,7 +100,7 @@ def test_forward_backward_match_checker(device):
     # BF16 operands against the checker with the same rounding points
     sur.forward(xt, ltt, P, out, bf16=True)
     torch.cuda.synchronize()
-    ref16, _ = S.forward(sur.params.cpu().numpy().astype(np.float64), x, lt, round_bf16=True)
+    ref16, _ = S.forward(w16, x, lt, round_bf16=True)
     e16 = rel(out.cpu().numpy(), ref16)
     say("forward bf16", e16)
     assert e16 < 1e-2
@@ -147,7 +150,8 @@ def test_encode_targets_decode_match_checker(device):
     mom_h, x_h, lt_h, y_h, Qd_h = (t.cpu().numpy() for t in (mom, x, lt, y, Qd))
     for i in range(6):
         m_ref, a_ref, lt_ref = S.encode(f[i], N, L)
-        assert np.max(np.abs(mom_h[i, :6] - m_ref) / np.maximum(np.abs(m_ref), 1e-3)) < 1e-10
+        assert np.max(np.abs(mom_h[i, :6] - m_ref[:6]) / np.maximum(np.abs(m_ref[:6]), 1e-3)) < 1e-10
+        assert abs(mom_h[i, 6] - m_ref[6]) < 1e-9
         assert np.max(np.abs(x_h[i] - a_ref)) < 1e-6 * max(1.0, np.abs(a_ref).max())
         assert abs(lt_h[i] - lt_ref) < 1e-6
         t_ref = S.targets(f[i], m_ref, N, L, 0.38, 1.0)
@@ -219,13 +223,13 @@ def test_eval_with_fallback_routes_points(device):
     Qx = torch.empty_like(f)
     op.eval_contig(f.data_ptr(), Qx.data_ptr(), 40, st)
     Qa = torch.full_like(f, float("nan"))
-    nfb = sur.eval_with_fallback(op, f, Qa, 40, threshold=-1.0)
+    nfb = sur.eval_with_fallback(op, f, Qa, 40, threshold=-1.0, threshold_hi=0.0)
     torch.cuda.synchronize()
     assert nfb == 40
     assert float((Qa - Qx).abs().max()) == 0.0
     # all surrogate: equals encode -> network -> decode -> projection
     Qb = torch.full_like(f, float("nan"))
-    nfb = sur.eval_with_fallback(op, f, Qb, 40, threshold=1e9)
+    nfb = sur.eval_with_fallback(op, f, Qb, 40, threshold=1e9, threshold_hi=0.0)
     x, lt, y, mom = sur.make_dataset(f, Qx)
     yp = sur.predict(x, lt)
     Qc = torch.empty_like(f)
@@ -237,7 +241,7 @@ def test_eval_with_fallback_routes_points(device):
     ind = mom[:, 5].cpu().numpy()
     thr = float(np.median(ind))
     Qm = torch.full_like(f, float("nan"))
-    nfb = sur.eval_with_fallback(op, f, Qm, 40, threshold=thr)
+    nfb = sur.eval_with_fallback(op, f, Qm, 40, threshold=thr, threshold_hi=0.0)
     torch.cuda.synchronize()
     hi = ind > thr
     assert nfb == int(hi.sum())
