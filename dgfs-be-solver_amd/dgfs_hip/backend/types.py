@@ -76,7 +76,7 @@ class MatrixBase:
 
     def get(self):
         # the solver may run on its own stream: wait for all of the device's work
-        self.backend.synchronize()
+        self._sync()
         host = self.data.detach().cpu().numpy()[:, : self.ncol]
         return self._unpack(np.ascontiguousarray(host))
 
@@ -86,7 +86,15 @@ class MatrixBase:
         buf = np.zeros((self.nrow, self.leaddim), dtype=self.dtype)
         buf[:, : self.ncol] = packed
         self.data.copy_(torch.from_numpy(buf))
-        self.backend.synchronize()
+        self._sync()
+
+    def _sync(self):
+        sync = getattr(self.backend, "synchronize", None)
+        if sync is not None:
+            sync()
+        else:
+            import torch
+            torch.cuda.synchronize(self.data.device)
 
 
 class Matrix(MatrixBase):
