@@ -11,8 +11,11 @@
 
 namespace dgfs {
 
+// naturally aligned (2 * sizeof(T)) so that LDS and global accesses of one
+// element are single 64-bit (FP32) or 128-bit (FP64) operations with wide
+// immediate offsets
 template <typename T>
-struct cpx {
+struct alignas(2 * sizeof(T)) cpx {
   T x, y;
 };
 

@@ -71,6 +71,18 @@ __device__ __forceinline__ void pencil_pass(Load load, Epi epi) {
   __syncthreads();
 }
 
+// Consecutive workgroups of a launch go round-robin to the 8 XCDs of an
+// MI355X, each XCD with its own L2.  xcd_slot() renumbers block b so that the
+// blocks of one XCD hold consecutive slots (persistent grids then give the
+// work items of one point to one XCD, and its spectrum stays in that L2).  It
+// is a bijection on [0, gridDim.x) whatever the hardware placement: only
+// locality depends on it, never the result.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_slot() {
+  const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  return (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+}
+
 template <int N>
 struct Shape {
   static constexpr int NT = (N >= 16) ? 128 : 64;
@@ -85,7 +97,7 @@ template <typename T>
 struct PrepArgs {
   const double* f;  // [npts][n] contiguous
   cpx<T>* F;        // [npts][n] natural spectral order, scaled by 1/n
-  cpx<T>* Ft;       // optional copy transposed to [kx][kz][ky] (register-resident gain)
+  cpx<T>* Ft;       // optional (1 + i) F transposed to [kx][kz][ky] (register-resident gain)
   cpx<T>* Z;        // scratch [nslots][n]
   int npts;
 };
@@ -112,7 +124,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_prep(PrepArgs<T> a) {
           lds, cnt, [&](int j, int r, int c) { return Z[(kx0 + j) * NN + r * N + c]; },
           [&](int j, int r, int c, cpx<T> v) {
             F[(kx0 + j) * NN + r * N + c] = rn * v;
-            if (Ft) Ft[(kx0 + j) * NN + c * N + r] = rn * v;
+            if (Ft) Ft[(kx0 + j) * NN + c * N + r] = rn * cpx<T>{v.x - v.y, v.x + v.y};  // (1 + i) F
           });
     }
   }

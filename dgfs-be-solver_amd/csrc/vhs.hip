@@ -201,59 +201,104 @@ __global__ __launch_bounds__(Shape<NA>::NT) void k_vhs_gain_pair(GainArgs<T> a, 
 // ---------------------------------------------------------------------------
 // Nyquist-plane corrections for every (point, p, q): W = IFFT(B) on the three
 // planes kx = N/2, ky = N/2 (kx != N/2), kz = N/2 (kx, ky != N/2), written to
-// Wbuf[pt][p*M + q][3][N][N].  Consumed by the register-resident gain kernel.
+// Wbuf[pt][p*M + q][3][N][N] (plane j indexed [r][c]: W1[y][z], W2[x][z],
+// W3[x][y]).  Consumed by the register-resident gain kernel.
+//
+// With S the set of axes whose index is N/2 at a spectral point, E_S the
+// product of their phase factors and e = a + ib the product of the others,
+// the anti-Hermitian part 0.5 F (conj D - i D), D = E - E~, reduces to
+//   B = (1 - i) F Im(E_S) (a - b),
+// a real weight times F.  One wave transforms 64/N planes, N lanes each:
+// lane c loads column c of its plane (coalesced; the kz = N/2 plane comes
+// from the transposed spectrum), transforms along the rows, and the plane
+// goes through a swizzled LDS tile (element (r, c) at r*N + (c ^ r), conflict
+// free both ways) for the column transform and back for coalesced stores.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct NyqArgs {
   const cpx<T>* F;      // [npts][n]
+  const cpx<T>* Ft;     // [npts][n] (1 + i) F transposed to [kx][kz][ky]
   cpx<T>* W;            // [npts][K][3][N][N]
-  const double* alpha;  // [Nrho]
-  const double* sz;     // [M][3]
-  int npts, Nrho, M;
+  const cpx<T>* ephtab; // [K][3][N] phase factors
+  long nplanes;         // npts * K * 3
+  int Nrho, M;
+};
+
+template <int N>
+struct NyqShape {
+  static constexpr int PW = 64 / N;        // planes per wave
+  static constexpr int PSTR = N * N + 16;  // LDS plane stride (planes on distinct banks)
 };
 
 template <typename T, int N>
-__global__ __launch_bounds__(Shape<N>::NT) void k_vhs_nyq(NyqArgs<T> a) {
-  constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
-  __shared__ cpx<T> lds[PB * N * (N + 1)];
-  __shared__ cpx<T> eph[3][N];
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
+  constexpr int PW = NyqShape<N>::PW, PSTR = NyqShape<N>::PSTR, NN = N * N, n = NN * N, h = N / 2;
+  static_assert(PW * N == 64, "one wave per PW planes");
+  __shared__ cpx<T> buf[PW * PSTR];
+  __shared__ cpx<T> erow[PW][N];  // row-axis factor, 1 at index N/2
+  __shared__ cpx<T> enyq[PW][2];  // row- and column-axis factors at index N/2
+  const int lane = threadIdx.x, w = lane / N, c = lane - w * N;
   const int K = a.Nrho * a.M;
-  const long nitems = static_cast<long>(a.npts) * K;
-  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+  cpx<T>* P = buf + w * PSTR;
+  for (long base = static_cast<long>(blockIdx.x) * PW; base < a.nplanes;
+       base += static_cast<long>(gridDim.x) * PW) {
+    const long pl = base + w;
+    const bool active = pl < a.nplanes;
+    const long item = active ? pl / 3 : 0;
+    const int j = active ? static_cast<int>(pl - item * 3) : 0;
     const int pt = static_cast<int>(item / K), pq = static_cast<int>(item - static_cast<long>(pt) * K);
-    const int p = pq / a.M, q = pq - p * a.M;
-    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
-    cpx<T>* W = a.W + static_cast<size_t>(item) * 3 * NN;
-    __syncthreads();
-    if (tid < 3 * N) {
-      const int d = tid / N, k = tid - d * N;
-      const double l = (k < h) ? static_cast<double>(k) : static_cast<double>(k - N);
-      double s, c;
-      sincos(a.alpha[p] * a.sz[q * 3 + d] * l, &s, &c);
-      eph[d][k] = cpx<T>{static_cast<T>(c), static_cast<T>(s)};
+    const int ra = (j == 0) ? 1 : 0;  // row axis
+    const int cb = (j == 2) ? 1 : 2;  // column axis
+    const cpx<T>* E = a.ephtab + static_cast<size_t>(pq) * 3 * N;
+    const cpx<T> er = E[ra * N + c], ec = E[cb * N + c], en = E[j * N + h];
+    erow[w][c] = (c == h) ? cpx<T>{T(1), T(0)} : er;
+    if (c == h) {
+      enyq[w][0] = er;
+      enyq[w][1] = ec;
     }
     __syncthreads();
-    slab_batch<T, N, NT, +1>(
-        lds, 3,
-        [&](int j, int r, int c) -> cpx<T> {
-          int kx, ky, kz;
-          if (j == 0) {
-            kx = h; ky = r; kz = c;
-          } else if (j == 1) {
-            kx = r; ky = h; kz = c;
-            if (kx == h) return cpx<T>{T(0), T(0)};
-          } else {
-            kx = r; ky = c; kz = h;
-            if (kx == h || ky == h) return cpx<T>{T(0), T(0)};
-          }
-          const cpx<T> ex = eph[0][kx], ey = eph[1][ky], ez = eph[2][kz];
-          const cpx<T> E = ex * ey * ez;
-          const cpx<T> Et = (kx == h ? conj(ex) : ex) * (ky == h ? conj(ey) : ey) * (kz == h ? conj(ez) : ez);
-          const cpx<T> D = E - Et;
-          return T(0.5) * (F[kx * NN + ky * N + kz] * (conj(D) - mul_i(D)));
-        },
-        [&](int j, int r, int c, cpx<T> v) { W[j * NN + r * N + c] = v; });
+    // Im(E_S) for the four Nyquist patterns (row N/2?, column N/2?)
+    const cpx<T> ya = enyq[w][0], yb = enyq[w][1];
+    const T s00 = en.y, s10 = (en * ya).y, s01 = (en * yb).y, s11 = (en * ya * yb).y;
+    const bool ch = (c == h);
+    T sc0 = ch ? s01 : s00;  // rows != N/2
+    T sc1 = ch ? s11 : s10;  // row N/2
+    if (j >= 1) sc1 = T(0);            // row kx = N/2 belongs to plane 0
+    if (j == 2 && ch) sc0 = T(0);      // column ky = N/2 belongs to plane 1
+    // column factor b' (1 at N/2): m(r, c) = Re(a' b') - Im(a' b') = ar u - ai v
+    const T br = ch ? T(1) : ec.x, bi = ch ? T(0) : ec.y;
+    const T u = br - bi, v = bi + br;
+    const cpx<T>* src = (j == 2 ? a.Ft : a.F) + static_cast<size_t>(pt) * n + (j == 0 ? h * NN : h * N) + c;
+    const int stride = (j == 0) ? N : NN;
+    cpx<T> x[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      const cpx<T> e = erow[w][r];
+      const T m = (r == h ? sc1 : sc0) * (e.x * u - e.y * v);
+      cpx<T> fv = active ? src[r * stride] : cpx<T>{T(0), T(0)};
+      if (j == 2) fv = cpx<T>{T(0.5) * (fv.x + fv.y), T(0.5) * (fv.y - fv.x)};  // undo the (1 + i)
+      x[r] = m * fv;
+    }
+    FFT<T, N, +1>::run(x);  // along the rows
+#pragma unroll
+    for (int r = 0; r < N; ++r) P[r * N + (c ^ r)] = x[r];
+    __syncthreads();
+    // lane c now owns row c of the plane
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = P[c * N + (k ^ c)];
+    FFT<T, N, +1>::run(x);  // along the columns
+#pragma unroll
+    for (int k = 0; k < N; ++k) P[c * N + (k ^ c)] = x[k];
+    __syncthreads();
+    if (active) {
+      cpx<T>* Wp = a.W + static_cast<size_t>(item) * 3 * NN + j * NN + c;
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        const cpx<T> y = P[r * N + (c ^ r)];
+        Wp[r * N] = cpx<T>{y.x + y.y, y.y - y.x};  // times (1 - i)
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -273,13 +318,14 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_nyq(NyqArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct GainRRArgs {
-  const cpx<T>* Ft;     // [npts][n] spectrum transposed to [kx][kz][ky]
+  const cpx<T>* Ft;     // [npts][n] (1 + i) F transposed to [kx][kz][ky]
   const cpx<T>* W;      // [npts][K][3][N][N]
   cpx<T>* Ghat;         // [npts*ngroups][n]
   const T* b1;          // [Nrho][n]
   const double* alpha;  // [Nrho]
   const double* wgt;    // [Nrho]
   const double* sz;     // [M][3]
+  const cpx<T>* ephtab; // [K][3][N] phase factors
   int nitems, ngroups, ppg, Nrho, M;
 };
 
@@ -312,16 +358,30 @@ struct RRLds {
 template <typename T, int N, int G>
 __device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Ft, cpx<T>* Xb,
                                               const cpx<T> (*eph)[N], int kx, int g, int l) {
+  // eph[0..2]: phase factors of the x, y, z axes; eph[3][k] = (Re + Im, Re - Im) of eph[2][k]
   constexpr int NP = N + 1, NN = N * N, GH = G / 2;
-  // z-pencil (kx, ky = l) from the transposed spectrum Ft[kx][kz][ky]
-  const cpx<T>* Fp = Ft + kx * NN + l;
+  // z-pencil (kx, ky = l) from the transposed spectrum Ft[kx][kz][ky].  The
+  // loads of a pencil use at most two base addresses and immediate offsets
+  // (4 KB reach): the second base is made opaque so the compiler does not
+  // precompute one 64-bit address per kz outside the (p, q) loop.
+  constexpr int KB = (4096 / static_cast<int>(N * sizeof(cpx<T>))) < N ? (4096 / static_cast<int>(N * sizeof(cpx<T>))) : N;
+  const cpx<T>* P0 = Ft + kx * NN + l;
+  int second = KB * N;
+  asm volatile("" : "+v"(second));
+  const cpx<T>* P1 = P0 + second;
+#pragma unroll
+  for (int kz = 0; kz < N; ++kz) v[kz] = (kz < KB) ? P0[kz * N] : P1[(kz - KB) * N];
+  // F (E + i conj E) = (Re E + Im E) (1 + i) F with E = exy ez(kz); the
+  // spectrum already carries the (1 + i), and Re E + Im E = A.x (B.x + B.y)
+  // + A.y (B.x - B.y) with A = exy, B = ez, the pair taken from eph[3]
   const cpx<T> exy = eph[0][kx] * eph[1][l];
 #pragma unroll
   for (int k0 = 0; k0 < N; k0 += 8) {
 #pragma unroll
     for (int kz = k0; kz < k0 + 8; ++kz) {
-      const cpx<T> E = exy * eph[2][kz];
-      v[kz] = Fp[kz * N] * (E + mul_i(conj(E)));
+      const cpx<T> pm = eph[3][kz];
+      const T s = exy.x * pm.x + exy.y * pm.y;
+      v[kz] = s * v[kz];
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -477,11 +537,11 @@ __device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx
 template <typename T, int N, bool INL>
 __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a) {
   constexpr int NT = RRShape<N>::NT, G = RRShape<N>::G, NN = N * N, n = N * N * N, h = N / 2;
-  extern __shared__ char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* Xb = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* Wl = Xb + RRLds<N>::xsize;
   T* S0 = reinterpret_cast<T*>(Wl + RRLds<N>::wl) + threadIdx.x;  // S0[x * NT]
-  __shared__ cpx<T> eph[3][N];
+  __shared__ cpx<T> eph[4][N];  // x, y, z phase factors and the z pair (Re + Im, Re - Im)
   const int tid = threadIdx.x;
   const int g = tid / N, l = tid - (tid / N) * N;
   const int K = a.Nrho * a.M;
@@ -489,7 +549,7 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
   cpx<T> v0[N], v1[N];
   T s1[N];
 
-  for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
     const int pt = item / a.ngroups, grp = item - pt * a.ngroups;
     const int p0 = grp * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
     const cpx<T>* Ft = a.Ft + static_cast<size_t>(pt) * n;
@@ -500,15 +560,12 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
         s1[x] = T(0);
         S0[x * NT] = T(0);
       }
-      const double alpha = a.alpha[p];
       for (int q = 0; q < a.M; ++q) {
         __syncthreads();
         if (tid < 3 * N) {
-          const int d = tid / N, k = tid - d * N;
-          const double lv = (k < h) ? static_cast<double>(k) : static_cast<double>(k - N);
-          double sn, cs;
-          sincos(alpha * a.sz[q * 3 + d] * lv, &sn, &cs);
-          eph[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
+          const cpx<T> e = a.ephtab[(static_cast<size_t>(p) * a.M + q) * 3 * N + tid];
+          (&eph[0][0])[tid] = e;
+          if (tid >= 2 * N) eph[3][tid - 2 * N] = cpx<T>{e.x + e.y, e.x - e.y};
         }
         if constexpr (INL) {
           // B = (F/2)(conj D - i D) on the planes kx = h, ky = h (kx != h),
@@ -533,7 +590,9 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
               const cpx<T> E = ex * ey * ez;
               const cpx<T> Et = (kx == h ? conj(ex) : ex) * (ky == h ? conj(ey) : ey) * (kz == h ? conj(ez) : ez);
               const cpx<T> D = E - Et;
-              v = T(0.5) * (Ft[kx * NN + kz * N + ky] * (conj(D) - mul_i(D)));
+              const cpx<T> fr = Ft[kx * NN + kz * N + ky];  // (1 + i) F
+              const cpx<T> f0{T(0.5) * (fr.x + fr.y), T(0.5) * (fr.y - fr.x)};
+              v = T(0.5) * (f0 * (conj(D) - mul_i(D)));
             }
             Wl[i] = v;
           }
@@ -562,7 +621,12 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
         } else {
           // this (p, q)'s Nyquist planes into LDS (read after later barriers)
           const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
-          for (int i = tid; i < 3 * NN; i += NT) Wl[i] = Wq[i];
+          static_assert((3 * NN) % NT == 0, "Nyquist planes split evenly over the workgroup");
+          cpx<T> wv[3 * NN / NT];
+#pragma unroll
+          for (int k = 0; k < 3 * NN / NT; ++k) wv[k] = Wq[tid + k * NT];
+#pragma unroll
+          for (int k = 0; k < 3 * NN / NT; ++k) Wl[tid + k * NT] = wv[k];
           __syncthreads();
         }
         rr_zy_inverse<T, N, G>(v0, Ft, Xb, eph, g, g, l);
@@ -812,6 +876,7 @@ struct VhsHandle {
   double* alpha = nullptr;
   double* wgt = nullptr;
   double* szd = nullptr;
+  void* ephtab = nullptr;  // [K][3][N] cpx<T>: exp(i alpha_p sz_qd l(k)), per (p, q), axis d, index k
   double Ainv[25] = {};
   // workspace
   size_t cap_pts = 0, cap_items = 0;
@@ -925,12 +990,13 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
       h->cap_wny = wbytes;
     }
     if (!inl) {
-    NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Wny), h->alpha, h->szd, npts,
-                  h->Nrho, h->M};
-    const long nyq_items = static_cast<long>(npts) * K;
+    const long nplanes = static_cast<long>(npts) * K * 3;
+    NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<const cpx<T>*>(h->Ft),
+                  static_cast<cpx<T>*>(h->Wny), static_cast<const cpx<T>*>(h->ephtab), nplanes, h->Nrho, h->M};
+    const long waves = (nplanes + NyqShape<N>::PW - 1) / NyqShape<N>::PW;
     KTimerT<VhsHandle> t4(h, 4, st);
-    hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(nyq_items, 8L * cu_count()))),
-                       dim3(Shape<N>::NT), 0, st, na);
+    hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(waves, 16L * cu_count()))),
+                       dim3(64), 0, st, na);
     DGFS_LAUNCH_CHECK("vhs_nyq");
     t4.stop();
     }
@@ -944,7 +1010,7 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
     const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cu_count())));
     GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
                      static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
-                     static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+                     static_cast<const cpx<T>*>(h->ephtab), static_cast<int>(items), groups, ppg, h->Nrho, h->M};
     if (inl)
       hipLaunchKernelGGL((k_vhs_gain_rr<T, N, true>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
     else
@@ -1253,6 +1319,22 @@ extern "C" int dgfs_vhs_create(const dgfs_vhs_desc* d, void** out) {
   DGFS_HIP_TRY(hipMemcpy(h->alpha, alpha.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->wgt, wgt.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->szd, d->sz, h->M * 3 * sizeof(double), hipMemcpyHostToDevice));
+  {
+    // separable phase factors of every (p, q), evaluated in FP64 on the host
+    std::vector<double> tab(static_cast<size_t>(h->Nrho) * h->M * 3 * N * 2);
+    for (int p = 0; p < h->Nrho; ++p)
+      for (int q = 0; q < h->M; ++q)
+        for (int dd = 0; dd < 3; ++dd)
+          for (int k = 0; k < N; ++k) {
+            const double l = (k < N / 2) ? k : k - N;
+            const double ang = alpha[p] * d->sz[q * 3 + dd] * l;
+            const size_t o = ((static_cast<size_t>(p) * h->M + q) * 3 * N + dd * N + k) * 2;
+            tab[o] = std::cos(ang);
+            tab[o + 1] = std::sin(ang);
+          }
+    rc = upload(&h->ephtab, tab);
+    if (rc) return rc;
+  }
 
   // (C C^T)^{-1} / cw^2 for the projection: A_jk = sum_r phi_j phi_k cw
   double A[25] = {};
@@ -1278,6 +1360,7 @@ extern "C" int dgfs_vhs_destroy(void* p) {
   free_ws(h);
   (void)hipFree(h->b1); (void)hipFree(h->b2); (void)hipFree(h->alpha); (void)hipFree(h->wgt);
   (void)hipFree(h->szd);
+  (void)hipFree(h->ephtab);
   h->magic = 0;
   delete h;
   return DGFS_OK;
