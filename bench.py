@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--algorithm", type=int, default=0,
                     help="gain kernel of the FP32 path: 0 automatic, 1 multi-pass, 2 register-resident, "
                          "3 register-resident with in-kernel Nyquist planes")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="radial groups per point of the FP32 gain launch (0: the library's choice)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -433,6 +435,8 @@ def main():
     op32 = VHSOperator.from_mesh(vm, gamma, 1.0, precision=32, project=True)
     if args.algorithm:
         op32.set_algorithm(args.algorithm)
+    if args.groups:
+        op32.set_groups(args.groups)
     for _ in range(args.warmup):
         op32.eval_contig(f.data_ptr(), Q32.data_ptr(), P, stream)
     torch.cuda.synchronize()
