@@ -67,3 +67,60 @@ def cavity_cfg(Nv=32, L=6.0, Nrho=16, M=12, Kn=0.05, omega=0.81, order=3, ulid=0
         maxwellian_section(cfg, f"soln-bcs-{name}", 1.0, 1.0, (0, 0, 0), "dgfs-wall-diffuse")
     maxwellian_section(cfg, "soln-bcs-top", 1.0, 1.0, (ulid, 0, 0), "dgfs-wall-diffuse")
     return cfg
+
+
+def bi_couette_cfg(Nv=16, L=5.0, Nrho=4, M=12, Mpre=24, masses=(1.0, 2.1), nden=(0.5, 0.5),
+                   omega=0.5, alpha=1.0, prefactor=None, dRef=1.0, Tref=1.0, order=2, uwall=0.3,
+                   NvBatchSize=None, precision="double", project=None):
+    """Two-species (or more) planar Couette flow: x-periodic channel between
+    diffuse walls at T = 1 moving at +-uwall, initial species Maxwellians at
+    rest with number densities nden (in n0); VSS scattering with the same
+    omega and alpha for every pair (keys of frfs/solvers/dgfsbi/*).  With
+    `prefactor` set, every pair uses that kernel prefactor directly."""
+    cfg = Inifile()
+    for k, v in dict(H0=1.0, T0=1.0, rho0=1.0, molarMass0=1.0).items():
+        cfg.set("non-dim", k, v)
+    cfg.set("constants", "Nv", Nv)
+    cfg.set("constants", "NvBatchSize", NvBatchSize or Nv ** 3)
+    cfg.set("constants", "Nrho", Nrho)
+    cfg.set("velocity-mesh", "cmax", 0.0)
+    cfg.set("velocity-mesh", "Tmax", 1.0)
+    cfg.set("velocity-mesh", "dev", L)
+    for p, m in enumerate(masses):
+        cfg.set("velocity-mesh", f"molarMass{p + 1}", m)
+    cfg.set("spherical-design-rule", "ssrule", "womersley")
+    cfg.set("spherical-design-rule", "M", M)
+    cfg.set("scattering-model", "type", "vss-gll")
+    cfg.set("scattering-model", "Mpre", Mpre)
+    cfg.set("scattering-model", "precision", precision)
+    if project is not None:
+        cfg.set("scattering-model", "project", "true" if project else "false")
+    S = len(masses)
+    for p in range(S):
+        for q in range(S):
+            pq = f"{p}{q}"
+            cfg.set("scattering-model", "alpha" + pq, alpha)
+            cfg.set("scattering-model", "omega" + pq, omega)
+            if prefactor is not None:
+                cfg.set("scattering-model", "prefactor" + pq, prefactor)
+            else:
+                cfg.set("scattering-model", "dRef" + pq, dRef)
+                cfg.set("scattering-model", "Tref" + pq, Tref)
+    cfg.set("solver", "system", "dgfsbi")
+    cfg.set("solver", "order", order)
+    cfg.set("solver-interfaces", "riemann-solver", "rusanov")
+    cfg.set("solver-interfaces-line", "flux-pts", "gauss-legendre")
+    cfg.set("solver-elements-quad", "soln-pts", "gauss-legendre")
+    cfg.set("solver-elements-tri", "soln-pts", "williams-shunn")
+    n0 = 1.0 / 1.0 * 6.0221409e+23
+    cfg.set("soln-ics", "type", "maxwellian")
+    for p, nd in enumerate(nden):
+        cfg.set("soln-ics", f"nden{p + 1}", nd * n0)
+    for sect, u in (("soln-ics", 0.0), ("soln-bcs-bottom", -uwall), ("soln-bcs-top", uwall)):
+        cfg.set(sect, "T", 1.0)
+        cfg.set(sect, "ux", u * U0)
+        cfg.set(sect, "uy", 0.0)
+        cfg.set(sect, "uz", 0.0)
+    cfg.set("soln-bcs-bottom", "type", "dgfsbi-wall-diffuse")
+    cfg.set("soln-bcs-top", "type", "dgfsbi-wall-diffuse")
+    return cfg

@@ -138,7 +138,113 @@ class StdTVDRK2DGFSStepper(BaseDGFSStepper):
         return r0
 
 
-STEPPERS = {c.stepper_name: c for c in (StdEulerDGFSStepper, StdTVDRK2DGFSStepper)}
+class BaseDGFSBiStepper(BaseDGFSStepper):
+    """Multi-species steppers (steppers.py:229-476): registers are banked
+    species-major, nregs_orig per species; the solution of every species
+    stays in its register 0 (the reference raises otherwise)."""
+    _stepper_nregs_orig = None
+
+    def __init__(self, backend, system, dt, tstart=0.0):
+        self._stepper_nregs = self._stepper_nregs_orig * system._nspcs
+        super().__init__(backend, system, dt, tstart)
+
+    def _check_current(self):
+        if self._idxcurr != 0:
+            raise ValueError("Something wrong")
+
+    def _collide(self, t, rin, rout):
+        n0 = self._stepper_nregs_orig
+        self.system.collide(t, list(self._regidx[rin::n0]) + list(self._regidx[rout::n0]))
+
+
+class StdEulerDGFSBiStepper(BaseDGFSBiStepper):
+    stepper_name = "dgfsbi-euler"
+    _stepper_nregs_orig = 2
+    _stepper_order = 1
+
+    def step(self, t, dt):
+        add, rhs = self._add_dgfs, self.system.rhs
+        updatebc = self.system.updatebc
+        copyto, copyfrm = self._copy_to_reg_dgfs, self._copy_from_reg_dgfs
+        vm = self.system.vm
+        NvBS, n0 = vm.NvBatchSize(), self._stepper_nregs_orig
+        self._check_current()
+        self._collide(t, 0, 1)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0, r0, sidx, eidx - 1, 0, NvBS - 1)
+                updatebc(t, str(p), r0, r0 + 1, batch, sidx)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0, r0, sidx, eidx - 1, 0, NvBS - 1)
+                rhs(t, str(p), r0, r0 + 1, batch, sidx)
+                add(1.0, r0, dt, r0 + 1)
+                copyto(r0 + 1, r0 + 1, sidx, eidx - 1, 0, NvBS - 1)
+                add(1.0, r0, dt, r0 + 1)
+                copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
+        self.nfevals += 1
+        return 0
+
+
+class StdTVDRK2DGFSBiStepper(BaseDGFSBiStepper):
+    stepper_name = "dgfsbi-tvd-rk2"
+    _stepper_nregs_orig = 3
+    _stepper_order = 2
+
+    def step(self, t, dt):
+        add, rhs = self._add_dgfs, self.system.rhs
+        updatebc = self.system.updatebc
+        copyto, copyfrm = self._copy_to_reg_dgfs, self._copy_from_reg_dgfs
+        vm = self.system.vm
+        NvBS, n0 = vm.NvBatchSize(), self._stepper_nregs_orig
+        self._check_current()
+        # stage 1: r1 = r0 + dt (adv(r0) + Q(r0)) for every species
+        self._collide(t, 0, 1)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0, r0, sidx, eidx - 1, 0, NvBS - 1)
+                updatebc(t, str(p), r0, r0 + 1, batch, sidx)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0, r0, sidx, eidx - 1, 0, NvBS - 1)
+                rhs(t, str(p), r0, r0 + 1, batch, sidx)
+                add(1.0, r0, dt, r0 + 1)
+                copyto(r0 + 1, r0 + 1, sidx, eidx - 1, 0, NvBS - 1)
+                add(dt, r0 + 1, 1.0, r0)
+                copyfrm(r0 + 1, r0 + 1, 0, NvBS - 1, sidx, eidx - 1)
+        # stage 2: r0 = 0.5 r0 + 0.5 r1 + 0.5 dt (adv(r1) + Q(r1))
+        self._collide(t, 1, 2)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0 + 1, r0 + 1, sidx, eidx - 1, 0, NvBS - 1)
+                updatebc(t, str(p), r0 + 1, r0 + 2, batch, sidx)
+        for p in range(vm.nspcs()):
+            r0 = n0 * p
+            for batch in range(vm.NvBatches()):
+                sidx, eidx = batch * NvBS, (batch + 1) * NvBS
+                copyto(r0 + 2, r0 + 2, sidx, eidx - 1, 0, NvBS - 1)
+                copyto(r0, r0, sidx, eidx - 1, 0, NvBS - 1)
+                add(0.5, r0, 0.5 * dt, r0 + 2)
+                copyto(r0 + 1, r0 + 1, sidx, eidx - 1, 0, NvBS - 1)
+                rhs(t + dt, str(p), r0 + 1, r0 + 2, batch, sidx)
+                add(1.0, r0, 0.5, r0 + 1, 0.5 * dt, r0 + 2)
+                copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
+        self.nfevals += 2
+        return 0
+
+
+STEPPERS = {c.stepper_name: c for c in (StdEulerDGFSStepper, StdTVDRK2DGFSStepper,
+                                        StdEulerDGFSBiStepper, StdTVDRK2DGFSBiStepper)}
 
 
 def get_stepper(name, backend, system, dt, tstart=0.0):

@@ -1,0 +1,137 @@
+"""Multi-species DGFS system on the HIP path (mirrors
+frfs/solvers/dgfsbi/system.py:17-264).
+
+Every species lives on the shared velocity grid and is advected by the same
+FR operators; only the boundary distributions (per species) and the
+collision operator (Q_p = sum_q Q_pq, one batched VSS call for all species)
+couple them.  Registers are banked species-major as in the reference
+(system.py:135-154): with nregs0 registers per species, register r of
+species p is bank index p * nregs0 + r; the element batch registers follow
+the same numbering.  rhs/updatebc take the species index `idx` (a string in
+the reference's steppers) to pick that species' boundary kernels.
+"""
+import numpy as np
+
+from dgfs_hip.shapes import get_shape
+from dgfs_hip.solvers.dgfs.elements import DGFSElements
+from dgfs_hip.solvers.dgfs.inters import DGFSIntInters
+from dgfs_hip.solvers.dgfsbi.initcond import DGFSBiInitCondition
+from dgfs_hip.solvers.dgfsbi.inters import DGFSBiBCInters
+from dgfs_hip.solvers.dgfsbi.scattering import DGFSBiScatteringModel
+from dgfs_hip.solvers.dgfsbi.velocitymesh import DGFSBiVelocityMesh
+from dgfs_hip.util import subclass_where
+
+
+def _count_species(cfg):
+    n = 0
+    while cfg.hasopt("velocity-mesh", f"molarMass{n + 1}"):
+        n += 1
+    return n
+
+
+class DGFSBiSystem:
+    name = "dgfsbi"
+    elementscls = DGFSElements
+    velocitymeshcls = DGFSBiVelocityMesh
+    scatteringcls = DGFSBiScatteringModel
+
+    def __init__(self, backend, mesh, cfg, nregs, initsoln=None):
+        self.backend = be = backend
+        self.cfg = cfg
+        self.mesh = mesh
+        nspcs = _count_species(cfg)
+        if nspcs < 2:
+            raise ValueError("the multi-species system needs molarMass1, molarMass2, ... in [velocity-mesh]")
+        if nregs % nspcs:
+            raise ValueError("Should be multiple of nspcs")
+        self._nspcs = nspcs
+        self.nregs0 = nregs // nspcs
+        self.vm = vm = self.velocitymeshcls(backend, cfg, nspcs)
+        order = cfg.getint("solver", "order")
+        et = mesh.eletype
+        upts_rule = cfg.get(f"solver-elements-{et}", "soln-pts",
+                            "gauss-legendre" if et == "quad" else "williams-shunn")
+        fpts_rule = cfg.get("solver-interfaces-line", "flux-pts", "gauss-legendre")
+        shape = get_shape(et, order, upts_rule, fpts_rule)
+        self.nvars = vm.NvBatchSize()
+        self.ele = self.elementscls(be, shape, mesh.nodes, vm, self.nvars, nregs)
+        self.ele_types = [et]
+        self.ndims = self.ele.ndims
+        self.iint = DGFSIntInters(be, self.ele, mesh.interfaces)
+        self.bcints = []
+        for bname, faces in mesh.boundaries.items():
+            sect = f"soln-bcs-{bname}"
+            cls = subclass_where(DGFSBiBCInters, type=cfg.get(sect, "type"))
+            self.bcints.append(cls(be, self.ele, faces, cfg, sect, vm))
+
+        djac = self.ele.djac_at_np("upts")
+        self.unit_cfl_dt = float(np.sqrt(np.min(djac)) / vm.L() / self.ndims)
+
+        smn = cfg.get("scattering-model", "type")
+        self.sm = subclass_where(self.scatteringcls, scattering_model=smn)(be, cfg, vm)
+
+        nupts, neles, vsize = self.ele.nupts, self.ele.neles, vm.vsize()
+        if initsoln is not None:
+            # (nupts, nspcs * vsize, neles), species-major as the reference's
+            # solution files (system.py:116-125)
+            data = np.asarray(initsoln, dtype=float).reshape(nupts, nspcs * vsize, neles)
+            f0 = [data[:, p * vsize:(p + 1) * vsize, :] for p in range(nspcs)]
+        else:
+            icn = cfg.get("soln-ics", "type")
+            ic = subclass_where(DGFSBiInitCondition, model=icn)(be, cfg, vm, "soln-ics")
+            f0 = []
+            for p in range(nspcs):
+                a = np.empty((nupts, vsize, neles))
+                ic.apply_init_vals(p, a, self.ele)
+                f0.append(a)
+        self.eles_scal_upts_full = [be.matrix((nupts, vsize, neles), f0[p], tags={"align"})
+                                    for p in range(nspcs) for _ in range(self.nregs0)]
+        self.eles_scal_upts_inb_full = be.matrix_bank(self.eles_scal_upts_full)
+
+    def rhs(self, t, idx, uinbank, foutbank, batch, varidx):
+        """Advection of one velocity batch of species idx (system.py:160-200)."""
+        p = int(idx)
+        ele = self.ele
+        ele.disu(uinbank)
+        ele.tdisf(uinbank, varidx)
+        ele.tdivtpcorf(foutbank)
+        self.iint.comm_flux(varidx)
+        for b in self.bcints:
+            b.comm_flux(t, p, varidx)
+        ele.tdivtconf(foutbank)
+        ele.negdivconf(foutbank)
+
+    def updatebc(self, t, idx, uinbank, foutbank, batch, varidx):
+        """Boundary accumulations of species idx over one batch (system.py:203-220)."""
+        p = int(idx)
+        self.ele.disu(uinbank)
+        for b in self.bcints:
+            b.update_bc(t, p, varidx)
+
+    def collide(self, t, regids):
+        """regids = (in_0, ..., in_{S-1}, out_0, ..., out_{S-1}) full-register
+        bank indices (system.py:223-235): Q_p = sum_q Q_pq at every point."""
+        S = self._nspcs
+        if len(regids) != 2 * S:
+            raise ValueError("collide needs one input and one output register per species")
+        fs = self.eles_scal_upts_full
+        self.sm.fs_batch([fs[r] for r in regids[:S]], [fs[r] for r in regids[S:]])
+
+    def species_register(self, p, r):
+        return self.eles_scal_upts_full[p * self.nregs0 + r]
+
+    def moments(self, bank_per_species):
+        """Per species (nupts, 15, neles) moments of the number-density
+        distribution (plugins/dgfsmomwriterstd.py:14-214 per species, mass
+        weights left to the caller); bank_per_species[p] is the bank index."""
+        import torch
+        from dgfs_hip import fr
+        out = []
+        for p, bank in enumerate(bank_per_species):
+            f = self.eles_scal_upts_full[bank]
+            nupts, vsize, neles = f.ioshape
+            m = torch.empty((nupts, 15, neles), dtype=torch.float64, device=self.backend.device)
+            fr.moments(nupts, neles, vsize, f, f.leaddim, self.vm.d_cv(), self.vm.cw(), m,
+                       self.backend.stream_ptr)
+            out.append(m.cpu().numpy())
+        return out

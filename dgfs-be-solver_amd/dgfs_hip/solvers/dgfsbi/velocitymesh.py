@@ -117,3 +117,12 @@ class DGFSBiVelocityMesh:
     def u0(self): return self._u0
     def R(self): return self._R
     def S(self): return self._S
+    def rho0(self): return self._rho0
+    def molarMass0(self): return self._molarMass0
+
+    def d_cv(self):
+        """Velocity components on the device, shape (3, vsize) FP64 (the grid
+        is shared by every species)."""
+        if getattr(self, "_d_cv", None) is None:
+            self._d_cv = self.backend.const_array(self._cv)
+        return self._d_cv

@@ -1,0 +1,159 @@
+"""GPU parity of the multi-species DGFS system (dgfsbi) against the FP64
+oracles: FR advection of every species (oracle/fr.py, diffuse walls with
+each species' unit-density wall Maxwellian) plus the VSS collision operator
+Q_p = sum_q Q_pq (oracle/vss.py), stepped with the reference's dgfsbi-euler
+and dgfsbi-tvd-rk2 sequences.  FP64 bar: 1e-11 relative L2 per species."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fr as OF
+from oracle import vss as V
+
+pytestmark = pytest.mark.gpu
+MASSES = (1.0, 2.1)
+PAIRS = [(0, 0), (0, 1), (1, 0), (1, 1)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# 16^3 on [-4, 4]^3: the finest grid at which the heavier species' Maxwellian
+# passes the initial condition's 1e-5 moment check (h = 0.5)
+NV, LV, ORDER = 16, 4.0, 1
+NUPTS = (ORDER + 1) ** 2
+
+
+def build(nregs, NvBS=None, Nv=NV):
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import bi_couette_cfg
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfsbi.system import DGFSBiSystem
+    cfg = bi_couette_cfg(Nv=Nv, L=LV, Nrho=4, M=12, Mpre=24, masses=MASSES, nden=(0.6, 0.4),
+                         omega=0.5, alpha=1.2, prefactor=0.05, order=ORDER, uwall=0.3, NvBatchSize=NvBS)
+    be = HIPBackend(cfg)
+    mesh = rect_quads(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True)
+    return be, DGFSBiSystem(be, mesh, cfg, nregs)
+
+
+def oracles(sysm):
+    vm = sysm.vm
+    ov = V.BiVelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M(), MASSES)
+    assert np.allclose(ov.cv, vm.cv()) and abs(ov.cw - vm.cw()) < 1e-15
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    adv = []
+    for p in range(2):
+        walls = {"bottom": (np.array([-0.3, 0, 0]), V.species_maxwellian(ov, p, 1.0, [-0.3, 0, 0], 1.0)),
+                 "top": (np.array([0.3, 0, 0]), V.species_maxwellian(ov, p, 1.0, [0.3, 0, 0], 1.0))}
+        adv.append(OF.AdvectionOracle(ORDER, omesh, ov.cv, ov.cw, walls))
+    g = {k: 2.0 * (1.0 - 0.5) for k in PAIRS}
+    eta = {k: 0.2 for k in PAIRS}
+    qo = V.VSSOracle(ov, g, eta, {k: 0.05 for k in PAIRS}, V.full_sphere_design(24))
+
+    def Q(fs):
+        fl = [f.transpose(2, 0, 1).reshape(-1, ov.n) for f in fs]
+        return [q.reshape(6, NUPTS, ov.n).transpose(1, 2, 0) for q in qo.q(fl)]
+    return ov, adv, Q
+
+
+def initial(ov, seed):
+    rng = np.random.default_rng(seed)
+    return [V.species_maxwellian(ov, p, nd, [0.05, -0.02, 0], 1.0)[None, :, None]
+            * (1 + 0.1 * rng.random((NUPTS, 1, 6))) for p, nd in enumerate((0.6, 0.4))]
+
+
+def test_bi_initial_condition_and_registers(device):
+    be, sysm = build(6)
+    ov, _, _ = oracles(sysm)
+    say("bi registers", len(sysm.eles_scal_upts_full))
+    assert len(sysm.eles_scal_upts_full) == 6
+    for p, nd in enumerate((0.6, 0.4)):
+        f = sysm.species_register(p, 0).get()
+        want = V.species_maxwellian(ov, p, nd, [0, 0, 0], 1.0)
+        assert np.allclose(f, want[None, :, None], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("stepper,nbs", [("dgfsbi-euler", None), ("dgfsbi-tvd-rk2", None),
+                                         ("dgfsbi-tvd-rk2", 1024)])
+def test_bi_step_vs_oracle(stepper, nbs, device):
+    from dgfs_hip.integrators.steppers import get_stepper
+    n0 = 2 if stepper == "dgfsbi-euler" else 3
+    be, sysm = build(2 * n0, NvBS=nbs)
+    ov, adv, Q = oracles(sysm)
+    f0 = initial(ov, 11)
+    for p in range(2):
+        sysm.species_register(p, 0).set(f0[p])
+    dt = 0.01
+    st = get_stepper(stepper, be, sysm, dt)
+    say("bi step", stepper, "NvBatchSize", sysm.vm.NvBatchSize())
+    st.advance_to(2 * dt)
+    be.synchronize()
+    got = [sysm.species_register(p, 0).get() for p in range(2)]
+    f = [x.copy() for x in f0]
+    for _ in range(2):
+        q0 = Q(f)
+        f1 = [f[p] + dt * (adv[p].rhs(f[p]) + q0[p]) for p in range(2)]
+        if stepper == "dgfsbi-euler":
+            f = f1
+        else:
+            q1 = Q(f1)
+            f = [0.5 * f[p] + 0.5 * f1[p] + 0.5 * dt * (adv[p].rhs(f1[p]) + q1[p]) for p in range(2)]
+    for p in range(2):
+        err = rel(got[p], f[p])
+        say("  species", p, "rel L2", err)
+        assert err < 1e-11
+
+
+def test_bi_species_number_conserved(device):
+    """Resting diffuse walls reflect every incoming particle of each species
+    and the FP64 projection keeps every species' number in Q: the
+    quadrature-weighted number of each species is conserved by the steps
+    (1e-12), and a mixture of species Maxwellians at the walls' state stays
+    put up to the operator's equilibrium error."""
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import bi_couette_cfg
+    from dgfs_hip.integrators.steppers import get_stepper
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfsbi.system import DGFSBiSystem
+    cfg = bi_couette_cfg(Nv=NV, L=LV, Nrho=4, M=12, Mpre=24, masses=MASSES, nden=(0.6, 0.4),
+                         omega=0.5, alpha=1.2, prefactor=0.05, order=ORDER, uwall=0.0, project=True)
+    be = HIPBackend(cfg)
+    mesh = rect_quads(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True)
+    sysm = DGFSBiSystem(be, mesh, cfg, 6)
+    W = OF.QuadFR(ORDER).W
+    ov = V.BiVelocityMesh(NV, LV, 4, 12, MASSES)
+
+    def number(f):
+        return float(np.einsum("i,ive->", W, f))
+
+    eq = [sysm.species_register(p, 0).get() for p in range(2)]
+    f0 = initial(ov, 5)
+    for p in range(2):
+        sysm.species_register(p, 0).set(f0[p])
+    n0 = [number(f) for f in f0]
+    st = get_stepper("dgfsbi-tvd-rk2", be, sysm, 0.01)
+    say("bi number conservation, 3 steps")
+    st.advance_to(0.03)
+    be.synchronize()
+    for p in range(2):
+        f = sysm.species_register(p, 0).get()
+        d = abs(number(f) - n0[p]) / n0[p]
+        say("  species", p, "number drift", d)
+        assert d < 1e-12
+    for p in range(2):
+        sysm.species_register(p, 0).set(eq[p])
+    st2 = get_stepper("dgfsbi-tvd-rk2", be, sysm, 0.01)
+    st2.advance_to(0.03)
+    be.synchronize()
+    for p in range(2):
+        f = sysm.species_register(p, 0).get()
+        ch = rel(f, eq[p])
+        say("  species", p, "equilibrium change", ch)
+        assert ch < 1e-3

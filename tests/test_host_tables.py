@@ -48,3 +48,59 @@ def test_vm_batch_checks():
     with pytest.raises(ValueError):
         DGFSVelocityMesh(None, cfg)
     assert isinstance(cfg, Inifile)
+
+
+def test_bi_initial_condition_moments_and_check():
+    """Species Maxwellians n_p (m_p/(pi T))^{3/2} exp(-m_p |c-u|^2 / T)
+    (frfs/solvers/dgfsbi/initcond.py:66-73): on a grid that resolves the
+    heavier species they reproduce n_p, u and T (number density sum * cw,
+    mass-weighted velocity, (2/3) m_p sum f |c-U|^2 cw / n_p); on a coarse
+    grid the reference's 1e-5 check raises."""
+    import math
+    import numpy as np
+    from dgfs_hip.cases import bi_couette_cfg
+    from dgfs_hip.solvers.dgfsbi.initcond import DGFSBiMaxwellianInitCondition
+    from dgfs_hip.solvers.dgfsbi.velocitymesh import DGFSBiVelocityMesh
+    cfg = bi_couette_cfg(Nv=16, L=4.0, Nrho=4, M=12, Mpre=24, masses=(1.0, 2.1), nden=(0.6, 0.4),
+                         prefactor=0.05, uwall=0.3)
+    vm = DGFSBiVelocityMesh(None, cfg, 2)
+    ic = DGFSBiMaxwellianInitCondition(None, cfg, vm, "soln-ics")
+    cv, cw = vm.cv(), vm.cw()
+    for p, (nd, m) in enumerate(zip((0.6, 0.4), (1.0, 2.1))):
+        f = ic.get_init_vals()[p]
+        assert abs(f.sum() * cw - nd) < 1e-7
+        want = nd * (m / math.pi) ** 1.5 * np.exp(-m * np.sum(cv ** 2, axis=0))
+        assert np.allclose(f, want, rtol=1e-14, atol=0)
+    wall = DGFSBiMaxwellianInitCondition(None, cfg, vm, "soln-bcs-top", wall=True)
+    assert np.allclose(wall.unondim().ravel(), [0.3, 0, 0])
+    for p in range(2):
+        fw = wall.get_init_vals()[p]
+        assert abs(fw.sum() * cw - 1.0) < 1e-6
+        assert abs((fw * cv[0]).sum() * cw - 0.3) < 1e-6
+    coarse = bi_couette_cfg(Nv=8, L=5.0, Nrho=4, M=12, Mpre=24, masses=(1.0, 2.1), prefactor=0.05)
+    with pytest.raises(ValueError):
+        DGFSBiMaxwellianInitCondition(None, coarse, DGFSBiVelocityMesh(None, coarse, 2), "soln-ics")
+
+
+def test_bi_steppers_registered_and_bank_layout():
+    """dgfsbi-euler / dgfsbi-tvd-rk2 (steppers.py:229-476): nregs_orig 2 / 3
+    per species, collide reads every species' register r and writes r + 1,
+    banked species-major."""
+    from dgfs_hip.integrators.steppers import STEPPERS
+
+    class FakeSystem:
+        _nspcs = 2
+        calls = []
+
+        def collide(self, t, regids):
+            self.calls.append(list(regids))
+
+    for name, n0 in (("dgfsbi-euler", 2), ("dgfsbi-tvd-rk2", 3)):
+        cls = STEPPERS[name]
+        st = cls.__new__(cls)
+        st._stepper_nregs = cls._stepper_nregs_orig * 2
+        st._regidx = list(range(st._stepper_nregs))
+        st.system = FakeSystem()
+        st._collide(0.0, 0, 1)
+        assert st.system.calls[-1] == [0, n0, 1, n0 + 1]
+        assert st._stepper_nregs == 2 * n0
