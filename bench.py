@@ -498,7 +498,7 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "kernel": (f"k_vhs_gain_rr<float,{nv},{'true' if algo == 3 else 'false'}>" if algo in (2, 3)
-                       else f"k_vhs_gain<float,{nv}>"),
+                       else f"k_vhs_gain_rr1<float,{nv}>" if algo == 4 else f"k_vhs_gain<float,{nv}>"),
             "algorithmic_bytes_per_eval": bq32,
             "points_per_launch": pts_per_launch,
             "avg_launch_ms": gain_ms / max(gain_n, 1),

@@ -226,24 +226,42 @@ struct NyqArgs {
 
 template <int N>
 struct NyqShape {
-  static constexpr int PW = 64 / N;        // planes per wave
+  static constexpr int PW = 64 / N;        // planes per wave (lanes past PW * N idle)
   static constexpr int PSTR = N * N + 16;  // LDS plane stride (planes on distinct banks)
 };
 
+// column c of row r in an N x N tile: XOR swizzle for power-of-two N (both
+// row and column accesses conflict free), (c + r) mod N otherwise (a
+// permutation of the row as well)
+template <int N>
+__device__ __forceinline__ int nyq_swizzle(int c, int r) {
+  if constexpr ((N & (N - 1)) == 0) {
+    return c ^ r;
+  } else {
+    const int s = c + r;
+    return s >= N ? s - N : s;
+  }
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
+  auto sw = [](int c, int r) { return nyq_swizzle<N>(c, r); };
   constexpr int PW = NyqShape<N>::PW, PSTR = NyqShape<N>::PSTR, NN = N * N, n = NN * N, h = N / 2;
-  static_assert(PW * N == 64, "one wave per PW planes");
+  static_assert(PW >= 1 && PW * N <= 64, "one wave per PW planes");
   __shared__ cpx<T> buf[PW * PSTR];
   __shared__ cpx<T> erow[PW][N];  // row-axis factor, 1 at index N/2
   __shared__ cpx<T> enyq[PW][2];  // row- and column-axis factors at index N/2
-  const int lane = threadIdx.x, w = lane / N, c = lane - w * N;
+  // lanes past PW * N (N not dividing 64) mirror the last plane's lanes
+  // inactively: they take part in the barriers only
+  const int lane = threadIdx.x, wl = lane / N;
+  const bool spare = wl >= PW;
+  const int w = spare ? PW - 1 : wl, c = spare ? lane - (PW - 1) * N - N : lane - wl * N;
   const int K = a.Nrho * a.M;
   cpx<T>* P = buf + w * PSTR;
   for (long base = static_cast<long>(blockIdx.x) * PW; base < a.nplanes;
        base += static_cast<long>(gridDim.x) * PW) {
     const long pl = base + w;
-    const bool active = pl < a.nplanes;
+    const bool active = !spare && pl < a.nplanes;
     const long item = active ? pl / 3 : 0;
     const int j = active ? static_cast<int>(pl - item * 3) : 0;
     const int pt = static_cast<int>(item / K), pq = static_cast<int>(item - static_cast<long>(pt) * K);
@@ -251,10 +269,12 @@ __global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
     const int cb = (j == 2) ? 1 : 2;  // column axis
     const cpx<T>* E = a.ephtab + static_cast<size_t>(pq) * 3 * N;
     const cpx<T> er = E[ra * N + c], ec = E[cb * N + c], en = E[j * N + h];
-    erow[w][c] = (c == h) ? cpx<T>{T(1), T(0)} : er;
-    if (c == h) {
-      enyq[w][0] = er;
-      enyq[w][1] = ec;
+    if (!spare) {
+      erow[w][c] = (c == h) ? cpx<T>{T(1), T(0)} : er;
+      if (c == h) {
+        enyq[w][0] = er;
+        enyq[w][1] = ec;
+      }
     }
     __syncthreads();
     // Im(E_S) for the four Nyquist patterns (row N/2?, column N/2?)
@@ -280,21 +300,27 @@ __global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
       x[r] = m * fv;
     }
     FFT<T, N, +1>::run(x);  // along the rows
+    if (!spare) {
 #pragma unroll
-    for (int r = 0; r < N; ++r) P[r * N + (c ^ r)] = x[r];
+      for (int r = 0; r < N; ++r) P[r * N + sw(c, r)] = x[r];
+    }
     __syncthreads();
     // lane c now owns row c of the plane
+    if (!spare) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) x[k] = P[c * N + (k ^ c)];
+      for (int k = 0; k < N; ++k) x[k] = P[c * N + sw(k, c)];
+    }
     FFT<T, N, +1>::run(x);  // along the columns
+    if (!spare) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) P[c * N + (k ^ c)] = x[k];
+      for (int k = 0; k < N; ++k) P[c * N + sw(k, c)] = x[k];
+    }
     __syncthreads();
     if (active) {
       cpx<T>* Wp = a.W + static_cast<size_t>(item) * 3 * NN + j * NN + c;
 #pragma unroll
       for (int r = 0; r < N; ++r) {
-        const cpx<T> y = P[r * N + (c ^ r)];
+        const cpx<T> y = P[r * N + sw(c, r)];
         Wp[r * N] = cpx<T>{y.x + y.y, y.y - y.x};  // times (1 - i)
       }
     }
@@ -654,6 +680,147 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident gain kernel, one pencil per thread (N whose N^3 complex
+// field fits in LDS: 16, 24).  A workgroup of N*N threads owns one work item;
+// thread t = g N + l holds one pencil.  Per (p, q):
+//   z-pencil (kx = g, ky = l) from the transposed (1 + i)-folded spectrum,
+//   phase weight, z-IFFT; the kx plane goes through LDS ([ky][z], rows
+//   padded to N+1) to give the y-pencil (kx = g, z = l), y-IFFT; the whole
+//   field goes through LDS once ([kx][y][z]) to give the x-pencil (y = g,
+//   z = l), x-IFFT, Re(P+ P-) with the Nyquist-plane corrections, S_p += in
+//   registers.  At the end of p the reverse sequence forward transforms S_p
+//   and accumulates w_p b1_p FFT(S_p) into G^.  Same algebra and tables as
+//   k_vhs_gain_rr; every LDS access is a lane-consecutive row.
+// ---------------------------------------------------------------------------
+template <int N>
+struct RR1Shape {
+  static constexpr int NT = N * N;
+  static constexpr int PP = N + 1;          // padded row pitch of the kx planes
+  static constexpr int field = N * N * PP;  // complex; holds the N^3 exchange too
+  static constexpr int wl = 3 * N * N;      // complex, Nyquist planes
+  static_assert(field >= N * N * N, "exchange must fit the plane buffer");
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> a) {
+  constexpr int NT = RR1Shape<N>::NT, PP = RR1Shape<N>::PP, NN = N * N, n = NN * N;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* Wl = X + RR1Shape<N>::field;
+  __shared__ cpx<T> eph[4][N];  // x, y, z phase factors and the z pair (Re + Im, Re - Im)
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  const int K = a.Nrho * a.M;
+  cpx<T>* Xp = X + g * N * PP;  // kx plane g
+  T sacc[N];
+  cpx<T> v[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int pt = item / a.ngroups, grp = item - pt * a.ngroups;
+    const int p0 = grp * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
+    const cpx<T>* Ft = a.Ft + static_cast<size_t>(pt) * n;
+    cpx<T>* Gh = a.Ghat + static_cast<size_t>(item) * n;
+    for (int p = p0; p < p1; ++p) {
+#pragma unroll
+      for (int x = 0; x < N; ++x) sacc[x] = T(0);
+      for (int q = 0; q < a.M; ++q) {
+        __syncthreads();  // readers of X, Wl, eph from the previous (p, q) are done
+        if (tid < 3 * N) {
+          const cpx<T> e = a.ephtab[(static_cast<size_t>(p) * a.M + q) * 3 * N + tid];
+          (&eph[0][0])[tid] = e;
+          if (tid >= 2 * N) eph[3][tid - 2 * N] = cpx<T>{e.x + e.y, e.x - e.y};
+        }
+        {
+          const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
+          static_assert((3 * NN) % NT == 0, "Nyquist planes split evenly over the workgroup");
+          cpx<T> wv[3 * NN / NT];
+#pragma unroll
+          for (int k = 0; k < 3 * NN / NT; ++k) wv[k] = Wq[tid + k * NT];
+#pragma unroll
+          for (int k = 0; k < 3 * NN / NT; ++k) Wl[tid + k * NT] = wv[k];
+        }
+        // z-pencil (kx = g, ky = l): two base addresses with immediate offsets
+        {
+          constexpr int KB = (4096 / static_cast<int>(N * sizeof(cpx<T>))) < N
+                                 ? (4096 / static_cast<int>(N * sizeof(cpx<T>))) : N;
+          const cpx<T>* P0 = Ft + g * NN + l;
+          int second = KB * N;
+          asm volatile("" : "+v"(second));
+          const cpx<T>* P1 = P0 + second;
+#pragma unroll
+          for (int kz = 0; kz < N; ++kz) v[kz] = (kz < KB) ? P0[kz * N] : P1[(kz - KB) * N];
+        }
+        __syncthreads();  // eph and Wl are in place
+        {
+          const cpx<T> exy = eph[0][g] * eph[1][l];
+#pragma unroll
+          for (int kz = 0; kz < N; ++kz) {
+            const cpx<T> pm = eph[3][kz];
+            v[kz] = (exy.x * pm.x + exy.y * pm.y) * v[kz];
+          }
+        }
+        FFT<T, N, +1>::run(v);  // z
+#pragma unroll
+        for (int z = 0; z < N; ++z) Xp[l * PP + z] = v[z];
+        __syncthreads();
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) v[ky] = Xp[ky * PP + l];
+        FFT<T, N, +1>::run(v);  // y: (kx = g, y, z = l)
+        __syncthreads();
+#pragma unroll
+        for (int y = 0; y < N; ++y) X[(g * N + y) * N + l] = v[y];
+        __syncthreads();
+#pragma unroll
+        for (int kx = 0; kx < N; ++kx) v[kx] = X[(kx * N + g) * N + l];
+        FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
+        {
+          const cpx<T> w1 = Wl[g * N + l];
+          const cpx<T>* W2 = Wl + NN + l;
+          const cpx<T>* W3 = Wl + 2 * NN + g;
+          const T sy = (g & 1) ? T(-1) : T(1), szs = (l & 1) ? T(-1) : T(1);
+#pragma unroll
+          for (int x = 0; x < N; ++x) {
+            const T sx = (x & 1) ? T(-1) : T(1);
+            const cpx<T> w2 = W2[x * N], w3 = W3[x * N];
+            const T wr = sx * w1.x + sy * w2.x + szs * w3.x;
+            const T wi = sx * w1.y + sy * w2.y + szs * w3.y;
+            sacc[x] += (v[x].x + wi) * (v[x].y - wr) - wr * wi;
+          }
+        }
+      }
+      // forward transform of S_p and accumulation into G^
+#pragma unroll
+      for (int x = 0; x < N; ++x) v[x] = cpx<T>{sacc[x], T(0)};
+      FFT<T, N, -1>::run(v);  // x: (kx, y = g, z = l)
+      __syncthreads();
+#pragma unroll
+      for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = v[kx];
+      __syncthreads();
+#pragma unroll
+      for (int y = 0; y < N; ++y) v[y] = X[(g * N + y) * N + l];
+      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
+      __syncthreads();
+#pragma unroll
+      for (int ky = 0; ky < N; ++ky) Xp[ky * PP + l] = v[ky];
+      __syncthreads();
+#pragma unroll
+      for (int z = 0; z < N; ++z) v[z] = Xp[l * PP + z];
+      FFT<T, N, -1>::run(v);  // z: (kx = g, ky = l, kz)
+      const T wp = static_cast<T>(a.wgt[p]);
+      const size_t o = static_cast<size_t>(g) * NN + l * N;
+      const T* b = a.b1 + static_cast<size_t>(p) * n + o;
+      cpx<T>* gp = Gh + o;
+      if (p == p0) {
+#pragma unroll
+        for (int kz = 0; kz < N; ++kz) gp[kz] = (wp * b[kz]) * v[kz];
+      } else {
+#pragma unroll
+        for (int kz = 0; kz < N; ++kz) gp[kz] = gp[kz] + (wp * b[kz]) * v[kz];
+      }
+    }
+  }
+}
+
 template <typename T>
 struct FinArgs {
   const cpx<T>* Ghat;  // [npts*ngroups][n]
@@ -891,7 +1058,9 @@ struct VhsHandle {
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent multi-pass grid (0: occupancy)
   int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident (Nyquist planes
-                         // from k_vhs_nyq), 3 register-resident with in-kernel Nyquist planes
+                         // from k_vhs_nyq), 3 register-resident with in-kernel Nyquist planes,
+                         // 4 register-resident with one pencil per thread (Nyquist planes
+                         // from k_vhs_nyq)
   int last_algorithm = 0;
   void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
   void* Ft = nullptr;    // spectrum transposed to [kx][kz][ky] [cap_pts][n]
@@ -962,25 +1131,41 @@ static int occupancy_slots() {
   return occ * cu_count();
 }
 
-// the register-resident gain kernel exists for FP32 at power-of-two N <= 32
+// the register-resident gain kernels exist for FP32: two pencils per thread
+// at N in {16, 32}, one pencil per thread at N in {16, 24}
 template <typename T, int N>
 constexpr bool rr_available() {
   return std::is_same<T, float>::value && (N == 16 || N == 32);
 }
-
-static bool use_rr(const VhsHandle* h) {
-  const bool avail = (h->prec == 32) && (h->N == 16 || h->N == 32);
-  if (h->algorithm == 1) return false;
-  if (h->algorithm == 2 || h->algorithm == 3) return avail;
-  return avail;
+template <typename T, int N>
+constexpr bool rr1_available() {
+  return std::is_same<T, float>::value && (N == 16 || N == 24);
 }
+static bool rr2_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 32); }
+static bool rr1_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 24); }
+
+// gain algorithm a launch uses: 1 multi-pass, 2/3 two pencils per thread,
+// 4 one pencil per thread.  The automatic choice takes the one-pencil kernel
+// where the field fits in LDS (Nv 16, 24: 1.8x the two-pencil kernel at 16^3
+// on MI355X), the two-pencil kernel at Nv 32, the multi-pass one otherwise.
+static int rr_kind(const VhsHandle* h) {
+  switch (h->algorithm) {
+    case 1: return 1;
+    case 2: return rr2_ok(h) ? 2 : 1;
+    case 3: return rr2_ok(h) ? 3 : 1;
+    case 4: return rr1_ok(h) ? 4 : 1;
+    default: return rr1_ok(h) ? 4 : (rr2_ok(h) ? 2 : 1);
+  }
+}
+
+static bool use_rr(const VhsHandle* h) { return rr_kind(h) != 1; }
 
 template <typename T, int N>
 static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items, hipStream_t st) {
   if constexpr (rr_available<T, N>()) {
     constexpr int NN = N * N;
     const int K = h->Nrho * h->M;
-    const bool inl = (h->algorithm == 3);
+    const bool inl = (rr_kind(h) == 3);
     h->last_algorithm = inl ? 3 : 2;
     const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
     if (!inl && wbytes > h->cap_wny) {
@@ -1023,6 +1208,51 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
   } else {
     (void)h; (void)npts; (void)groups; (void)ppg; (void)items; (void)st;
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel not built for this precision/N");
+  }
+}
+
+template <typename T, int N>
+static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t items, hipStream_t st) {
+  if constexpr (rr1_available<T, N>()) {
+    constexpr int NN = N * N;
+    const int K = h->Nrho * h->M;
+    h->last_algorithm = 4;
+    const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
+    if (wbytes > h->cap_wny) {
+      (void)hipFree(h->Wny);
+      h->Wny = nullptr;
+      DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
+      h->cap_wny = wbytes;
+    }
+    const long nplanes = static_cast<long>(npts) * K * 3;
+    NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<const cpx<T>*>(h->Ft),
+                  static_cast<cpx<T>*>(h->Wny), static_cast<const cpx<T>*>(h->ephtab), nplanes, h->Nrho, h->M};
+    const long waves = (nplanes + NyqShape<N>::PW - 1) / NyqShape<N>::PW;
+    KTimerT<VhsHandle> t4(h, 4, st);
+    hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(waves, 16L * cu_count()))),
+                       dim3(64), 0, st, na);
+    DGFS_LAUNCH_CHECK("vhs_nyq");
+    t4.stop();
+    KTimerT<VhsHandle> t1(h, 1, st);
+    const size_t lds = (static_cast<size_t>(RR1Shape<N>::field) + RR1Shape<N>::wl) * sizeof(cpx<T>);
+    const void* kfn = reinterpret_cast<const void*>(k_vhs_gain_rr1<T, N>);
+    DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, RR1Shape<N>::NT, lds) != hipSuccess || occ < 1)
+      occ = 1;
+    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(occ) * cu_count()));
+    GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
+                     static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
+                     static_cast<const cpx<T>*>(h->ephtab), static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+    hipLaunchKernelGGL((k_vhs_gain_rr1<T, N>), dim3(grid), dim3(RR1Shape<N>::NT), lds, st, ga);
+    DGFS_LAUNCH_CHECK("vhs_gain_rr1");
+    t1.stop();
+    h->threads = RR1Shape<N>::NT;
+    h->last_slots = grid;
+    return DGFS_OK;
+  } else {
+    (void)h; (void)npts; (void)groups; (void)ppg; (void)items; (void)st;
+    return fail(DGFS_ERR_UNSUPPORTED, "one-pencil register-resident kernel not built for this precision/N");
   }
 }
 
@@ -1108,7 +1338,10 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   int rc = stage_prep<T, N>(h, fc, npts, groups, slots, rr, &ga, st);
   if (rc) return rc;
   if (rr) {
-    rc = run_gain_rr<T, N>(h, npts, groups, ga.ppg, static_cast<size_t>(ga.nitems), st);
+    if (rr_kind(h) == 4)
+      rc = run_gain_rr1<T, N>(h, npts, groups, ga.ppg, static_cast<size_t>(ga.nitems), st);
+    else
+      rc = run_gain_rr<T, N>(h, npts, groups, ga.ppg, static_cast<size_t>(ga.nitems), st);
     if (rc) return rc;
   } else {
     h->last_algorithm = 1;
@@ -1447,9 +1680,11 @@ extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
 extern "C" int dgfs_vhs_set_algorithm(void* p, int algorithm) {
   VhsHandle* h = as_handle(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
-  if (algorithm < 0 || algorithm > 3) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1, 2 or 3");
-  if (algorithm >= 2 && !(h->prec == 32 && (h->N == 16 || h->N == 32)))
-    return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel: FP32 and Nv in {16, 32} only");
+  if (algorithm < 0 || algorithm > 4) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1, 2, 3 or 4");
+  if ((algorithm == 2 || algorithm == 3) && !rr2_ok(h))
+    return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (two pencils per thread): FP32 and Nv in {16, 32} only");
+  if (algorithm == 4 && !rr1_ok(h))
+    return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (one pencil per thread): FP32 and Nv in {16, 24} only");
   h->algorithm = algorithm;
   return DGFS_OK;
 }

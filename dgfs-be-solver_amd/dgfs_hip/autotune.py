@@ -2,8 +2,9 @@
 
 For a given operator (velocity grid, precision, K = M Nrho) and batch size it
 times every candidate of
-  * the gain algorithm: multi-pass (any grid) or register-resident (FP32,
-    Nv 16 / 32; Nyquist planes from a separate kernel or inside the kernel);
+  * the gain algorithm: multi-pass (any grid) or register-resident (FP32:
+    two pencils per thread at Nv 16 / 32 with the Nyquist planes from a
+    separate kernel or inside the kernel; one pencil per thread at Nv 16 / 24);
   * the radial-group split (work items per point: 1, 2, 4, ..., Nrho);
   * the persistent grid of the multi-pass kernels (1, 2, 4 workgroups per CU,
     or the occupancy limit),
@@ -49,6 +50,8 @@ class Autotuner:
     @staticmethod
     def candidates(op, cus):
         algos = [1]
+        if op.precision == 32 and op.Nv in (16, 24):
+            algos[:0] = [4]
         if op.precision == 32 and op.Nv in (16, 32):
             algos[:0] = [2, 3]
         groups = [0]

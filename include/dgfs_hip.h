@@ -103,7 +103,11 @@ int dgfs_vhs_set_slots(void* handle, int nslots);
  * through a per-workgroup scratch, any N and precision), 2 register-resident
  * (the whole N^3 field in registers, FP32 with Nv 16 or 32; Nyquist-plane
  * corrections precomputed by a separate kernel), 3 register-resident with the
- * Nyquist-plane corrections computed inside the gain kernel.  get_algorithm reports the one the last eval ran.
+ * Nyquist-plane corrections computed inside the gain kernel, 4 register-resident
+ * with one pencil per thread and the transposes through LDS (FP32 with Nv 16
+ * or 24; Nyquist planes from the separate kernel).  The automatic choice is 4
+ * at Nv 16 and 24, 2 at Nv 32, 1 otherwise.  get_algorithm reports the one the
+ * last eval ran.
  * No reference counterpart (the reference has one cuFFT dataflow). */
 int dgfs_vhs_set_algorithm(void* handle, int algorithm);
 int dgfs_vhs_get_algorithm(void* handle, int* algorithm);
