@@ -99,4 +99,5 @@ class DGFSSystem:
         out = torch.empty((nupts, 15, neles), dtype=torch.float64, device=self.backend.device)
         fr.moments(nupts, neles, vsize, f, f.leaddim, self.vm.d_cv(), self.vm.cw(), out,
                    self.backend.stream_ptr)
+        self.backend.synchronize()  # the kernel may run on the backend's own stream
         return out.cpu().numpy()

@@ -133,5 +133,6 @@ class DGFSBiSystem:
             m = torch.empty((nupts, 15, neles), dtype=torch.float64, device=self.backend.device)
             fr.moments(nupts, neles, vsize, f, f.leaddim, self.vm.d_cv(), self.vm.cw(), m,
                        self.backend.stream_ptr)
+            self.backend.synchronize()  # the kernel may run on the backend's own stream
             out.append(m.cpu().numpy())
         return out
