@@ -105,7 +105,9 @@ def bench_rhs(args, dev):
     be = HIPBackend(cfg, own_stream=True)
     mesh = rect_quads(16, 16, 0.0, 1.0, 0.0, 1.0)
     sysm = DGFSSystem(be, mesh, cfg, 3)
-    dt = 0.5 * sysm.unit_cfl_dt
+    # SSP-RK2 with the p = 3 FR operator needs a step well below the unit-CFL
+    # step to stay bounded over long runs; the throughput does not depend on it
+    dt = 0.05 * sysm.unit_cfl_dt
     st = get_stepper("dgfs-tvd-rk2", be, sysm, dt)
     gs = GraphStepper(st)
     out = {"workload": "cavity 16x16 quads p=3 (4096 points), Nv=32, M=12, Nrho=16, SSP-RK2, "
