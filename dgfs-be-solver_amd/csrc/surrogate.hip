@@ -615,11 +615,19 @@ __device__ void herm_project(const double* __restrict__ g, const double* V, cons
   const double* Vx = V;
   const double* Vy = V + N * kSurD;
   const double* Vz = V + 2 * N * kSurD;
-  for (int e = threadIdx.x; e < kSurD * NN; e += blockDim.x) {  // t1[a][j][k]
-    const int a = e / NN, jk = e - a * NN;
-    double s = 0.0;
-    for (int i = 0; i < N; ++i) s += g[i * NN + jk] * Vx[i * kSurD + a];
-    t1[e] = s;
+  // t1[a][j][k]: each thread owns columns jk and reads every g value once,
+  // accumulating all kSurD x-modes of it
+  for (int jk = threadIdx.x; jk < NN; jk += blockDim.x) {
+    double acc[kSurD];
+#pragma unroll
+    for (int a = 0; a < kSurD; ++a) acc[a] = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double gv = g[i * NN + jk];
+#pragma unroll
+      for (int a = 0; a < kSurD; ++a) acc[a] += gv * Vx[i * kSurD + a];
+    }
+#pragma unroll
+    for (int a = 0; a < kSurD; ++a) t1[a * NN + jk] = acc[a];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kSurD * kSurD * N; e += blockDim.x) {  // t2[a][b][k]
