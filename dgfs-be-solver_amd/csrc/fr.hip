@@ -518,3 +518,66 @@ extern "C" int dgfs_moments(int nupts, int neles, int vsize, const double* f, lo
   DGFS_LAUNCH_CHECK("dgfs_moments");
   return DGFS_OK;
 }
+
+// Residual of two full-velocity AoSoA registers (plugins/dgfsresidualstd.py:
+// 41-49): out2 = (sum (curr - prev)^2, sum prev^2) over the valid entries
+// (nupts rows, vsize variables, neles elements; AoSoA padding and row padding
+// excluded).  Two deterministic stages: per-block partials, then one block;
+// `work` holds 2 * kResidBlocks doubles.
+namespace dgfs {
+constexpr int kResidBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_resid_part(int nupts, int neles, int vsize, const double* __restrict__ prev,
+                                                    const double* __restrict__ curr, long ld, double* part) {
+  __shared__ double r0[256], r1[256];
+  const long nb = (neles + 31) / 32;
+  const long per_row = nb * vsize * 32;
+  const long total = static_cast<long>(nupts) * per_row;
+  double a = 0.0, b = 0.0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const long row = i / per_row, j = i - row * per_row;
+    const long eb = j / (static_cast<long>(vsize) * 32);
+    if (eb * 32 + (j & 31) >= neles) continue;  // AoSoA padding of the last block
+    const double p = prev[row * ld + j], c = curr[row * ld + j];
+    a += (c - p) * (c - p);
+    b += p * p;
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0[0];
+    part[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_resid_final(const double* part, int nblk, double* out2) {
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int k = 0; k < nblk; ++k) s += part[2 * k + threadIdx.x];
+    out2[threadIdx.x] = s;
+  }
+}
+}  // namespace dgfs
+
+extern "C" int dgfs_residual_norms(int nupts, int neles, int vsize, const double* prev, const double* curr, long ld,
+                                   double* out2, double* work, void* stream) {
+  using namespace dgfs;
+  if (!prev || !curr || !out2 || !work) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_residual_norms: null pointer");
+  const long nparr = ((neles + 31L) / 32) * 32;
+  if (ld < nparr * vsize) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_residual_norms: leading dimension");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_resid_part, dim3(kResidBlocks), dim3(256), 0, st, nupts, neles, vsize, prev, curr, ld, work);
+  DGFS_LAUNCH_CHECK("dgfs_residual_norms");
+  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, st, work, kResidBlocks, out2);
+  DGFS_LAUNCH_CHECK("dgfs_residual_norms");
+  return DGFS_OK;
+}

@@ -438,30 +438,33 @@ __device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __re
   __builtin_amdgcn_sched_barrier(0);
 }
 
-// exchange for target slot r (y = g + G r): two sub-rounds move the y-half r
-// of plane slot 0 (kx = g) then of plane slot 1 (kx = g + G); then the x-IFFT
-// and Re(P+ P-) with the Nyquist corrections, returned in val[x]
+// x exchange for target slot r (y = g + G r), in two sub-rounds: sub-round sr
+// moves the y-half r of plane slot sr (kx = g + G sr) through the buffer
+// [kxl][yy][z]; the caller places the barriers, so that a write of the next
+// target's first sub-round can overlap the previous target's x-IFFT
 template <typename T, int N, int G>
-__device__ __forceinline__ void rr_x_inverse(T (&val)[N], const cpx<T> (&v0)[N], const cpx<T> (&v1)[N],
-                                             cpx<T>* Xb, const cpx<T>* Wl, int r, int g, int l) {
-  constexpr int NN = N * N;
-  cpx<T> u[N];
+__device__ __forceinline__ void rr_x_write(const cpx<T> (&v)[N], cpx<T>* Xb, int r, int g, int l) {
   cpx<T>* Xw = Xb + (g * G) * N + l;  // [kxl][yy][z]
-  const cpx<T>* Xr = Xb + g * N + l;   // read (kxl, yy = g, z = l)
 #pragma unroll
-  for (int sr = 0; sr < 2; ++sr) {
-    const cpx<T> (&v)[N] = sr ? v1 : v0;
-    __syncthreads();
+  for (int yy = 0; yy < G; ++yy) Xw[yy * N] = v[G * r + yy];
+}
+
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_x_read(cpx<T> (&u)[N], const cpx<T>* Xb, int sr, int g, int l) {
+  const cpx<T>* Xr = Xb + g * N + l;  // read (kxl, yy = g, z = l)
 #pragma unroll
-    for (int yy = 0; yy < G; ++yy) Xw[yy * N] = v[G * r + yy];
-    __syncthreads();
+  for (int k0 = 0; k0 < G; k0 += 8) {
 #pragma unroll
-    for (int k0 = 0; k0 < G; k0 += 8) {
-#pragma unroll
-      for (int kk = k0; kk < k0 + 8; ++kk) u[G * sr + kk] = Xr[kk * G * N];
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    for (int kk = k0; kk < k0 + 8; ++kk) u[G * sr + kk] = Xr[kk * G * N];
+    __builtin_amdgcn_sched_barrier(0);
   }
+}
+
+// x-IFFT of the gathered pencil (y = g + G r, z = l) and Re(P+ P-) with the
+// Nyquist-plane corrections, returned in val[x]
+template <typename T, int N, int G>
+__device__ __forceinline__ void rr_x_compute(T (&val)[N], cpx<T> (&u)[N], const cpx<T>* Wl, int r, int g, int l) {
+  constexpr int NN = N * N;
   FFT<T, N, +1>::run(u);
   __builtin_amdgcn_sched_barrier(0);
   const int y = g + G * r, z = l;
@@ -658,10 +661,29 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
         rr_zy_inverse<T, N, G>(v0, Ft, Xb, eph, g, g, l);
         rr_zy_inverse<T, N, G>(v1, Ft, Xb, eph, g + G, g, l);
         T val[N];
-        rr_x_inverse<T, N, G>(val, v0, v1, Xb, Wl, 0, g, l);
+        cpx<T> u[N];
+        // target slot 0 (y = g): both sub-rounds
+        __syncthreads();
+        rr_x_write<T, N, G>(v0, Xb, 0, g, l);
+        __syncthreads();
+        rr_x_read<T, N, G>(u, Xb, 0, g, l);
+        __syncthreads();
+        rr_x_write<T, N, G>(v1, Xb, 0, g, l);
+        __syncthreads();
+        rr_x_read<T, N, G>(u, Xb, 1, g, l);
+        __syncthreads();
+        // target slot 1's first sub-round is written while slot 0 transforms
+        rr_x_write<T, N, G>(v0, Xb, 1, g, l);
+        rr_x_compute<T, N, G>(val, u, Wl, 0, g, l);
 #pragma unroll
         for (int x = 0; x < N; ++x) S0[x * NT] += val[x];
-        rr_x_inverse<T, N, G>(val, v0, v1, Xb, Wl, 1, g, l);
+        __syncthreads();
+        rr_x_read<T, N, G>(u, Xb, 0, g, l);
+        __syncthreads();
+        rr_x_write<T, N, G>(v1, Xb, 1, g, l);
+        __syncthreads();
+        rr_x_read<T, N, G>(u, Xb, 1, g, l);
+        rr_x_compute<T, N, G>(val, u, Wl, 1, g, l);
 #pragma unroll
         for (int x = 0; x < N; ++x) s1[x] += val[x];
       }

@@ -15,6 +15,7 @@ _lib.register({
     "frfs_axnpby": [_L, _I, C.POINTER(_P), C.POINTER(_D), _P],
     "dgfs_copy_reg": [_I, _I, _P, _L, _I, _I, _P, _L, _I, _I, _I, _P],
     "dgfs_moments": [_I, _I, _I, _P, _L, _P, _D, _P, _P],
+    "dgfs_residual_norms": [_I, _I, _I, _P, _P, _L, _P, _P, _P],
 })
 
 BC_DIFFUSE, BC_SPECULAR, BC_INLET, BC_OUTLET, BC_VANISH, BC_INLET_NS = range(6)
@@ -75,3 +76,7 @@ def copy_reg(nupts, neles, src, lds, nvsrc, s0, dst, ldd, nvdst, d0, count, stre
 
 def moments(nupts, neles, vsize, f, ld, cv, cw, out, stream=None):
     _lib.call("dgfs_moments", nupts, neles, vsize, ptr(f), ld, ptr(cv), cw, ptr(out), stream)
+
+
+def residual_norms(nupts, neles, vsize, prev, curr, ld, out2, work, stream=None):
+    _lib.call("dgfs_residual_norms", nupts, neles, vsize, ptr(prev), ptr(curr), ld, ptr(out2), ptr(work), stream)

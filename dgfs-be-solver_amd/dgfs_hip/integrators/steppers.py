@@ -23,6 +23,9 @@ class BaseDGFSStepper:
         self._axk = {}
         self._cpk = {}
         self.nfevals = 0
+        # called as plugin(stepper) after every accepted step of advance_to
+        # (the reference integrator's completed-step handlers)
+        self.plugins = []
 
     # register plumbing ----------------------------------------------------
     def _add_dgfs(self, *args):
@@ -54,6 +57,8 @@ class BaseDGFSStepper:
             self._idxcurr = self.step(self.tcurr, dt)
             self.tcurr += dt
             self.nacptsteps += 1
+            for pl in self.plugins:
+                pl(self)
 
     @property
     def soln_bank(self):

@@ -1,0 +1,58 @@
+"""Solver plug-ins on the HIP path.
+
+DGFSResidualStdPlugin mirrors frfs/plugins/dgfsresidualstd.py:9-80: every
+`nsteps` accepted steps it reports sqrt(sum ||f_n - f_{n-1}||^2 /
+sum ||f_{n-1}||^2) over the full-velocity registers (previous = the state
+one step earlier).  The reference copies both registers to the host; here
+the previous state is a device copy and the two sums come from one HIP
+reduction (dgfs_residual_norms), so only two doubles cross to the host.
+"""
+import math
+
+
+class DGFSResidualStdPlugin:
+    name = "dgfsresidualstd"
+    systems = ["dgfs"]
+    formulations = ["std"]
+
+    def __init__(self, stepper, nsteps, outf=None):
+        import torch
+        self.stepper = stepper
+        self.nsteps = int(nsteps)
+        if self.nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
+        self.outf = outf
+        be = stepper.backend
+        self._work = torch.empty(1024, dtype=torch.float64, device=be.device)
+        self._out2 = torch.empty(2, dtype=torch.float64, device=be.device)
+        self._prev = None
+        self.rows = []
+        self(stepper)
+
+    def _regs(self):
+        st = self.stepper
+        return [st.system.eles_scal_upts_full[st.soln_bank]]
+
+    def __call__(self, stepper=None):
+        from dgfs_hip import fr
+        st = self.stepper
+        n = st.nacptsteps
+        if n % self.nsteps == 0 and n and self._prev is not None:
+            num = den = 0.0
+            for p, c in zip(self._prev, self._regs()):
+                nupts, vsize, neles = c.ioshape
+                fr.residual_norms(nupts, neles, vsize, p, c, c.leaddim, self._out2, self._work,
+                                  st.backend.stream_ptr)
+                st.backend.synchronize()
+                a, b = (float(v) for v in self._out2.cpu())
+                num += a
+                den += b
+            row = [st.tcurr, math.sqrt(num / den)]
+            self.rows.append(row)
+            if self.outf is not None:
+                print(",".join(str(r) for r in row), file=self.outf)
+                self.outf.flush()
+            self._prev = None
+        if (n + 1) % self.nsteps == 0:
+            st.backend.synchronize()
+            self._prev = [r.data.clone() for r in self._regs()]

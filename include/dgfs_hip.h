@@ -327,6 +327,14 @@ int dgfs_copy_reg(int nupts, int neles, const double* src, long lds, int nvsrc, 
 int dgfs_moments(int nupts, int neles, int vsize, const double* f, long ld, const double* cv,
                  double cw, double* out, void* stream);
 
+/* Residual monitor (frfs/plugins/dgfsresidualstd.py:41-49): out2[0] =
+ * sum (curr - prev)^2, out2[1] = sum prev^2 over the valid entries of two
+ * full-velocity AoSoA registers of leading dimension ld (row and AoSoA
+ * padding excluded); work is caller-owned device scratch of 1024 doubles.
+ * The host takes sqrt(out2[0] / out2[1]) as the residual. */
+int dgfs_residual_norms(int nupts, int neles, int vsize, const double* prev, const double* curr, long ld,
+                        double* out2, double* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,3 +229,28 @@ def _tri_nodal_weights(ele):
     pts = np.stack([xi.ravel(), B.ravel()], axis=1)
     L = ele.basis.ubasis.nodal_basis_at(pts)     # (npts, nupts)
     return WA.ravel() @ L
+
+
+def test_residual_plugin(device):
+    """DGFSResidualStdPlugin (plugins/dgfsresidualstd.py:9-80): every nsteps
+    steps sqrt(sum (f_n - f_{n-1})^2 / sum f_{n-1}^2) over the valid entries
+    of the register, from the HIP reduction, against numpy on the unpacked
+    registers (1e-12)."""
+    from dgfs_hip.integrators.steppers import get_stepper
+    from dgfs_hip.plugins import DGFSResidualStdPlugin
+    be, sysm, _ = build_system(nregs=3)
+    rng = np.random.default_rng(9)
+    f0 = sysm.eles_scal_upts_full[0].get()
+    sysm.eles_scal_upts_full[0].set(f0 * (1 + 0.05 * rng.random(f0.shape)))
+    st = get_stepper("dgfs-tvd-rk2", be, sysm, 0.01)
+    seen = []
+    st.plugins.append(lambda s: seen.append(s.system.eles_scal_upts_full[s.soln_bank].get()))
+    pl = DGFSResidualStdPlugin(st, 2)
+    st.plugins.append(pl)
+    say("residual plugin, 4 steps")
+    st.advance_to(0.04)
+    assert len(pl.rows) == 2
+    for k, (i, j) in enumerate([(0, 1), (2, 3)]):
+        want = math.sqrt(np.sum((seen[j] - seen[i]) ** 2) / np.sum(seen[i] ** 2))
+        say("  ", pl.rows[k][1], want)
+        assert abs(pl.rows[k][1] - want) <= 1e-12 * want
