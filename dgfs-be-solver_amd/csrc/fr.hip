@@ -158,7 +158,19 @@ __global__ __launch_bounds__(256) void k_intcflux(int ndims, int nfpts, int nvar
 }
 
 // boundary types of bccflux.mako with bcs/*.mako
-enum BcType { BC_DIFFUSE = 0, BC_SPECULAR = 1, BC_INLET = 2, BC_OUTLET = 3, BC_VANISH = 4, BC_INLET_NS = 5 };
+enum BcType {
+  BC_DIFFUSE = 0, BC_SPECULAR = 1, BC_INLET = 2, BC_OUTLET = 3, BC_VANISH = 4, BC_INLET_NS = 5,
+  BC_DIFFUSE_CYL = 6
+};
+
+// wall of dgfs-wall-diffuse-cyl.mako: per-point wall velocity and
+// temperature st[4][nfpts] = (Ux, Uy, Uz, T) and the unnormalised wall
+// Maxwellian exp(-|c - U|^2 / T) (its normalisation cancels in the flux
+// balance ratio)
+__device__ inline double cyl_wall_f(const double* cv, long vsize, int vv, const double U[3], double T) {
+  const double dx = cv[vv] - U[0], dy = cv[vsize + vv] - U[1], dz = cv[2 * vsize + vv] - U[2];
+  return exp(-(dx * dx + dy * dy + dz * dz) / T);
+}
 
 struct BcArgs {
   int bctype, ndims, nfpts, nvars, vidx, vchunk, vsize;
@@ -166,7 +178,7 @@ struct BcArgs {
   const long* loff;
   const double* nl;
   const double* magnl;
-  const double* f0;       // wall/inlet Maxwellian [vsize]
+  const double* f0;       // wall/inlet Maxwellian [vsize]; cylinder wall: st[4][nfpts]
   const double* bcvals;   // diffuse: [2][nfpts]; specular: stored ul [vsize][nfpts]
   const int* rmap;        // specular reflection map [vsize]
   const double* cv;
@@ -182,21 +194,28 @@ __global__ __launch_bounds__(256) void k_bccflux(BcArgs a) {
   double n[3] = {0, 0, 0};
   for (int d = 0; d < a.ndims; ++d) n[d] = a.nl[d * static_cast<long>(a.nfpts) + i];
   const double mag = a.magnl[i];
-  double ratio = 0.0;
-  if (a.bctype == BC_DIFFUSE) ratio = -a.bcvals[i] / a.bcvals[a.nfpts + i];
+  double ratio = 0.0, U[3] = {a.U[0], a.U[1], a.U[2]}, Tw = 1.0;
+  if (a.bctype == BC_DIFFUSE || a.bctype == BC_DIFFUSE_CYL) ratio = -a.bcvals[i] / a.bcvals[a.nfpts + i];
+  if (a.bctype == BC_DIFFUSE_CYL) {
+    for (int d = 0; d < 3; ++d) U[d] = a.f0[d * static_cast<long>(a.nfpts) + i];
+    Tw = a.f0[3L * a.nfpts + i];
+  }
   for (int v = v0; v < v1; ++v) {
     const int vv = a.vidx + v;
     double nc = 0.0, fac = 0.0;
     for (int d = 0; d < a.ndims; ++d) {
       const double c = a.cv[d * static_cast<long>(a.vsize) + vv];
       nc += n[d] * c;
-      fac += (c - a.U[d]) * n[d];
+      fac += (c - U[d]) * n[d];
     }
     const double ul = a.L[ol + 32L * v];
     double ur = ul;
     switch (a.bctype) {
       case BC_DIFFUSE:   // dgfs-wall-diffuse.mako:13-17
         ur = (fac < 0) ? a.f0[vv] * ratio : ul;
+        break;
+      case BC_DIFFUSE_CYL:  // dgfs-wall-diffuse-cyl.mako:12-27
+        ur = (fac < 0) ? cyl_wall_f(a.cv, a.vsize, vv, U, Tw) * ratio : ul;
         break;
       case BC_SPECULAR:  // dgfs-wall-specular.mako:12-15
         ur = a.bcvals[static_cast<long>(a.rmap[vv]) * a.nfpts + i];
@@ -221,7 +240,9 @@ __global__ __launch_bounds__(256) void k_bccflux(BcArgs a) {
 
 // diffuse-wall flux balance over the velocities of this batch
 // (dgfs-wall-diffuse-update.mako:12-36): bc0 += sum_{fac>=0} fac ul cw,
-// bc1 += sum_{fac<0} fac f0 cw; reset when vidx == 0.  One block per wall
+// bc1 += sum_{fac<0} fac f0 cw; reset when vidx == 0.  With st set (the
+// cylinder wall, dgfs-wall-diffuse-cyl-update.mako:12-44) U and T are
+// per point and f0 is the unnormalised Maxwellian.  One block per wall
 // point, a tree reduction over the velocity index.
 __global__ __launch_bounds__(256) void k_bccupdate_diffuse(int ndims, int nfpts, int nvars, int vidx,
                                                            const double* __restrict__ L,
@@ -230,20 +251,25 @@ __global__ __launch_bounds__(256) void k_bccupdate_diffuse(int ndims, int nfpts,
                                                            const double* __restrict__ f0,
                                                            double* __restrict__ bcvals, double Ux,
                                                            double Uy, double Uz, double cw,
-                                                           const double* __restrict__ cv, int vsize) {
+                                                           const double* __restrict__ cv, int vsize,
+                                                           const double* __restrict__ st) {
   __shared__ double r0[256], r1[256];
   const int i = blockIdx.x, tid = threadIdx.x;
   const long ol = loff[i];
   double n[3] = {0, 0, 0};
   for (int d = 0; d < ndims; ++d) n[d] = nl[d * static_cast<long>(nfpts) + i];
-  const double U[3] = {Ux, Uy, Uz};
+  double U[3] = {Ux, Uy, Uz}, Tw = 1.0;
+  if (st) {
+    for (int d = 0; d < 3; ++d) U[d] = st[d * static_cast<long>(nfpts) + i];
+    Tw = st[3L * nfpts + i];
+  }
   double s0 = 0.0, s1 = 0.0;
   for (int v = tid; v < nvars; v += 256) {
     const int vv = vidx + v;
     double fac = 0.0;
     for (int d = 0; d < ndims; ++d) fac += (cv[d * static_cast<long>(vsize) + vv] - U[d]) * n[d];
     if (fac >= 0) s0 += fac * L[ol + 32L * v] * cw;
-    else s1 += fac * f0[vv] * cw;
+    else s1 += fac * (st ? cyl_wall_f(cv, vsize, vv, U, Tw) : f0[vv]) * cw;
   }
   r0[tid] = s0;
   r1[tid] = s1;
@@ -430,10 +456,10 @@ extern "C" int dgfs_bccflux(int bctype, int ndims, int nfpts, int nvars, int vid
                             const int* rmap, double Ux, double Uy, double Uz, const double* cv, int vsize,
                             void* stream) {
   if (!L || !loff || !nl || !magnl || !cv) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: null pointer");
-  if (bctype < BC_DIFFUSE || bctype > BC_INLET_NS) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: bctype");
-  if ((bctype == BC_DIFFUSE || bctype == BC_INLET || bctype == BC_INLET_NS) && !f0)
+  if (bctype < BC_DIFFUSE || bctype > BC_DIFFUSE_CYL) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: bctype");
+  if ((bctype == BC_DIFFUSE || bctype == BC_INLET || bctype == BC_INLET_NS || bctype == BC_DIFFUSE_CYL) && !f0)
     return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: this boundary needs f0");
-  if ((bctype == BC_DIFFUSE || bctype == BC_SPECULAR) && !bcvals)
+  if ((bctype == BC_DIFFUSE || bctype == BC_SPECULAR || bctype == BC_DIFFUSE_CYL) && !bcvals)
     return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: this boundary needs bc_vals");
   if (bctype == BC_SPECULAR && !rmap) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccflux: specular needs rmap");
   if (ndims < 1 || ndims > 3 || vidx < 0 || vidx + nvars > vsize)
@@ -458,10 +484,11 @@ extern "C" int dgfs_bccupdate(int bctype, int ndims, int nfpts, int nvars, int v
   if (vidx < 0 || vidx + nvars > vsize) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccupdate: velocity range");
   if (nfpts == 0 || nvars == 0) return DGFS_OK;
   auto st = static_cast<hipStream_t>(stream);
-  if (bctype == BC_DIFFUSE) {
+  if (bctype == BC_DIFFUSE || bctype == BC_DIFFUSE_CYL) {
     if (!f0) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_bccupdate: diffuse wall needs f0");
+    const bool cyl = bctype == BC_DIFFUSE_CYL;
     hipLaunchKernelGGL(k_bccupdate_diffuse, dim3(nfpts), dim3(256), 0, st, ndims, nfpts, nvars, vidx, L, loff,
-                       nl, f0, bcvals, Ux, Uy, Uz, cw, cv, vsize);
+                       nl, cyl ? nullptr : f0, bcvals, Ux, Uy, Uz, cw, cv, vsize, cyl ? f0 : nullptr);
   } else if (bctype == BC_SPECULAR) {
     hipLaunchKernelGGL(k_bccupdate_specular, dim3((nfpts + 255) / 256, nvars), dim3(256), 0, st, nfpts, nvars,
                        vidx, L, loff, bcvals);

@@ -18,7 +18,7 @@ _lib.register({
     "dgfs_residual_norms": [_I, _I, _I, _P, _P, _L, _P, _P, _P],
 })
 
-BC_DIFFUSE, BC_SPECULAR, BC_INLET, BC_OUTLET, BC_VANISH, BC_INLET_NS = range(6)
+BC_DIFFUSE, BC_SPECULAR, BC_INLET, BC_OUTLET, BC_VANISH, BC_INLET_NS, BC_DIFFUSE_CYL = range(7)
 
 
 def ptr(x):

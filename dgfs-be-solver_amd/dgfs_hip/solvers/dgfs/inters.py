@@ -180,3 +180,38 @@ class DGFSOutletBCInters(DGFSBCInters):
 class DGFSVanishBCInters(DGFSBCInters):
     type = "dgfs-vanish"
     bctype = fr.BC_VANISH
+
+
+class DGFSWallDiffuseCylBCInters(DGFSBCInters):
+    """Diffuse wall whose velocity and temperature vary along it
+    (inters.py:446-492, bcs/dgfs-wall-diffuse-cyl*.mako): ux, uy, uz and T
+    are expressions of x, y and [constants], evaluated at every wall flux
+    point; the wall emits the unnormalised Maxwellian exp(-|c - U|^2 / T)
+    scaled by the per-point flux balance."""
+    type = "dgfs-wall-diffuse-cyl"
+    bctype = fr.BC_DIFFUSE_CYL
+
+    def __init__(self, be, ele, faces, cfg, cfgsect):
+        super().__init__(be, ele, faces, cfg, cfgsect)
+        from dgfs_hip.nputil import eval_at_points
+        vm = ele.vm
+        n = self.ninterfpts
+        ploc = self.ploc.reshape(n, ele.ndims)
+        st = np.empty((4, n))
+        for d, k in enumerate(("ux", "uy", "uz")):
+            st[d] = eval_at_points(cfg, cfgsect, k, ploc) / vm.u0()
+        st[3] = eval_at_points(cfg, cfgsect, "T", ploc) / vm.T0()
+        if np.any(st[3] <= 0):
+            raise ValueError(f"{cfgsect}: wall temperature must be positive")
+        self.wall_state = st
+        self._f0 = be.const_array(st.ravel())
+        import torch
+        self._bcvals = torch.zeros(2 * max(n, 1), dtype=torch.float64, device=be.device)
+
+    def update_bc(self, t, vidx):
+        ele, vm = self.ele, self.ele.vm
+        if self.ninterfpts == 0:
+            return
+        fr.bccupdate(self.bctype, ele.ndims, self.ninterfpts, ele.nvars, vidx, ele._scal_fpts, self._loff,
+                     self._nl, self._f0, self._bcvals, self._U, vm.cw(), vm.d_cv(), vm.vsize(),
+                     self.be.stream_ptr)

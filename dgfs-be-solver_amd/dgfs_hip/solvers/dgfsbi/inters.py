@@ -76,6 +76,31 @@ class DGFSBiWallSpecularBCInters(DGFSBiBCInters):
         self.ninterfpts = self.parts[0].ninterfpts
 
 
+class DGFSBiWallDiffuseCylBCInters(DGFSBiBCInters):
+    """inters.py:679-763, bcs/dgfsbi-wall-diffuse-cyl*.mako: the wall of
+    dgfs-wall-diffuse-cyl per species, whose Maxwellian
+    exp(-|c - U|^2 m_p / T) is the single-species one at temperature T / m_p
+    (m_p the species' mass ratio)."""
+    type = "dgfsbi-wall-diffuse-cyl"
+    single = I1.DGFSWallDiffuseCylBCInters
+
+    def __init__(self, be, ele, faces, cfg, cfgsect, vm):
+        self.be = be
+        self.ele = ele
+        self.cfg = cfg
+        self.cfgsect = cfgsect
+        self.vm = vm
+        self.parts = []
+        for m in vm.masses():
+            b = I1.DGFSWallDiffuseCylBCInters(be, ele, faces, cfg, cfgsect)
+            st = b.wall_state.copy()
+            st[3] /= m
+            b.wall_state = st
+            b._f0 = be.const_array(st.ravel())
+            self.parts.append(b)
+        self.ninterfpts = self.parts[0].ninterfpts
+
+
 class DGFSBiInletBCInters(DGFSBiBCInters):
     """inters.py:424-495: inflow Maxwellians with the section's nden<p>."""
     type = "dgfsbi-inlet"

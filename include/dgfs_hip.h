@@ -287,13 +287,17 @@ int dgfs_intcflux(int ndims, int nfpts, int nvars, int vidx, double* L, const lo
                   int vsize, void* stream);
 
 /* Boundary common flux, bctype 0 diffuse wall, 1 specular wall, 2 inlet,
- * 3 outlet, 4 vanish, 5 normal-shock inlet: ghost state then the upwind flux,
- * ul = |n| fn.  f0: wall/inlet Maxwellian [vsize]; bc_vals: diffuse [2][nfpts]
- * flux balance, specular stored states [vsize][nfpts]; rmap: specular
- * reflection map [vsize].  Replaces 'bccflux' (frfs/solvers/dgfs/kernels/
- * bccflux.mako:8-39 with bcs/dgfs-wall-diffuse.mako, dgfs-wall-specular.mako,
- * dgfs-inlet.mako, dgfs-outlet.mako, dgfs-vanish.mako,
- * dgfs-inlet-normalshock.mako; frfs/solvers/dgfs/inters.py:175-443). */
+ * 3 outlet, 4 vanish, 5 normal-shock inlet, 6 diffuse wall with per-point
+ * wall velocity and temperature: ghost state then the upwind flux,
+ * ul = |n| fn.  f0: wall/inlet Maxwellian [vsize], or for bctype 6 the wall
+ * state [4][nfpts] (Ux, Uy, Uz, T, nondimensional) whose unnormalised
+ * Maxwellian exp(-|c-U|^2/T) is formed in the kernel; bc_vals: diffuse
+ * [2][nfpts] flux balance, specular stored states [vsize][nfpts]; rmap:
+ * specular reflection map [vsize].  Replaces 'bccflux' (frfs/solvers/dgfs/
+ * kernels/bccflux.mako:8-39 with bcs/dgfs-wall-diffuse.mako,
+ * dgfs-wall-specular.mako, dgfs-inlet.mako, dgfs-outlet.mako,
+ * dgfs-vanish.mako, dgfs-inlet-normalshock.mako, dgfs-wall-diffuse-cyl.mako;
+ * frfs/solvers/dgfs/inters.py:175-492). */
 int dgfs_bccflux(int bctype, int ndims, int nfpts, int nvars, int vidx, double* L, const long* loff,
                  const double* nl, const double* magnl, const double* f0, const double* bc_vals,
                  const int* rmap, double Ux, double Uy, double Uz, const double* cv, int vsize,
@@ -301,9 +305,11 @@ int dgfs_bccflux(int bctype, int ndims, int nfpts, int nvars, int vidx, double* 
 
 /* Boundary update over one velocity batch: diffuse wall accumulates
  * bc0 += sum_{(c-U).n>=0} (c-U).n ul cw, bc1 += sum_{<0} (c-U).n f0 cw (reset
- * when vidx == 0); specular stores ul.  Replaces 'bccupdate'
+ * when vidx == 0); bctype 6 does the same with the per-point wall state of
+ * dgfs_bccflux; specular stores ul.  Replaces 'bccupdate'
  * (frfs/solvers/dgfs/kernels/bccupdate.mako:8-24, bcs/dgfs-wall-diffuse-
- * update.mako:4-40, dgfs-wall-specular-update.mako:4-16). */
+ * update.mako:4-40, dgfs-wall-diffuse-cyl-update.mako:4-48,
+ * dgfs-wall-specular-update.mako:4-16). */
 int dgfs_bccupdate(int bctype, int ndims, int nfpts, int nvars, int vidx, const double* L,
                    const long* loff, const double* nl, const double* f0, double* bc_vals, double Ux,
                    double Uy, double Uz, double cw, const double* cv, int vsize, void* stream);

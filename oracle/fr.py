@@ -178,8 +178,11 @@ def maxwellian(cv, rho, u, T):
 class AdvectionOracle:
     """FP64 advection rhs of a DGFS state f[upt][v][e] on a RectMesh.
 
-    walls: {name: (U (3,), f_w (vsize,))} diffuse walls; faces without a
-    neighbour must be listed."""
+    walls: {name: (U (3,), f_w (vsize,))} diffuse walls, or {name: fn} for
+    a wall whose velocity and temperature vary along it
+    (bcs/dgfs-wall-diffuse-cyl.mako): fn(xy (n, 2)) -> (U (n, 3), T (n,)),
+    emitting exp(-|c - U|^2 / T) with the same flux balance; faces without
+    a neighbour must be listed."""
 
     def __init__(self, p, mesh: RectMesh, cv, cw, walls=None):
         self.fr = fr = QuadFR(p)
@@ -226,12 +229,28 @@ class AdvectionOracle:
                     rows2 = np.arange(f2 * n, (f2 + 1) * n)
                     ur = uf[rows2, :, e2]
                 else:
-                    U, fw = self.walls[mesh.boundary_name(e, fc)]
-                    fac = nrm[0] * (cv[0] - U[0]) + nrm[1] * (cv[1] - U[1])
-                    num = np.sum(np.where(fac >= 0, fac * ul * self.cw, 0.0), axis=1)
-                    den = np.sum(np.where(fac < 0, fac * fw * self.cw, 0.0))
-                    ur = np.where(fac[None, :] < 0, fw[None, :] * (-num[:, None] / den), ul)
+                    wall = self.walls[mesh.boundary_name(e, fc)]
+                    if callable(wall):
+                        xy = mesh.upts_xy(fr.fpts[rows])[:, e]
+                        Up, Tp = wall(xy)
+                        ur = np.empty_like(ul)
+                        for r in range(n):
+                            fw = np.exp(-np.sum((cv - Up[r][:, None]) ** 2, axis=0) / Tp[r])
+                            ur[r] = self._diffuse_ghost(nrm, Up[r], fw, ul[r:r + 1])[0]
+                    else:
+                        U, fw = wall
+                        ur = self._diffuse_ghost(nrm, U, fw, ul)
                 fn = 0.5 * nc * (ul + ur) + 0.5 * np.abs(nc) * (ul - ur)
                 Fn_com[rows, :, e] = mag * fn
         corr = np.einsum("pf,fve->pve", self.M3, Fn_com - Fn_int)
         return -(div + corr) / self.djac
+
+    def _diffuse_ghost(self, nrm, U, fw, ul):
+        """Ghost state of a diffuse wall for wall-side states ul (rows, vsize):
+        the wall Maxwellian on incoming velocities, scaled so that the
+        normal mass flux through every point vanishes."""
+        cv = self.cv
+        fac = nrm[0] * (cv[0] - U[0]) + nrm[1] * (cv[1] - U[1])
+        num = np.sum(np.where(fac >= 0, fac * ul * self.cw, 0.0), axis=1)
+        den = np.sum(np.where(fac < 0, fac * fw * self.cw, 0.0))
+        return np.where(fac[None, :] < 0, fw[None, :] * (-num[:, None] / den), ul)

@@ -157,3 +157,45 @@ def test_bi_species_number_conserved(device):
         ch = rel(f, eq[p])
         say("  species", p, "equilibrium change", ch)
         assert ch < 1e-3
+
+
+def test_bi_cyl_wall_vs_oracle(device):
+    """dgfsbi-wall-diffuse-cyl (inters.py:679-763): per-point wall velocity
+    and temperature from expressions of x, each species emitting
+    exp(-m_p |c - U|^2 / T), against the oracle's per-point diffuse wall
+    at temperature T / m_p (1e-12 per species)."""
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import U0, bi_couette_cfg
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfsbi.system import DGFSBiSystem
+    cfg = bi_couette_cfg(Nv=NV, L=LV, Nrho=4, M=12, Mpre=24, masses=MASSES, nden=(0.6, 0.4),
+                         omega=0.5, alpha=1.2, prefactor=0.05, order=ORDER, uwall=0.3)
+    cfg.set("constants", "uw", 0.2 * U0)
+    sect = "soln-bcs-bottom"
+    cfg.set(sect, "type", "dgfsbi-wall-diffuse-cyl")
+    cfg.set(sect, "ux", "uw*cos(2*pi*x/1.5)")
+    cfg.set(sect, "uy", "0")
+    cfg.set(sect, "uz", "0")
+    cfg.set(sect, "T", "1.1 - 0.1*x")
+    be = HIPBackend(cfg)
+    sysm = DGFSBiSystem(be, rect_quads(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True), cfg, 4)
+    ov = V.BiVelocityMesh(NV, LV, 4, 12, MASSES)
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    f0 = initial(ov, 13)
+    for p in range(2):
+        m = MASSES[p]
+
+        def wall(xy, m=m):
+            U = np.zeros((len(xy), 3))
+            U[:, 0] = 0.2 * np.cos(2 * np.pi * xy[:, 0] / 1.5)
+            return U, (1.1 - 0.1 * xy[:, 0]) / m
+
+        top = (np.array([0.3, 0, 0]), V.species_maxwellian(ov, p, 1.0, [0.3, 0, 0], 1.0))
+        orc = OF.AdvectionOracle(ORDER, omesh, ov.cv, ov.cw, {"bottom": wall, "top": top})
+        sysm.ele.scal_upts[0].set(f0[p])
+        sysm.updatebc(0.0, p, 0, 1, 0, 0)
+        sysm.rhs(0.0, p, 0, 1, 0, 0)
+        be.synchronize()
+        err = rel(sysm.ele.scal_upts[1].get(), orc.rhs(f0[p]))
+        say("bi cyl wall species", p, "rel L2", err)
+        assert err < 1e-12

@@ -254,3 +254,87 @@ def test_residual_plugin(device):
         want = math.sqrt(np.sum((seen[j] - seen[i]) ** 2) / np.sum(seen[i] ** 2))
         say("  ", pl.rows[k][1], want)
         assert abs(pl.rows[k][1] - want) <= 1e-12 * want
+
+
+def _channel_oracle(vm, bottom_wall=None):
+    ov = O.VelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M())
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    walls = {"bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+             "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))}
+    if bottom_wall is not None:
+        walls["bottom"] = bottom_wall
+    return ov, OF.AdvectionOracle(2, omesh, ov.cv, ov.cw, walls)
+
+
+def _perturbed(ov, seed):
+    rng = np.random.default_rng(seed)
+    return OF.maxwellian(ov.cv, 1.0, [0.1, -0.05, 0], 0.95)[None, :, None] * \
+        (1 + 0.1 * rng.random((9, 1, 6))) * (1 + 0.05 * rng.random((1, ov.n, 1)))
+
+
+def _rhs_of(be, sysm, f):
+    sysm.ele.scal_upts[0].set(f)
+    sysm.updatebc(0.0, 0, 1, 0, 0)
+    sysm.rhs(0.0, 0, 1, 0, 0)
+    be.synchronize()
+    return sysm.ele.scal_upts[1].get()
+
+
+def test_cyl_wall_vs_oracle(device):
+    """dgfs-wall-diffuse-cyl (inters.py:446-492): a wall whose velocity and
+    temperature are expressions of x, against the oracle's per-point diffuse
+    wall (1e-12)."""
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import U0, couette_cfg
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    cfg = couette_cfg(Nv=16, L=5.0, Nrho=4, M=6, Kn=1.0, omega=0.5, order=2, uwall=0.3)
+    cfg.set("constants", "uw", 0.3 * U0)
+    sect = "soln-bcs-bottom"
+    cfg.set(sect, "type", "dgfs-wall-diffuse-cyl")
+    cfg.set(sect, "ux", "uw*sin(2*pi*x/1.5)")
+    cfg.set(sect, "uy", "0")
+    cfg.set(sect, "uz", "0")
+    cfg.set(sect, "T", "1 + 0.2*x")
+    be = HIPBackend(cfg)
+    sysm = DGFSSystem(be, rect_quads(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True), cfg, 3)
+
+    def wall(xy):
+        U = np.zeros((len(xy), 3))
+        U[:, 0] = 0.3 * np.sin(2 * np.pi * xy[:, 0] / 1.5)
+        return U, 1.0 + 0.2 * xy[:, 0]
+
+    ov, orc = _channel_oracle(sysm.vm, wall)
+    f = _perturbed(ov, 11)
+    say("cyl wall p=2 Nv=16")
+    r = _rhs_of(be, sysm, f)
+    err = rel(r, orc.rhs(f))
+    say("  rel L2", err)
+    assert err < 1e-12
+
+
+def test_gmsh_mesh_system_vs_oracle(device, tmp_path):
+    """A channel read from a gmsh file (periodic x pair, named walls) runs
+    the same advection as the oracle on the structured mesh (1e-12)."""
+    from test_gmsh import write_msh
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import couette_cfg
+    from dgfs_hip.gmsh import read_gmsh
+    from dgfs_hip.mesh import rect_quads
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    walls = rect_quads(3, 2, 0.0, 1.5, 0.0, 1.0)
+    path = tmp_path / "channel.msh"
+    path.write_text("\n".join(write_msh(walls, ["bottom", "top", "left", "right"],
+                                        rename={"left": "periodic_0_r", "right": "periodic_0_l"},
+                                        shuffle_seed=1)) + "\n")
+    mesh = read_gmsh(str(path))
+    cfg = couette_cfg(Nv=16, L=5.0, Nrho=4, M=6, Kn=1.0, omega=0.5, order=2, uwall=0.3)
+    be = HIPBackend(cfg)
+    sysm = DGFSSystem(be, mesh, cfg, 3)
+    ov, orc = _channel_oracle(sysm.vm)
+    f = _perturbed(ov, 12)
+    say("gmsh channel p=2 Nv=16")
+    r = _rhs_of(be, sysm, f)
+    err = rel(r, orc.rhs(f))
+    say("  rel L2", err)
+    assert err < 1e-12

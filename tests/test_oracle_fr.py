@@ -103,6 +103,66 @@ def test_diffuse_wall_zero_net_mass_flux():
     assert abs(mass) < 1e-12 * np.abs(r).sum() * vm.cw
 
 
+def _cyl_state(vm, seed):
+    rng = np.random.default_rng(seed)
+    return OF.maxwellian(vm.cv, 1.0, [0.1, 0, 0], 0.9)[None, :, None] * (1 + 0.2 * rng.random((9, 1, 6)))
+
+
+def test_cyl_wall_constant_equals_diffuse():
+    """The unnormalised wall Maxwellian of the varying wall with constant U, T
+    gives the diffuse wall exactly: the normalisation cancels in the balance."""
+    vm = _vm(12, 5.0)
+    mesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1, periodic_x=True)
+    U0, U1 = np.array([0.3, 0, 0]), np.array([-0.2, 0, 0])
+    top = (U1, OF.maxwellian(vm.cv, 1.0, U1, 1.3))
+    a = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw,
+                           walls={"bottom": (U0, OF.maxwellian(vm.cv, 1.0, U0, 0.8)), "top": top})
+    b = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw, walls={
+        "bottom": lambda xy: (np.tile(U0, (len(xy), 1)), np.full(len(xy), 0.8)), "top": top})
+    f = _cyl_state(vm, 5)
+    ra, rb = a.rhs(f), b.rhs(f)
+    assert np.abs(ra - rb).max() < 1e-13 * np.abs(ra).max()
+
+
+def test_cyl_wall_uses_the_point_location():
+    """A wall velocity that is piecewise constant in x gives, element by
+    element, the rhs of the constant wall that holds there."""
+    vm = _vm(12, 5.0)
+    mesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1, periodic_x=True)
+    top = (np.zeros(3), OF.maxwellian(vm.cv, 1.0, [0, 0, 0], 1.0))
+    Ua, Ub = np.array([0.3, 0, 0]), np.array([-0.3, 0, 0])
+
+    def wall(xy):
+        U = np.where((xy[:, 0] < 0.5)[:, None], Ua, Ub)
+        return U, np.where(xy[:, 0] < 0.5, 1.0, 1.2)
+
+    f = _cyl_state(vm, 6)
+    r = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw, walls={"bottom": wall, "top": top}).rhs(f)
+    ra = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw,
+                            walls={"bottom": (Ua, OF.maxwellian(vm.cv, 1, Ua, 1.0)), "top": top}).rhs(f)
+    rb = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw,
+                            walls={"bottom": (Ub, OF.maxwellian(vm.cv, 1, Ub, 1.2)), "top": top}).rhs(f)
+    # bottom row: element 0 lies at x < 0.5, elements 1 and 2 beyond
+    assert np.abs(r[..., 0] - ra[..., 0]).max() < 1e-13 * np.abs(ra).max()
+    assert np.abs(r[..., 1:3] - rb[..., 1:3]).max() < 1e-13 * np.abs(rb).max()
+    assert np.abs(r[..., 0] - rb[..., 0]).max() > 1e-3 * np.abs(rb).max()
+
+
+def test_cyl_wall_zero_net_mass_flux():
+    vm = _vm(12, 5.0)
+    mesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1, periodic_x=True)
+
+    def wall(xy):
+        U = np.zeros((len(xy), 3))
+        U[:, 0] = 0.3 * np.sin(2 * np.pi * xy[:, 0] / 1.5)
+        return U, 1.0 + 0.2 * xy[:, 0]
+
+    orc = OF.AdvectionOracle(2, mesh, vm.cv, vm.cw, walls={"bottom": wall, "top": wall})
+    r = orc.rhs(_cyl_state(vm, 7))
+    mass = np.einsum("p,pve->", orc.fr.W, r) * orc.djac * vm.cw
+    assert abs(mass) < 1e-12 * np.abs(r).sum() * vm.cw
+
+
 def test_wall_equilibrium_rest():
     vm = _vm(12, 5.0)
     mesh = OF.RectMesh(2, 2, 0, 1, 0, 1, periodic_x=True)
