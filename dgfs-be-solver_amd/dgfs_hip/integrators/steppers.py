@@ -64,6 +64,12 @@ class BaseDGFSStepper:
     def soln_bank(self):
         return self._idxcurr
 
+    def restart(self, initsoln):
+        """Resume the clock of a solution file: tcurr and the accepted step
+        count (integrators/base.py:35-38)."""
+        from dgfs_hip.native import restart_stats
+        self.tcurr, self.nacptsteps = restart_stats(initsoln)
+
 
 class StdEulerDGFSStepper(BaseDGFSStepper):
     stepper_name = "dgfs-euler"

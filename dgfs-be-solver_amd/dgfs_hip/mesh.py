@@ -19,6 +19,23 @@ class Mesh:
     def neles(self):
         return self.nodes.shape[1]
 
+    @property
+    def uuid(self):
+        """Identity of the mesh for solution files (the reference's
+        mesh_uuid): a name-based UUID of the element type, nodes,
+        interfaces and boundaries, so equal meshes share it."""
+        import hashlib
+        import uuid
+        h = hashlib.sha256(self.eletype.encode())
+        h.update(np.ascontiguousarray(self.nodes, dtype=float).tobytes())
+        for (eL, fL, eR, fR, sh) in self.interfaces:
+            h.update(np.array([eL, fL, eR, fR], dtype=np.int64).tobytes())
+            h.update(np.asarray(sh, dtype=float).tobytes())
+        for name in sorted(self.boundaries):
+            h.update(name.encode())
+            h.update(np.array(self.boundaries[name], dtype=np.int64).tobytes())
+        return str(uuid.UUID(bytes=h.digest()[:16], version=5))
+
 
 def rect_quads(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic_x=False, periodic_y=False):
     """Quad faces: 0 bottom, 1 right, 2 top, 3 left; element e = ix + nx*iy.

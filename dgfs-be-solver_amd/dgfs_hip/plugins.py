@@ -6,6 +6,9 @@ sum ||f_{n-1}||^2) over the full-velocity registers (previous = the state
 one step earlier).  The reference copies both registers to the host; here
 the previous state is a device copy and the two sums come from one HIP
 reduction (dgfs_residual_norms), so only two doubles cross to the host.
+
+DGFSDistWriterStdPlugin writes the solution files that a restart reads
+(dgfs_hip.native).
 """
 import math
 
@@ -56,3 +59,46 @@ class DGFSResidualStdPlugin:
         if (n + 1) % self.nsteps == 0:
             st.backend.synchronize()
             self._prev = [r.data.clone() for r in self._regs()]
+
+
+class DGFSDistWriterStdPlugin:
+    """Distribution writer (plugins/dgfsdistwriterstd.py:8-80 and
+    dgfsdistwriterbi.py for several species): every dt_out of simulated
+    time it writes the full distribution of the current solution register
+    to a numbered solution file with the config, the stats (fields, prefix,
+    tcurr, nsteps) and the mesh's uuid, the input of a restart.  Writes the
+    initial state unless restarting."""
+    name = "dgfsdistwriterstd"
+    systems = ["dgfs", "dgfsbi"]
+    formulations = ["std"]
+
+    def __init__(self, stepper, dt_out, basedir, basename, isrestart=False, container="auto"):
+        from dgfs_hip.native import NativeWriter
+        self.stepper = stepper
+        self.dt_out = float(dt_out)
+        self.writer = NativeWriter(basedir, basename, prefix="soln", isrestart=isrestart,
+                                   container=container)
+        # output times are met to within a few ulps of the accumulated time
+        # (the reference's plugins use 5 dt_min, base.py)
+        self.tol = 5e-12 * max(1.0, abs(stepper.tcurr) + self.dt_out)
+        self.tout_next = stepper.tcurr
+        self.paths = []
+        if not isrestart:
+            self(stepper)
+        else:
+            self.tout_next += self.dt_out
+
+    def __call__(self, stepper=None):
+        from dgfs_hip.inifile import Inifile
+        st = self.stepper
+        if abs(self.tout_next - st.tcurr) > self.tol:
+            return
+        sysm = st.system
+        stats = Inifile()
+        stats.set("data", "fields", sysm.dist_fields())
+        stats.set("data", "prefix", "soln")
+        stats.set("solver-time-integrator", "tcurr", repr(float(st.tcurr)))
+        stats.set("solver-time-integrator", "nsteps", st.nacptsteps)
+        metadata = dict(config=sysm.cfg.tostr(), stats=stats.tostr(), mesh_uuid=sysm.mesh.uuid)
+        self.paths.append(self.writer.write(sysm.soln_arrays(st.soln_bank), metadata, st.tcurr))
+        self.tout_next = st.tcurr + self.dt_out

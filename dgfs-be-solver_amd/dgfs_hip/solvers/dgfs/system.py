@@ -6,6 +6,8 @@ interfaces, the scattering model and the full-velocity registers
 NvBatchSize velocities; rhs/updatebc act on the batch starting at varidx,
 collide on the full registers, exactly as the reference's steppers expect.
 """
+from collections.abc import Mapping
+
 import numpy as np
 
 from dgfs_hip.shapes import get_shape
@@ -55,7 +57,14 @@ class DGFSSystem:
 
         # full-velocity registers
         nupts, neles, vsize = self.ele.nupts, self.ele.neles, vm.vsize()
-        if initsoln is not None:
+        if isinstance(initsoln, Mapping):
+            # a solution file (system.py:79-108): same fields and mesh,
+            # interpolated onto this order's points
+            from dgfs_hip.native import dist_fields, soln_for_system
+            if initsoln.get("mesh_uuid", mesh.uuid) != mesh.uuid:
+                raise RuntimeError("Invalid solution for mesh.")
+            f0 = soln_for_system(initsoln, cfg, self.ele, dist_fields(vsize), et)
+        elif initsoln is not None:
             f0 = np.asarray(initsoln, dtype=float).reshape(nupts, vsize, neles)
         else:
             icn = cfg.get("soln-ics", "type")
@@ -88,6 +97,15 @@ class DGFSSystem:
         """Q at every solution point: one batched call (system.py:199-218)."""
         fsoln = self.eles_scal_upts_full
         self.sm.fs_batch(fsoln[uinbank], fsoln[foutbank])
+
+    def soln_arrays(self, bank):
+        """{etype: (nupts, vsize, neles)} of full register `bank`, the
+        arrays of a distribution file (dgfsdistwriterstd.py:59-61)."""
+        return {self.ele_types[0]: self.eles_scal_upts_full[bank].get()}
+
+    def dist_fields(self):
+        from dgfs_hip.native import dist_fields
+        return dist_fields(self.vm.vsize())
 
     def moments(self, bank):
         """(nupts, 15, neles): rho, Ux, Uy, Uz, T, qx, qy, qz, Pxx, Pxy, Pxz,

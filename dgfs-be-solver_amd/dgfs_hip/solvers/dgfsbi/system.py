@@ -10,6 +10,8 @@ species p is bank index p * nregs0 + r; the element batch registers follow
 the same numbering.  rhs/updatebc take the species index `idx` (a string in
 the reference's steppers) to pick that species' boundary kernels.
 """
+from collections.abc import Mapping
+
 import numpy as np
 
 from dgfs_hip.shapes import get_shape
@@ -73,7 +75,12 @@ class DGFSBiSystem:
         nupts, neles, vsize = self.ele.nupts, self.ele.neles, vm.vsize()
         if initsoln is not None:
             # (nupts, nspcs * vsize, neles), species-major as the reference's
-            # solution files (system.py:116-125)
+            # solution files (system.py:88-135)
+            if isinstance(initsoln, Mapping):
+                from dgfs_hip.native import dist_fields, soln_for_system
+                if initsoln.get("mesh_uuid", mesh.uuid) != mesh.uuid:
+                    raise RuntimeError("Invalid solution for mesh.")
+                initsoln = soln_for_system(initsoln, cfg, self.ele, dist_fields(vsize, nspcs), mesh.eletype)
             data = np.asarray(initsoln, dtype=float).reshape(nupts, nspcs * vsize, neles)
             f0 = [data[:, p * vsize:(p + 1) * vsize, :] for p in range(nspcs)]
         else:
@@ -116,6 +123,16 @@ class DGFSBiSystem:
             raise ValueError("collide needs one input and one output register per species")
         fs = self.eles_scal_upts_full
         self.sm.fs_batch([fs[r] for r in regids[:S]], [fs[r] for r in regids[S:]])
+
+    def soln_arrays(self, bank=0):
+        """{etype: (nupts, nspcs * vsize, neles)}: register `bank` of every
+        species side by side (dgfsdistwriterbi.py:59-76)."""
+        return {self.mesh.eletype: np.concatenate(
+            [self.species_register(p, bank).get() for p in range(self._nspcs)], axis=1)}
+
+    def dist_fields(self):
+        from dgfs_hip.native import dist_fields
+        return dist_fields(self.vm.vsize(), self._nspcs)
 
     def species_register(self, p, r):
         return self.eles_scal_upts_full[p * self.nregs0 + r]

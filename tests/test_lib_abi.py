@@ -39,6 +39,7 @@ def test_binding_covers_declared_symbols():
     from dgfs_hip import surrogate  # noqa: F401  registers the surrogate entry points
     from dgfs_hip.integrators import graphs  # noqa: F401  registers the graph entry points
     from dgfs_hip.solvers.dgfsbi import system, initcond, inters  # noqa: F401
+    from dgfs_hip import gmsh, native, nputil, plugins  # noqa: F401
     assert declared_symbols() <= set(_lib.exported_symbols())
 
 
