@@ -65,6 +65,12 @@ def parse():
                          "3 register-resident with in-kernel Nyquist planes")
     ap.add_argument("--groups", type=int, default=0,
                     help="radial groups per point of the FP32 gain launch (0: the library's choice)")
+    ap.add_argument("--vss-points", type=int, default=2048,
+                    help="two-species mixture batch (0 skips): points per species")
+    ap.add_argument("--vss-steps", type=int, default=3)
+    ap.add_argument("--vss-nv", type=int, default=24)
+    ap.add_argument("--vss-L", type=float, default=6.0)
+    ap.add_argument("--vss-cpu-points", type=int, default=8)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -158,6 +164,126 @@ def bench_rhs(args, dev):
     gs.close()
     del sysm, be
     torch.cuda.empty_cache()
+    return out
+
+
+def vss_bytes(nv, K, S, c):
+    """Algorithmic HBM bytes of one mixture point in the reference dataflow
+    (dgfsbi/scattering.py:259-330): per ordered pair 3K transforms (two
+    inverse, one forward per (radial, direction) item), per species the
+    forward transform of f_p and the inverse of its gain and loss, each
+    reading and writing one complex cube of c bytes per value."""
+    n = nv ** 3
+    return 2 * c * n * (3 * K * S * S + 3 * S)
+
+
+def bench_vss(args, dev, stream):
+    """Two-species VSS mixture (mass ratio 2.1, omega 0.81 / 0.805 / 0.80,
+    VHS alpha = 1) on one Nv^3 grid: FP32 + FP64 mixture projection against
+    the plain FP64 path, per-species error, mixture conservation."""
+    import numpy as np
+    import torch
+    from dgfs_hip.solvers.dgfsbi.scattering import VSSOperator
+    from dgfs_hip.solvers.dgfsbi.velocitymesh import DGFSBiVelocityMesh
+    from dgfs_hip.sphericaldesign import get_sphquadrule
+    nv, L, Nrho, M, Mpre, P = args.vss_nv, args.vss_L, 12, 12, 24, args.vss_points
+    masses = (1.0, 2.1)
+    omegas = np.array([[0.81, 0.805], [0.805, 0.80]])
+    vm = DGFSBiVelocityMesh.from_params(None, nv, L, Nrho, M, masses)
+    S, K = 2, Nrho * M
+    gamma, eta, prefac = 2.0 * (1.0 - omegas), np.zeros((S, S)), np.ones((S, S))
+    szpre = get_sphquadrule("symmetric", npts=Mpre).pts
+    cv = torch.as_tensor(vm.cv(), device=dev)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    fs = []
+    for p, m in enumerate(vm.masses()):
+        u = (torch.rand((P, 3, 1, 2), generator=g, dtype=torch.float64) * 2 - 1).to(dev)
+        T = (0.6 + 1.2 * torch.rand((P, 1, 2), generator=g, dtype=torch.float64)).to(dev)
+        c2 = ((cv[None, :, :, None] - u) ** 2).sum(1)                 # (P, n, 2)
+        mx = (m / (math.pi * T)) ** 1.5 * torch.exp(-m * c2 / T)
+        fs.append((0.5 * mx[..., 0] + 0.5 * mx[..., 1]).contiguous())
+    Qs32 = [torch.empty_like(f) for f in fs]
+    ptr = lambda a: [x.data_ptr() for x in a]
+    op32 = VSSOperator(nv, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), vm.sz(), szpre,
+                       vm.masses(), gamma, eta, prefac, precision=32, project=True)
+    op32.eval_contig(ptr(fs), ptr(Qs32), P, stream)
+    torch.cuda.synchronize()
+    op32.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.vss_steps):
+        op32.eval_contig(ptr(fs), ptr(Qs32), P, stream)
+    torch.cuda.synchronize()
+    w32 = time.perf_counter() - t0
+    tim = op32.get_timing()
+    cfgv = op32.get_config()
+    op32.set_timing(False)
+    gain_ms, gain_n = tim["gain"]
+    bq = vss_bytes(nv, K, S, 8)
+    achieved = bq * P * args.vss_steps / (gain_ms / 1e3)
+    out = {
+        "workload": f"two-species VSS mixture: {P} points, Nv={nv} on [-{L:g},{L:g}]^3, M={M}, Nrho={Nrho}, "
+                    f"Mpre={Mpre}, masses 1 / 2.1, omega 0.81 / 0.805 / 0.80, bimodal species states",
+        "unit": "mixture points/s (every species' Q_p at one point)",
+        "value": P * args.vss_steps / w32,
+        "pair_evals_per_s": S * S * P * args.vss_steps / w32,
+        "ms_per_step": 1e3 * w32 / args.vss_steps,
+        "dtype": "f32+fp64-projection",
+        "kernel_ms_per_step": {k: v[0] / args.vss_steps for k, v in tim.items()},
+        "config": cfgv,
+        "roofline": {"bound": "hbm", "kernel": f"k_vss_gain<float,{nv}>", "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "algorithmic_bytes_per_point": bq,
+                     "avg_launch_ms": gain_ms / max(gain_n, 1), "traffic": None,
+                     "note": "reference dataflow with every transform crossing HBM once; the kernel "
+                             "keeps them in LDS/registers"},
+    }
+    op32.close()
+    op64 = VSSOperator(nv, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), vm.sz(), szpre,
+                       vm.masses(), gamma, eta, prefac, precision=64, project=False)
+    Qs64 = [torch.empty_like(f) for f in fs]
+    op64.eval_contig(ptr(fs), ptr(Qs64), P, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op64.eval_contig(ptr(fs), ptr(Qs64), P, stream)
+    torch.cuda.synchronize()
+    w64 = time.perf_counter() - t0
+    op64.close()
+    op64p = VSSOperator(nv, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), vm.sz(), szpre,
+                        vm.masses(), gamma, eta, prefac, precision=64, project=True)
+    Qs64p = [torch.empty_like(f) for f in fs]
+    op64p.eval_contig(ptr(fs), ptr(Qs64p), P, stream)
+    torch.cuda.synchronize()
+    op64p.close()
+    out["fp64_plain"] = {"value": P / w64, "ms_per_step": 1e3 * w64}
+    out["fp32_speedup_vs_fp64"] = out["value"] / out["fp64_plain"]["value"]
+    out["fp32_vs_fp64_rel_l2_per_species"] = [
+        float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(Qs32, Qs64p)]
+    # mixture invariants of the FP32 result: species masses, total momentum, total energy
+    w = vm.cw()
+    mass = max(float(((q.sum(1) * w).abs() / (q.abs().sum(1) * w)).max()) for q in Qs32)
+    scale = sum(m * q.abs().sum(1) * w for m, q in zip(vm.masses(), Qs32))
+    mom = sum(m * (q @ cv.T) * w for m, q in zip(vm.masses(), Qs32))            # (P, 3)
+    en = sum(m * (q @ (cv ** 2).sum(0)) * w for m, q in zip(vm.masses(), Qs32))
+    out["conservation_after_projection_max_rel"] = {
+        "species_mass": mass, "momentum": float((mom.abs().max(1).values / scale).max()),
+        "energy": float((en.abs() / scale).max())}
+    if args.vss_cpu_points > 0 and not args.no_cpu:
+        from oracle import vss as V
+        ov = V.BiVelocityMesh(nv, L, Nrho, M, masses)
+        pairs = [(p, q) for p in range(S) for q in range(S)]
+        orc = V.VSSOracle(ov, {k: gamma[k] for k in pairs}, {k: 0.0 for k in pairs},
+                          {k: 1.0 for k in pairs}, szpre)
+        k = args.vss_cpu_points
+        sample = [f[:k].cpu().numpy() for f in fs]
+        t0 = time.perf_counter()
+        qo = orc.q(sample)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / tc, "unit": out["unit"], "cores": 1, "kind": "port",
+                               "sample": f"FP64 numpy/scipy.fft restatement of the reference dataflow on "
+                                         f"the first {k} point(s) of the same batch, one core"}
+        out["fp64_gpu_vs_oracle_rel_l2"] = [
+            float(np.linalg.norm(q64[:k].cpu().numpy() - o) / np.linalg.norm(o)) for q64, o in zip(Qs64, qo)]
+    del Qs32, Qs64, Qs64p, fs
     return out
 
 
@@ -686,6 +812,9 @@ def main():
         torch.cuda.empty_cache()
     if args.rhs_steps > 0:
         line["rhs"] = bench_rhs(args, dev)
+    if args.vss_points > 0:
+        line["vss_mixture"] = bench_vss(args, dev, stream)
+        torch.cuda.empty_cache()
 
     if not args.no_cpu:
         from oracle import vhs as O
