@@ -23,6 +23,16 @@
 // after item, so they live in L2 / Infinity Cache.  The work items of all
 // species pairs go into one launch.
 //
+// Antipodal sharing.  With a = (pi/L) qz_x (s_y . l) odd in s_y, the product
+// of pair (q, p) at direction -s_y is the product of pair (p, q) at s_y, and
+// the product of a pair (p, p) is even in s_y.  When the design holds every
+// direction's antipode, one product per (x, y) therefore serves G^_pq with
+// b_pq(x, y) and G^_qp with b_qp(x, -y), and a self pair runs over one
+// direction of every antipodal pair with b(x, y) + b(x, -y): the transform
+// count per point drops from 3 K S^2 to 3 K S (S + 1) / 2 ... 3 K S^2 / 2.
+// The work is listed as jobs: a job names the two spectra, the phases and
+// one or two (table, G^ output) pairs with their direction rows.
+//
 // Finalize: Re IFFT(Z) = IFFT(H(Z)) with H the Hermitian part,
 // H(Z)(k) = (Z(k) + conj Z(-k))/2, so one complex transform of
 // sum_q prefac_pq [H(G^_pq) + i H(c_pq G_q)] yields the gain (real part) and
@@ -101,19 +111,31 @@ __global__ __launch_bounds__(256) void k_vss_ctab(cpx<T>* c, const double* __res
 // ---------------------------------------------------------------------------
 // gain: one work item = (pair, point, radial group)
 // ---------------------------------------------------------------------------
+// one job: products of spectra sp (phase mu1) and sq (phase -mu2) over
+// ndir directions dirs[dir_off + i]; G^_out1 += w tab1[x][rows1[.]] FFT(.),
+// and with out2 >= 0 also G^_out2 += w tab2[x][rows2[.]] FFT(.)
+template <typename T>
+struct VssJob {
+  const cpx<T>* tab1;  // [Nrho][tabM1][n]
+  const cpx<T>* tab2;  // [Nrho][tabM2][n] or null
+  double mu1, mu2;     // m_q/(m_p+m_q), m_p/(m_p+m_q)
+  int sp, sq, out1, out2, tabM1, tabM2, ndir, dir_off;
+};
+
 template <typename T>
 struct VssGainArgs {
   const cpx<T>* F;   // [nspcs][npts][n] spectra, scaled by 1/n
-  cpx<T>* Ghat;      // [npairs][npts][ngroups][n]
+  cpx<T>* Ghat;      // [npairs][npts][ngroups][n], pair index p * nspcs + q
   cpx<T>* Z1;        // scratch [nslots][n]
   cpx<T>* Z2;        // scratch [nslots][n]
   const double* alpha;  // [Nrho] (pi/L) qz_x
   const double* wgt;    // [Nrho] qw_x sw / n
   const double* sz;     // [M][3]
-  const cpx<T>* b[kVssMaxPairs];  // [Nrho*M][n] per pair
-  int sp[kVssMaxPairs], sq[kVssMaxPairs];
-  double mu1[kVssMaxPairs], mu2[kVssMaxPairs];  // m_q/(m_p+m_q), m_p/(m_p+m_q)
-  int nitems, npts, ngroups, ppg, Nrho, M;
+  const int* dirs;      // direction, row-1 and row-2 lists of every job
+  const int* rows1;
+  const int* rows2;
+  VssJob<T> job[kVssMaxPairs];
+  int nitems, npts, ngroups, ppg, Nrho;
 };
 
 template <typename T, int N>
@@ -124,22 +146,27 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
   const int tid = threadIdx.x;
   cpx<T>* Z1 = a.Z1 + static_cast<size_t>(blockIdx.x) * n;
   cpx<T>* Z2 = a.Z2 + static_cast<size_t>(blockIdx.x) * n;
-  const int per_pair = a.npts * a.ngroups;
+  const int per_job = a.npts * a.ngroups;
 
   for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
-    const int pair = item / per_pair, rem = item - pair * per_pair;
+    const int j = item / per_job, rem = item - j * per_job;
     const int pt = rem / a.ngroups, g = rem - pt * a.ngroups;
     const int x0 = g * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
-    const cpx<T>* F1 = a.F + (static_cast<size_t>(a.sp[pair]) * a.npts + pt) * n;
-    const cpx<T>* F2 = a.F + (static_cast<size_t>(a.sq[pair]) * a.npts + pt) * n;
-    const cpx<T>* bp = a.b[pair];
-    const double mu1 = a.mu1[pair], mu2 = a.mu2[pair];
-    cpx<T>* G = a.Ghat + static_cast<size_t>(item) * n;
+    const VssJob<T>& jb = a.job[j];
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const double mu1 = jb.mu1, mu2 = jb.mu2;
+    cpx<T>* G = a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + g) * n;
+    cpx<T>* G2 = (jb.out2 >= 0) ? a.Ghat + ((static_cast<size_t>(jb.out2) * a.npts + pt) * a.ngroups + g) * n
+                                : nullptr;
     for (int i = tid; i < n; i += NT) G[i] = cpx<T>{T(0), T(0)};
+    if (G2)
+      for (int i = tid; i < n; i += NT) G2[i] = cpx<T>{T(0), T(0)};
     for (int x = x0; x < x1; ++x) {
       const double alpha = a.alpha[x];
       const T w = static_cast<T>(a.wgt[x]);
-      for (int y = 0; y < a.M; ++y) {
+      for (int iy = 0; iy < jb.ndir; ++iy) {
+        const int y = a.dirs[jb.dir_off + iy];
         __syncthreads();
         if (tid < 3 * N) {
           const int d = tid / N, k = tid - d * N;
@@ -189,15 +216,29 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
 #pragma unroll
                                     for (int k = 0; k < N; ++k) Z2[k * NN + idx] = v[k];
                                   });
-        const cpx<T>* bxy = bp + static_cast<size_t>(x * a.M + y) * n;
-        for (int kx0 = 0; kx0 < N; kx0 += PB) {
-          const int cnt = min(PB, N - kx0);
-          slab_batch<T, N, NT, -1>(
-              lds, cnt, [&](int j, int r, int c) { return Z2[(kx0 + j) * NN + r * N + c]; },
-              [&](int j, int r, int c, cpx<T> v) {
-                const int o = (kx0 + j) * NN + r * N + c;
-                G[o] = G[o] + (w * bxy[o]) * v;
-              });
+        const cpx<T>* bxy = jb.tab1 + (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n;
+        if (G2) {
+          const cpx<T>* bxy2 = jb.tab2 + (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off + iy]) * n;
+          for (int kx0 = 0; kx0 < N; kx0 += PB) {
+            const int cnt = min(PB, N - kx0);
+            slab_batch<T, N, NT, -1>(
+                lds, cnt, [&](int jj, int r, int c) { return Z2[(kx0 + jj) * NN + r * N + c]; },
+                [&](int jj, int r, int c, cpx<T> v) {
+                  const int o = (kx0 + jj) * NN + r * N + c;
+                  G[o] = G[o] + (w * bxy[o]) * v;
+                  G2[o] = G2[o] + (w * bxy2[o]) * v;
+                });
+          }
+        } else {
+          for (int kx0 = 0; kx0 < N; kx0 += PB) {
+            const int cnt = min(PB, N - kx0);
+            slab_batch<T, N, NT, -1>(
+                lds, cnt, [&](int jj, int r, int c) { return Z2[(kx0 + jj) * NN + r * N + c]; },
+                [&](int jj, int r, int c, cpx<T> v) {
+                  const int o = (kx0 + jj) * NN + r * N + c;
+                  G[o] = G[o] + (w * bxy[o]) * v;
+                });
+          }
         }
       }
     }
@@ -359,6 +400,17 @@ struct VssHandle {
   double mu1[kVssMaxPairs] = {}, mu2[kVssMaxPairs] = {};
   void* b[kVssMaxPairs] = {};
   void* c[kVssMaxPairs] = {};
+  // antipodal sharing: merged self-pair tables [Nrho][M/2][n] and the job lists
+  void* bsym[kVssMaxSpecies] = {};
+  int sym_ok = 0, sym = 0;
+  std::vector<int> anti, half;
+  struct Job {
+    int sp, sq, out1, out2, tab1, tab2, sym1, tabM1, tabM2, ndir, dir_off;
+  };
+  std::vector<Job> jobs_plain, jobs_sym;
+  int* dirs_d = nullptr;
+  int* rows1_d = nullptr;
+  int* rows2_d = nullptr;
   double* alpha = nullptr;
   double* wgt = nullptr;
   double* szd = nullptr;
@@ -385,6 +437,8 @@ struct VssHandle {
       (void)hipFree(b[k]);
       (void)hipFree(c[k]);
     }
+    for (int k = 0; k < kVssMaxSpecies; ++k) (void)hipFree(bsym[k]);
+    (void)hipFree(dirs_d); (void)hipFree(rows1_d); (void)hipFree(rows2_d);
     (void)hipFree(alpha); (void)hipFree(wgt); (void)hipFree(szd);
     (void)hipFree(fc); (void)hipFree(Qc); (void)hipFree(F); (void)hipFree(Ghat);
     (void)hipFree(Z1); (void)hipFree(Z2);
@@ -471,20 +525,23 @@ template <typename T, int N>
 static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Qs, int npts, hipStream_t st) {
   constexpr int n = N * N * N;
   const int slots = h->cap_slots;
+  const std::vector<VssHandle::Job>& jobs = h->sym ? h->jobs_sym : h->jobs_plain;
+  const int njobs = static_cast<int>(jobs.size());
   int groups = h->groups_override;
   if (groups <= 0) {
-    const int base = h->npairs * std::max(npts, 1);
+    const int base = njobs * std::max(npts, 1);
     groups = std::max(1, std::min(h->Nrho, (2 * slots + base - 1) / base));
   }
   const int ppg = (h->Nrho + groups - 1) / groups;
   groups = (h->Nrho + ppg - 1) / ppg;
-  const size_t items = static_cast<size_t>(h->npairs) * npts * groups;
-  if (items > h->cap_items) {
+  const size_t items = static_cast<size_t>(njobs) * npts * groups;
+  const size_t outs = static_cast<size_t>(h->npairs) * npts * groups;
+  if (outs > h->cap_items) {
     (void)hipFree(h->Ghat);
     h->Ghat = nullptr;
     h->cap_items = 0;
-    DGFS_HIP_TRY(hipMalloc(&h->Ghat, items * n * sizeof(cpx<T>)));
-    h->cap_items = items;
+    DGFS_HIP_TRY(hipMalloc(&h->Ghat, outs * n * sizeof(cpx<T>)));
+    h->cap_items = outs;
   }
   h->last_groups = groups;
   h->last_slots = slots;
@@ -510,19 +567,30 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   ga.alpha = h->alpha;
   ga.wgt = h->wgt;
   ga.sz = h->szd;
-  for (int k = 0; k < h->npairs; ++k) {
-    ga.b[k] = static_cast<const cpx<T>*>(h->b[k]);
-    ga.sp[k] = h->sp[k];
-    ga.sq[k] = h->sq[k];
-    ga.mu1[k] = h->mu1[k];
-    ga.mu2[k] = h->mu2[k];
+  ga.dirs = h->dirs_d;
+  ga.rows1 = h->rows1_d;
+  ga.rows2 = h->rows2_d;
+  for (int k = 0; k < njobs; ++k) {
+    const VssHandle::Job& jb = jobs[k];
+    VssJob<T>& o = ga.job[k];
+    o.tab1 = static_cast<const cpx<T>*>(jb.sym1 ? h->bsym[jb.tab1] : h->b[jb.tab1]);
+    o.tab2 = (jb.out2 >= 0) ? static_cast<const cpx<T>*>(h->b[jb.tab2]) : nullptr;
+    o.mu1 = h->masses[jb.sq] / (h->masses[jb.sp] + h->masses[jb.sq]);
+    o.mu2 = h->masses[jb.sp] / (h->masses[jb.sp] + h->masses[jb.sq]);
+    o.sp = jb.sp;
+    o.sq = jb.sq;
+    o.out1 = jb.out1;
+    o.out2 = jb.out2;
+    o.tabM1 = jb.tabM1;
+    o.tabM2 = jb.tabM2;
+    o.ndir = jb.ndir;
+    o.dir_off = jb.dir_off;
   }
   ga.nitems = static_cast<int>(items);
   ga.npts = npts;
   ga.ngroups = groups;
   ga.ppg = ppg;
   ga.Nrho = h->Nrho;
-  ga.M = h->M;
   {
     KTimerT<VssHandle> t1(h, 1, st);
     hipLaunchKernelGGL((k_vss_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
@@ -694,6 +762,89 @@ static int vss_build_tables(VssHandle* h, const dgfs_vss_desc* d) {
     // the host arrays are reused for the next pair
     DGFS_HIP_TRY(hipDeviceSynchronize());
   }
+  if (h->sym_ok) {
+    // b(x, y) + b(x, -y) for the self pairs: the sum of the two rows of B
+    // taken in FP64 before the table is formed
+    const int Mh = static_cast<int>(h->half.size());
+    std::vector<double> Bh(static_cast<size_t>(Mh) * Mpre);
+    for (int p = 0; p < h->nspcs; ++p) {
+      const double eta = d->eta[p * h->nspcs + p], gamma = d->gamma[p * h->nspcs + p];
+      for (int i = 0; i < Mh; ++i)
+        for (int j = 0; j < Mpre; ++j) {
+          double v = 0.0;
+          for (const int y : {h->half[i], h->anti[h->half[i]]}) {
+            const double dot = d->sz[3 * y] * d->szpre[3 * j] + d->sz[3 * y + 1] * d->szpre[3 * j + 1] +
+                               d->sz[3 * y + 2] * d->szpre[3 * j + 2];
+            v += std::pow(std::max(0.0, 1.0 + dot), eta);
+          }
+          Bh[static_cast<size_t>(i) * Mpre + j] = v;
+        }
+      DGFS_HIP_TRY(hipMemcpy(Byj, Bh.data(), Bh.size() * sizeof(double), hipMemcpyHostToDevice));
+      DGFS_HIP_TRY(hipMalloc(&h->bsym[p], static_cast<size_t>(h->Nrho) * Mh * n * sizeof(cpx<T>)));
+      hipLaunchKernelGGL(k_vss_btab<T>, dim3((n + 255) / 256, h->Nrho * Mh), dim3(256), 0, 0,
+                         static_cast<cpx<T>*>(h->bsym[p]), Byj, szpre, qz, h->L, gamma, 0.5, swpre, N, Mh, Mpre);
+      DGFS_LAUNCH_CHECK("vss_btab_sym");
+      DGFS_HIP_TRY(hipDeviceSynchronize());
+    }
+  }
+  return DGFS_OK;
+}
+
+// antipodes of the design (exact negation) and the two job lists
+static int vss_build_jobs(VssHandle* h, const double* sz) {
+  const int M = h->M, S = h->nspcs;
+  h->anti.assign(M, -1);
+  h->sym_ok = 1;
+  for (int y = 0; y < M && h->sym_ok; ++y) {
+    for (int z = 0; z < M; ++z)
+      if (z != y && sz[3 * z] == -sz[3 * y] && sz[3 * z + 1] == -sz[3 * y + 1] && sz[3 * z + 2] == -sz[3 * y + 2]) {
+        h->anti[y] = z;
+        break;
+      }
+    if (h->anti[y] < 0) h->sym_ok = 0;
+  }
+  h->half.clear();
+  if (h->sym_ok)
+    for (int y = 0; y < M; ++y)
+      if (h->anti[y] > y) h->half.push_back(y);
+  if (h->sym_ok && 2 * static_cast<int>(h->half.size()) != M) h->sym_ok = 0;
+  std::vector<int> dirs, rows1, rows2;
+  auto add = [&](std::vector<VssHandle::Job>& list, VssHandle::Job jb, const std::vector<int>& d,
+                 const std::vector<int>& r1, const std::vector<int>& r2) {
+    jb.ndir = static_cast<int>(d.size());
+    jb.dir_off = static_cast<int>(dirs.size());
+    dirs.insert(dirs.end(), d.begin(), d.end());
+    rows1.insert(rows1.end(), r1.begin(), r1.end());
+    rows2.insert(rows2.end(), r2.begin(), r2.end());
+    list.push_back(jb);
+  };
+  std::vector<int> all(M);
+  for (int y = 0; y < M; ++y) all[y] = y;
+  for (int p = 0; p < S; ++p)
+    for (int q = 0; q < S; ++q) {
+      const int k = p * S + q;
+      add(h->jobs_plain, VssHandle::Job{p, q, k, -1, k, 0, 0, M, 0, 0, 0}, all, all, all);
+    }
+  if (h->sym_ok) {
+    // cross pairs first (a full sweep of directions each), then the self pairs
+    for (int p = 0; p < S; ++p)
+      for (int q = p + 1; q < S; ++q)
+        add(h->jobs_sym, VssHandle::Job{p, q, p * S + q, q * S + p, p * S + q, q * S + p, 0, M, M, 0, 0}, all,
+            all, h->anti);
+    std::vector<int> hrow(h->half.size());
+    for (size_t i = 0; i < hrow.size(); ++i) hrow[i] = static_cast<int>(i);
+    for (int p = 0; p < S; ++p)
+      add(h->jobs_sym, VssHandle::Job{p, p, p * S + p, -1, p, 0, 1, static_cast<int>(hrow.size()), 0, 0, 0},
+          h->half, hrow, hrow);
+  }
+  h->sym = h->sym_ok;
+  const size_t bytes = dirs.size() * sizeof(int);
+  DGFS_HIP_TRY(hipMalloc(&h->dirs_d, bytes));
+  DGFS_HIP_TRY(hipMalloc(&h->rows1_d, bytes));
+  DGFS_HIP_TRY(hipMalloc(&h->rows2_d, bytes));
+  DGFS_HIP_TRY(hipMemcpy(h->dirs_d, dirs.data(), bytes, hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->rows1_d, rows1.data(), bytes, hipMemcpyHostToDevice));
+  DGFS_HIP_TRY(hipMemcpy(h->rows2_d, rows2.data(), bytes, hipMemcpyHostToDevice));
   return DGFS_OK;
 }
 
@@ -756,7 +907,9 @@ extern "C" int dgfs_vss_create(const dgfs_vss_desc* d, void** out) {
   DGFS_HIP_TRY(hipMemcpy(h->alpha, alpha.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->wgt, wgt.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->szd, d->sz, h->M * 3 * sizeof(double), hipMemcpyHostToDevice));
-  const int rc = (h->prec == 64) ? vss_build_tables<double>(h.get(), d) : vss_build_tables<float>(h.get(), d);
+  int rc = vss_build_jobs(h.get(), d->sz);
+  if (rc) return rc;
+  rc = (h->prec == 64) ? vss_build_tables<double>(h.get(), d) : vss_build_tables<float>(h.get(), d);
   if (rc) return rc;
 
   // Gram matrix of the invariants: rows mass_s (s < S), then m_s c_x, m_s c_y,
@@ -850,6 +1003,23 @@ extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
   if (ngroups < 0 || ngroups > h->Nrho) return fail(DGFS_ERR_INVALID_VALUE, "ngroups must be 0..Nrho");
   h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_set_symmetry(void* p, int on) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (on && !h->sym_ok)
+    return fail(DGFS_ERR_UNSUPPORTED, "the direction design does not hold the antipode of every direction");
+  h->sym = on ? 1 : 0;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_get_symmetry(void* p, int* on) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (!on) return fail(DGFS_ERR_INVALID_VALUE, "null output");
+  *on = h->sym;
   return DGFS_OK;
 }
 

@@ -26,6 +26,8 @@ _lib.register({
     "dgfs_vss_destroy": [_P],
     "dgfs_vss_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "dgfs_vss_set_groups": [_P, _I],
+    "dgfs_vss_set_symmetry": [_P, _I],
+    "dgfs_vss_get_symmetry": [_P, C.POINTER(_I)],
     "dgfs_vss_set_timing": [_P, _I],
     "dgfs_vss_get_timing": [_P, C.POINTER(C.c_double), C.POINTER(_I)],
 })
@@ -76,6 +78,16 @@ class VSSOperator:
 
     def set_groups(self, ngroups):
         _lib.call("dgfs_vss_set_groups", self._h, int(ngroups))
+
+    def set_symmetry(self, on=True):
+        """Antipodal sharing of the gain products (default: on when the design
+        allows it); off runs the reference's per-pair direction loop."""
+        _lib.call("dgfs_vss_set_symmetry", self._h, int(bool(on)))
+
+    def get_symmetry(self):
+        v = C.c_int()
+        _lib.call("dgfs_vss_get_symmetry", self._h, C.byref(v))
+        return bool(v.value)
 
     def get_config(self):
         g, s, t = C.c_int(), C.c_int(), C.c_int()

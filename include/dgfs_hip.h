@@ -172,6 +172,16 @@ int dgfs_vss_eval_contig(void* handle, const double* const* f, double* const* Q,
 int dgfs_vss_destroy(void* handle);
 int dgfs_vss_get_config(void* handle, int* ngroups, int* nslots, int* threads);
 int dgfs_vss_set_groups(void* handle, int ngroups);
+
+/* Antipodal sharing of the gain products (on by default when the direction
+ * design holds the exact negation of every direction): one product per
+ * (radial node, direction) serves both ordered pairs (p, q) and (q, p), and a
+ * self pair runs over half of the directions with the two antipodal table
+ * rows summed.  The result equals the unshared sum up to rounding; on = 0
+ * selects the reference's per-pair loop.  Refines 'gainsum' of
+ * frfs/solvers/dgfsbi/scattering.py:259-320 (no counterpart switch there). */
+int dgfs_vss_set_symmetry(void* handle, int on);
+int dgfs_vss_get_symmetry(void* handle, int* on);
 /* Kernel classes {prep, gain, finalize, project, unused}. */
 int dgfs_vss_set_timing(void* handle, int on);
 int dgfs_vss_get_timing(void* handle, double* ms /*[5]*/, int* counts /*[5]*/);

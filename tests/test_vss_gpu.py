@@ -178,3 +178,55 @@ def test_bad_arguments_raise(device):
         op.eval_contig([f.data_ptr(), f.data_ptr()], [f.data_ptr(), f.data_ptr()], 2)
     with pytest.raises(ValueError):
         op.eval_contig([f.data_ptr()], [f.data_ptr()], 2)
+
+
+def _mixture_states(ov, npts, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for p in range(len(ov.masses)):
+        rows = []
+        for _ in range(npts):
+            u1, u2 = rng.uniform(-0.5, 0.5, 3), rng.uniform(-0.5, 0.5, 3)
+            t1, t2 = rng.uniform(0.7, 1.6, 2)
+            rows.append(0.6 * V.species_maxwellian(ov, p, 1.0, u1, t1) + 0.4 * V.species_maxwellian(ov, p, 1.0, u2, t2))
+        out.append(np.array(rows))
+    return out
+
+
+@pytest.mark.parametrize("masses,precision,tol", [((1.0, 2.1), 64, 1e-13), ((1.0, 2.1, 0.7), 64, 1e-13),
+                                                  ((1.0, 2.1), 32, 1e-5)])
+def test_antipodal_sharing_matches_pair_loop(masses, precision, tol, device):
+    """One product per (x, y) for both (p, q) and (q, p) at -y, and half the
+    directions with summed antipodal rows for (p, p), against the
+    reference's per-pair loop over every direction (same handle)."""
+    S = len(masses)
+    N, L, Nrho, M, Mpre = 8, 5.0, 4, 12, 24
+    omegas = tuple(0.8 + 0.005 * (p + q) for p in range(S) for q in range(S))
+    prefac = tuple(1.0 + 0.1 * k for k in range(S * S))
+    op, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.2, omegas, prefac, precision=precision)
+    assert op.get_symmetry()
+    fs = _mixture_states(V.BiVelocityMesh(N, L, Nrho, M, masses), 6, seed=21)
+    Qs = run_contig(op, fs, device)
+    op.set_symmetry(False)
+    assert not op.get_symmetry()
+    Qp = run_contig(op, fs, device)
+    e = [rel(Qs[p], Qp[p]) for p in range(S)]
+    say("sharing vs pair loop", S, precision, e)
+    assert max(e) < tol
+
+
+def test_sharing_needs_antipodes(device):
+    from dgfs_hip import _lib
+    from dgfs_hip.solvers.dgfsbi.scattering import VSSOperator
+    from dgfs_hip.solvers.dgfsbi.velocitymesh import DGFSBiVelocityMesh
+    from dgfs_hip.sphericaldesign import get_sphquadrule
+    N, L, Nrho, M = 4, 5.0, 2, 6
+    vm = DGFSBiVelocityMesh.from_params(None, N, L, Nrho, M, (1.0, 2.1))
+    sz = np.array(vm.sz()).reshape(M, 3).copy()
+    sz[0] = np.array([0.6, 0.8, 0.0])       # no longer the negation of its partner
+    op = VSSOperator(N, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), sz,
+                     get_sphquadrule("symmetric", npts=12).pts, vm.masses(), np.full((2, 2), 0.4),
+                     np.zeros((2, 2)), np.ones((2, 2)), precision=64)
+    assert not op.get_symmetry()
+    with pytest.raises(_lib.DGFSUnsupported):
+        op.set_symmetry(True)
