@@ -425,6 +425,7 @@ struct VssHandle {
   void* Z1 = nullptr;
   void* Z2 = nullptr;
   int groups_override = 0;
+  int slots_override = 0;  // cap on the persistent gain grid (0: occupancy)
   int last_groups = 0, last_slots = 0, threads = 0;
   int timing = 0;
   std::vector<hipEvent_t> pool;
@@ -524,7 +525,7 @@ static int vss_chunk_points(const VssHandle* h, int total) {
 template <typename T, int N>
 static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Qs, int npts, hipStream_t st) {
   constexpr int n = N * N * N;
-  const int slots = h->cap_slots;
+  const int slots = h->slots_override > 0 ? std::min(h->cap_slots, h->slots_override) : h->cap_slots;
   const std::vector<VssHandle::Job>& jobs = h->sym ? h->jobs_sym : h->jobs_plain;
   const int njobs = static_cast<int>(jobs.size());
   int groups = h->groups_override;
@@ -1003,6 +1004,14 @@ extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
   if (ngroups < 0 || ngroups > h->Nrho) return fail(DGFS_ERR_INVALID_VALUE, "ngroups must be 0..Nrho");
   h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_set_slots(void* p, int nslots) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (nslots < 0) return fail(DGFS_ERR_INVALID_VALUE, "nslots must be >= 0");
+  h->slots_override = nslots;
   return DGFS_OK;
 }
 

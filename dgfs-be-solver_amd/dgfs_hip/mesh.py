@@ -116,3 +116,47 @@ def rect_tris(nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic_x=False):
     if periodic_x:
         del bnd["left"], bnd["right"]
     return Mesh("tri", nodes, inters, bnd)
+
+
+def annulus_tris(nr, ntheta, r0=0.5, r1=5.0, growth=1.1, flow=(1.0, 0.0)):
+    """Triangles around a cylinder of radius r0 out to a circular far field of
+    radius r1: nr rings with radial spacing growing by `growth`, ntheta
+    sectors, each ring cell split along its diagonal into two triangles
+    (counter-clockwise nodes, faces as in rect_tris).  Boundaries:
+    'cylinder' (r = r0), and on r = r1 'inlet' where the outward normal
+    faces the flow direction and 'outlet' elsewhere.  Sides are straight,
+    so the cylinder is a regular ntheta-gon."""
+    dr = np.array([growth ** i for i in range(nr)])
+    rs = r0 + (r1 - r0) * np.concatenate([[0.0], np.cumsum(dr)]) / dr.sum()
+    th = 2 * np.pi * np.arange(ntheta) / ntheta
+    P = np.stack([rs[:, None] * np.cos(th)[None, :], rs[:, None] * np.sin(th)[None, :]], axis=-1)
+    nq = nr * ntheta
+    nodes = np.empty((3, 2 * nq, 2))
+    for i in range(nr):
+        for j in range(ntheta):
+            q = j + ntheta * i
+            jn = (j + 1) % ntheta
+            a, b, c, d = P[i, j], P[i + 1, j], P[i + 1, jn], P[i, jn]
+            nodes[:, 2 * q] = [a, b, c]
+            nodes[:, 2 * q + 1] = [c, d, a]
+    fd = np.asarray(flow, dtype=float)
+    inters, bnd = [], {"cylinder": [], "inlet": [], "outlet": []}
+    zero = np.zeros(2)
+    # lo = [a, b, c]: face 0 the sector edge at theta_j, face 1 the ring edge at
+    # r_{i+1}, face 2 the diagonal; up = [c, d, a]: face 0 the sector edge at
+    # theta_{j+1}, face 1 the ring edge at r_i, face 2 the diagonal
+    for i in range(nr):
+        for j in range(ntheta):
+            q = j + ntheta * i
+            lo, up = 2 * q, 2 * q + 1
+            inters.append((lo, 2, up, 2, zero))
+            if i + 1 < nr:
+                inters.append((lo, 1, 2 * (q + ntheta) + 1, 1, zero))
+            else:
+                mid = 0.5 * (P[nr, j] + P[nr, (j + 1) % ntheta])
+                bnd["inlet" if mid @ fd < 0 else "outlet"].append((lo, 1))
+            qn = (j + 1) % ntheta + ntheta * i
+            inters.append((up, 0, 2 * qn, 0, zero))
+            if i == 0:
+                bnd["cylinder"].append((up, 1))
+    return Mesh("tri", nodes, inters, bnd)

@@ -27,6 +27,7 @@ _lib.register({
     "dgfs_vss_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "dgfs_vss_set_groups": [_P, _I],
     "dgfs_vss_set_symmetry": [_P, _I],
+    "dgfs_vss_set_slots": [_P, _I],
     "dgfs_vss_get_symmetry": [_P, C.POINTER(_I)],
     "dgfs_vss_set_timing": [_P, _I],
     "dgfs_vss_get_timing": [_P, C.POINTER(C.c_double), C.POINTER(_I)],
@@ -78,6 +79,10 @@ class VSSOperator:
 
     def set_groups(self, ngroups):
         _lib.call("dgfs_vss_set_groups", self._h, int(ngroups))
+
+    def set_slots(self, nslots):
+        """Cap the persistent gain grid at nslots workgroups (0: occupancy)."""
+        _lib.call("dgfs_vss_set_slots", self._h, int(nslots))
 
     def set_symmetry(self, on=True):
         """Antipodal sharing of the gain products (default: on when the design

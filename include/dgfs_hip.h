@@ -172,6 +172,9 @@ int dgfs_vss_eval_contig(void* handle, const double* const* f, double* const* Q,
 int dgfs_vss_destroy(void* handle);
 int dgfs_vss_get_config(void* handle, int* ngroups, int* nslots, int* threads);
 int dgfs_vss_set_groups(void* handle, int ngroups);
+/* Cap the persistent grid (and so the scratch cubes in use) at nslots
+ * workgroups; 0 = the occupancy limit. */
+int dgfs_vss_set_slots(void* handle, int nslots);
 
 /* Antipodal sharing of the gain products (on by default when the direction
  * design holds the exact negation of every direction): one product per

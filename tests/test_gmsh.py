@@ -110,3 +110,34 @@ def test_gmsh_errors():
     with pytest.raises(ValueError, match="Required section"):
         i, j = good.index("$Nodes"), good.index("$EndNodes")
         read_gmsh(good[:i] + good[j + 1:])
+
+
+def test_annulus_mesh_is_consistent():
+    """The cylinder mesh: counter-clockwise triangles, every face either
+    paired once with a face of the same end points or on one boundary, the
+    annulus area, and a round trip through the gmsh reader."""
+    from dgfs_hip.mesh import annulus_tris
+    m = annulus_tris(4, 12, r0=0.5, r1=3.0, growth=1.2)
+    x = m.nodes
+    area = 0.5 * ((x[1, :, 0] - x[0, :, 0]) * (x[2, :, 1] - x[0, :, 1])
+                  - (x[2, :, 0] - x[0, :, 0]) * (x[1, :, 1] - x[0, :, 1]))
+    assert np.all(area > 0)
+    poly = 0.5 * 12 * np.sin(2 * np.pi / 12)
+    assert abs(area.sum() - poly * (3.0 ** 2 - 0.5 ** 2)) < 1e-12
+    seen = {}
+    ends = lambda e, f: {tuple(np.round(x[a, e], 12)) for a in _FACE_VERTS["tri"][f]}
+    for (eL, fL, eR, fR, sh) in m.interfaces:
+        assert ends(eL, fL) == ends(eR, fR)
+        for k in ((eL, fL), (eR, fR)):
+            assert k not in seen
+            seen[k] = 1
+    for name, faces in m.boundaries.items():
+        for k in faces:
+            assert k not in seen
+            seen[k] = 1
+    assert len(seen) == 3 * m.neles
+    r = lambda faces: [np.linalg.norm(x[_FACE_VERTS["tri"][f][0], e]) for e, f in faces]
+    assert np.allclose(r(m.boundaries["cylinder"]), 0.5) and np.allclose(r(m.boundaries["inlet"]), 3.0)
+    assert len(m.boundaries["inlet"]) + len(m.boundaries["outlet"]) == 12
+    back = read_gmsh(write_msh(m, ["cylinder", "inlet", "outlet"]))
+    _same_mesh(back, m)
