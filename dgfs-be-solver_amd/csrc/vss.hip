@@ -247,6 +247,133 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident gain, one pencil per thread (FP32, N = 16, 24: the N^3
+// field fits in LDS).  A workgroup of N*N threads owns one work item; thread
+// t = g N + l holds one pencil.  Per direction product:
+//   t3: z-pencil (kx = g, ky = l) of F_p times the phase, z-IFFT; the kx
+//       plane through LDS ([ky][z], rows padded to N+1) gives the y-pencil
+//       (kx = g, z = l), y-IFFT; the field through LDS once ([kx][y][z])
+//       gives the x-pencil (y = g, z = l), x-IFFT -- kept in registers;
+//   t4: the same for F_q with the conjugate-side phase;
+//   S = t3 t4 in registers, then the reverse sequence forward transforms S
+//   to z-pencils (kx = g, ky = l) and adds w b FFT(S) to this thread's
+//   pencil of G^ (and of the second output of a shared cross-pair job).
+// Same jobs, tables and arithmetic as k_vss_gain; no global scratch.
+// ---------------------------------------------------------------------------
+template <int N>
+struct VssRR1Shape {
+  static constexpr int NT = N * N;
+  static constexpr int PP = N + 1;          // padded row pitch of the kx planes
+  static constexpr int field = N * N * PP;  // complex; holds the N^3 exchange too
+  static_assert(field >= N * N * N, "exchange must fit the plane buffer");
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Fp, const cpx<T> (*e)[N],
+                                                cpx<T>* X, cpx<T>* Xp, int g, int l) {
+  constexpr int PP = VssRR1Shape<N>::PP, NN = N * N;
+  const cpx<T>* src = Fp + g * NN + l * N;
+#pragma unroll
+  for (int kz = 0; kz < N; ++kz) v[kz] = src[kz];
+  const cpx<T> exy = e[0][g] * e[1][l];
+#pragma unroll
+  for (int kz = 0; kz < N; ++kz) v[kz] = v[kz] * (exy * e[2][kz]);
+  FFT<T, N, +1>::run(v);  // z
+#pragma unroll
+  for (int z = 0; z < N; ++z) Xp[l * PP + z] = v[z];
+  __syncthreads();
+#pragma unroll
+  for (int ky = 0; ky < N; ++ky) v[ky] = Xp[ky * PP + l];
+  FFT<T, N, +1>::run(v);  // y: (kx = g, y, z = l)
+  __syncthreads();
+#pragma unroll
+  for (int y = 0; y < N; ++y) X[(g * N + y) * N + l] = v[y];
+  __syncthreads();
+#pragma unroll
+  for (int kx = 0; kx < N; ++kx) v[kx] = X[(kx * N + g) * N + l];
+  FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs<T> a) {
+  constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> e1[3][N], e2[3][N];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  cpx<T>* Xp = X + g * N * PP;  // kx plane g
+  const int per_job = a.npts * a.ngroups;
+  cpx<T> u[N], v[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int j = item / per_job, rem = item - j * per_job;
+    const int pt = rem / a.ngroups, grp = rem - pt * a.ngroups;
+    const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
+    const VssJob<T>& jb = a.job[j];
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const double mu1 = jb.mu1, mu2 = jb.mu2;
+    const size_t o = static_cast<size_t>(g) * NN + l * N;  // this thread's G^ pencil
+    cpx<T>* G = a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + grp) * n + o;
+    cpx<T>* G2 = (jb.out2 >= 0)
+                     ? a.Ghat + ((static_cast<size_t>(jb.out2) * a.npts + pt) * a.ngroups + grp) * n + o
+                     : nullptr;
+#pragma unroll
+    for (int k = 0; k < N; ++k) G[k] = cpx<T>{T(0), T(0)};
+    if (G2) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) G2[k] = cpx<T>{T(0), T(0)};
+    }
+    for (int x = x0; x < x1; ++x) {
+      const double alpha = a.alpha[x];
+      const T w = static_cast<T>(a.wgt[x]);
+      for (int iy = 0; iy < jb.ndir; ++iy) {
+        const int y = a.dirs[jb.dir_off + iy];
+        __syncthreads();  // readers of e1, e2 and the LDS field of the previous product are done
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double th = alpha * a.sz[y * 3 + d] * lfreq(k, N);
+          double sn, cs;
+          sincos(mu1 * th, &sn, &cs);
+          e1[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
+          sincos(mu2 * th, &sn, &cs);
+          e2[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(-sn)};
+        }
+        __syncthreads();
+        vss_rr1_inverse<T, N>(u, F1, e1, X, Xp, g, l);  // t3 (x-pencil at y = g, z = l)
+        __syncthreads();  // every x-pencil of t3 is read before the planes are rewritten
+        vss_rr1_inverse<T, N>(v, F2, e2, X, Xp, g, l);  // t4
+#pragma unroll
+        for (int xx = 0; xx < N; ++xx) u[xx] = u[xx] * v[xx];
+        FFT<T, N, -1>::run(u);  // x: (kx, y = g, z = l)
+        __syncthreads();
+#pragma unroll
+        for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = u[kx];
+        __syncthreads();
+#pragma unroll
+        for (int yy = 0; yy < N; ++yy) v[yy] = X[(g * N + yy) * N + l];
+        FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
+        __syncthreads();
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) Xp[ky * PP + l] = v[ky];
+        __syncthreads();
+#pragma unroll
+        for (int z = 0; z < N; ++z) v[z] = Xp[l * PP + z];
+        FFT<T, N, -1>::run(v);  // z: (kx = g, ky = l, kz)
+        const cpx<T>* b1 = jb.tab1 + (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n + o;
+#pragma unroll
+        for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
+        if (G2) {
+          const cpx<T>* b2 = jb.tab2 + (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off + iy]) * n + o;
+#pragma unroll
+          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // finalize: one item = (species p, point)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -426,6 +553,8 @@ struct VssHandle {
   void* Z2 = nullptr;
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent gain grid (0: occupancy)
+  int algorithm = 0;       // 0: automatic, 1: multi-pass, 4: register-resident (FP32, Nv 16 / 24)
+  int last_algorithm = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
   int timing = 0;
   std::vector<hipEvent_t> pool;
@@ -521,6 +650,15 @@ static int vss_chunk_points(const VssHandle* h, int total) {
   return (total + nchunks - 1) / nchunks;
 }
 
+// gain algorithm of this call: 4 (register-resident) where it is built and
+// not switched off, else 1 (multi-pass)
+template <typename T, int N>
+static int vss_rr_kind(const VssHandle* h) {
+  constexpr bool rr_ok = std::is_same<T, float>::value && (N == 16 || N == 24);
+  if (!rr_ok || h->algorithm == 1) return 1;
+  return 4;
+}
+
 // fs[s], Qs[s]: [npts][n] contiguous per species
 template <typename T, int N>
 static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Qs, int npts, hipStream_t st) {
@@ -592,12 +730,31 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   ga.ngroups = groups;
   ga.ppg = ppg;
   ga.Nrho = h->Nrho;
-  {
+  if (vss_rr_kind<T, N>(h) == 4) {
+    if constexpr (std::is_same<T, float>::value && (N == 16 || N == 24)) {
+      const size_t lds = static_cast<size_t>(VssRR1Shape<N>::field) * sizeof(cpx<T>);
+      const void* kfn = reinterpret_cast<const void*>(k_vss_gain_rr1<T, N>);
+      DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      int occ = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, VssRR1Shape<N>::NT, lds) != hipSuccess || occ < 1)
+        occ = 1;
+      int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(occ) * cu_count()));
+      if (h->slots_override > 0) grid = std::min(grid, h->slots_override);
+      KTimerT<VssHandle> t1(h, 1, st);
+      hipLaunchKernelGGL((k_vss_gain_rr1<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
+      DGFS_LAUNCH_CHECK("vss_gain_rr1");
+      t1.stop();
+      h->last_slots = grid;
+      h->threads = VssRR1Shape<N>::NT;
+      h->last_algorithm = 4;
+    }
+  } else {
     KTimerT<VssHandle> t1(h, 1, st);
     hipLaunchKernelGGL((k_vss_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
                        dim3(Shape<N>::NT), 0, st, ga);
     DGFS_LAUNCH_CHECK("vss_gain");
     t1.stop();
+    h->last_algorithm = 1;
   }
 
   VssFinArgs<T> fa{};
@@ -1004,6 +1161,25 @@ extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
   if (ngroups < 0 || ngroups > h->Nrho) return fail(DGFS_ERR_INVALID_VALUE, "ngroups must be 0..Nrho");
   h->groups_override = ngroups;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_set_algorithm(void* p, int algorithm) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (algorithm != 0 && algorithm != 1 && algorithm != 4)
+    return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0 (automatic), 1 (multi-pass) or 4 (register-resident)");
+  if (algorithm == 4 && !(h->prec == 32 && (h->N == 16 || h->N == 24)))
+    return fail(DGFS_ERR_UNSUPPORTED, "the register-resident VSS gain is built for FP32 at Nv 16 and 24");
+  h->algorithm = algorithm;
+  return DGFS_OK;
+}
+
+extern "C" int dgfs_vss_get_algorithm(void* p, int* algorithm) {
+  VssHandle* h = as_vss(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
+  if (!algorithm) return fail(DGFS_ERR_INVALID_VALUE, "null output");
+  *algorithm = h->last_algorithm;
   return DGFS_OK;
 }
 

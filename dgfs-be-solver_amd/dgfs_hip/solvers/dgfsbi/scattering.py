@@ -28,6 +28,8 @@ _lib.register({
     "dgfs_vss_set_groups": [_P, _I],
     "dgfs_vss_set_symmetry": [_P, _I],
     "dgfs_vss_set_slots": [_P, _I],
+    "dgfs_vss_set_algorithm": [_P, _I],
+    "dgfs_vss_get_algorithm": [_P, C.POINTER(_I)],
     "dgfs_vss_get_symmetry": [_P, C.POINTER(_I)],
     "dgfs_vss_set_timing": [_P, _I],
     "dgfs_vss_get_timing": [_P, C.POINTER(C.c_double), C.POINTER(_I)],
@@ -79,6 +81,16 @@ class VSSOperator:
 
     def set_groups(self, ngroups):
         _lib.call("dgfs_vss_set_groups", self._h, int(ngroups))
+
+    def set_algorithm(self, algorithm):
+        """Gain kernel: 0 automatic, 1 multi-pass, 4 register-resident (FP32, Nv 16 / 24)."""
+        _lib.call("dgfs_vss_set_algorithm", self._h, int(algorithm))
+
+    def get_algorithm(self):
+        """Gain kernel of the last call (1 or 4)."""
+        v = C.c_int()
+        _lib.call("dgfs_vss_get_algorithm", self._h, C.byref(v))
+        return v.value
 
     def set_slots(self, nslots):
         """Cap the persistent gain grid at nslots workgroups (0: occupancy)."""

@@ -175,6 +175,12 @@ int dgfs_vss_set_groups(void* handle, int ngroups);
 /* Cap the persistent grid (and so the scratch cubes in use) at nslots
  * workgroups; 0 = the occupancy limit. */
 int dgfs_vss_set_slots(void* handle, int nslots);
+/* Gain kernel: 0 automatic (register-resident where built, else multi-pass),
+ * 1 multi-pass (any precision and grid), 4 register-resident with one pencil
+ * per thread and no global scratch (FP32 at Nv 16 and 24).  get_algorithm
+ * reports the kernel of the last call. */
+int dgfs_vss_set_algorithm(void* handle, int algorithm);
+int dgfs_vss_get_algorithm(void* handle, int* algorithm);
 
 /* Antipodal sharing of the gain products (on by default when the direction
  * design holds the exact negation of every direction): one product per

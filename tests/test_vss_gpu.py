@@ -230,3 +230,46 @@ def test_sharing_needs_antipodes(device):
     assert not op.get_symmetry()
     with pytest.raises(_lib.DGFSUnsupported):
         op.set_symmetry(True)
+
+
+@pytest.mark.parametrize("N,L,masses,sym", [(16, 5.0, (1.0, 2.1), True), (24, 6.0, (1.0, 2.1), True),
+                                            (24, 6.0, (1.0, 2.1), False), (16, 5.0, (1.0, 2.1, 0.7), True)])
+def test_register_resident_matches_multipass(N, L, masses, sym, device):
+    """The register-resident FP32 gain (algorithm 4) and the multi-pass FP32
+    gain on the same handle and states, both against the FP64 operator: the
+    same sums with other FP32 roundings, so the register-resident error is
+    within 1e-4 and within twice the multi-pass error, and the two FP32
+    results agree to 5e-5."""
+    S = len(masses)
+    Nrho, M, Mpre = 6, 12, 24
+    omegas = tuple(0.81 - 0.005 * (p + q) for p in range(S) for q in range(S))
+    op, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=32)
+    op64, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=64)
+    op.set_symmetry(sym)
+    fs = _mixture_states(V.BiVelocityMesh(N, L, Nrho, M, masses), 5, seed=31)
+    Q64 = run_contig(op64, fs, device)
+    op.set_algorithm(4)
+    Qr = run_contig(op, fs, device)
+    assert op.get_algorithm() == 4
+    op.set_algorithm(1)
+    Qm = run_contig(op, fs, device)
+    assert op.get_algorithm() == 1
+    er = [rel(Qr[p], Q64[p]) for p in range(S)]
+    em = [rel(Qm[p], Q64[p]) for p in range(S)]
+    e = [rel(Qr[p], Qm[p]) for p in range(S)]
+    say("register-resident / multi-pass vs FP64, and between", N, S, sym, er, em, e)
+    for p in range(S):
+        assert er[p] < 1e-4 and er[p] < 2 * em[p] + 1e-6
+    assert max(e) < 5e-5
+
+
+def test_register_resident_needs_fp32_and_grid(device):
+    from dgfs_hip import _lib
+    op64, _ = make_op(16, 5.0, 4, 12, 24, (1.0, 2.1), 1.0, (0.8,) * 4, (1,) * 4, precision=64)
+    with pytest.raises(_lib.DGFSUnsupported):
+        op64.set_algorithm(4)
+    op8, _ = make_op(8, 5.0, 4, 12, 24, (1.0, 2.1), 1.0, (0.8,) * 4, (1,) * 4, precision=32)
+    with pytest.raises(_lib.DGFSUnsupported):
+        op8.set_algorithm(4)
+    with pytest.raises(_lib.DGFSInvalidValue):
+        op8.set_algorithm(3)
