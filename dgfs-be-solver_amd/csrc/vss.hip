@@ -250,15 +250,18 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
 // Register-resident gain, one pencil per thread (FP32, N = 16, 24: the N^3
 // field fits in LDS).  A workgroup of N*N threads owns one work item; thread
 // t = g N + l holds one pencil.  Per direction product:
-//   t3: z-pencil (kx = g, ky = l) of F_p times the phase, z-IFFT; the kx
-//       plane through LDS ([ky][z], rows padded to N+1) gives the y-pencil
-//       (kx = g, z = l), y-IFFT; the field through LDS once ([kx][y][z])
-//       gives the x-pencil (y = g, z = l), x-IFFT -- kept in registers;
+//   t3: y-pencil (kx = g, kz = l) of F_p times the phase, y-IFFT; the kx
+//       plane through LDS ([y][kz], rows padded to N+1) gives the z-pencil
+//       (kx = g, y = l), z-IFFT; the field through LDS once ([kx][z][y])
+//       gives the x-pencil (y = l, z = g), x-IFFT -- kept in registers;
 //   t4: the same for F_q with the conjugate-side phase;
-//   S = t3 t4 in registers, then the reverse sequence forward transforms S
-//   to z-pencils (kx = g, ky = l) and adds w b FFT(S) to this thread's
-//   pencil of G^ (and of the second output of a shared cross-pair job).
-// Same jobs, tables and arithmetic as k_vss_gain; no global scratch.
+//   S = t3 t4 in registers; the reverse sequence (x, then z through the
+//   field, then y through the plane) forward transforms S to y-pencils
+//   (kx = g, kz = l) and adds w b FFT(S) to this thread's pencil of G^ (and
+//   of the second output of a shared cross-pair job).
+// The pencils that touch global memory (F, b, G^) run along ky, so at a
+// fixed ky the lanes of a wave address consecutive values.  Same jobs,
+// tables and arithmetic as k_vss_gain; no global scratch.
 // ---------------------------------------------------------------------------
 template <int N>
 struct VssRR1Shape {
@@ -268,30 +271,48 @@ struct VssRR1Shape {
   static_assert(field >= N * N * N, "exchange must fit the plane buffer");
 };
 
+// a pencil of stride N complex values in global memory addressed from two
+// bases with immediate offsets (the offsets of one base stay under 4 KiB);
+// the second base's distance is opaque to the compiler, so it does not fold
+// the two bases back into one address per element
+template <typename T, int N>
+struct StridedPencil {
+  static constexpr int KB = (4096 / static_cast<int>(N * sizeof(cpx<T>))) < N
+                                ? (4096 / static_cast<int>(N * sizeof(cpx<T>))) : N;
+  cpx<T>* p0;
+  cpx<T>* p1;
+  __device__ __forceinline__ explicit StridedPencil(cpx<T>* base) : p0(base) {
+    int second = KB * N;
+    asm volatile("" : "+v"(second));
+    p1 = base + second;
+  }
+  __device__ __forceinline__ cpx<T>& operator[](int k) const { return (k < KB) ? p0[k * N] : p1[(k - KB) * N]; }
+};
+
 template <typename T, int N>
 __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Fp, const cpx<T> (*e)[N],
                                                 cpx<T>* X, cpx<T>* Xp, int g, int l) {
   constexpr int PP = VssRR1Shape<N>::PP, NN = N * N;
-  const cpx<T>* src = Fp + g * NN + l * N;
+  const StridedPencil<T, N> src(const_cast<cpx<T>*>(Fp) + g * NN + l);
 #pragma unroll
-  for (int kz = 0; kz < N; ++kz) v[kz] = src[kz];
-  const cpx<T> exy = e[0][g] * e[1][l];
+  for (int ky = 0; ky < N; ++ky) v[ky] = src[ky];
+  const cpx<T> exz = e[0][g] * e[2][l];
 #pragma unroll
-  for (int kz = 0; kz < N; ++kz) v[kz] = v[kz] * (exy * e[2][kz]);
-  FFT<T, N, +1>::run(v);  // z
+  for (int ky = 0; ky < N; ++ky) v[ky] = v[ky] * (exz * e[1][ky]);
+  FFT<T, N, +1>::run(v);  // y: (kx = g, y, kz = l)
 #pragma unroll
-  for (int z = 0; z < N; ++z) Xp[l * PP + z] = v[z];
+  for (int y = 0; y < N; ++y) Xp[y * PP + l] = v[y];
   __syncthreads();
 #pragma unroll
-  for (int ky = 0; ky < N; ++ky) v[ky] = Xp[ky * PP + l];
-  FFT<T, N, +1>::run(v);  // y: (kx = g, y, z = l)
+  for (int kz = 0; kz < N; ++kz) v[kz] = Xp[l * PP + kz];
+  FFT<T, N, +1>::run(v);  // z: (kx = g, y = l, z)
   __syncthreads();
 #pragma unroll
-  for (int y = 0; y < N; ++y) X[(g * N + y) * N + l] = v[y];
+  for (int z = 0; z < N; ++z) X[(g * N + z) * N + l] = v[z];
   __syncthreads();
 #pragma unroll
   for (int kx = 0; kx < N; ++kx) v[kx] = X[(kx * N + g) * N + l];
-  FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
+  FFT<T, N, +1>::run(v);  // x: (x, y = l, z = g)
 }
 
 template <typename T, int N>
@@ -313,14 +334,14 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
     const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
     const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
     const double mu1 = jb.mu1, mu2 = jb.mu2;
-    const size_t o = static_cast<size_t>(g) * NN + l * N;  // this thread's G^ pencil
-    cpx<T>* G = a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + grp) * n + o;
-    cpx<T>* G2 = (jb.out2 >= 0)
-                     ? a.Ghat + ((static_cast<size_t>(jb.out2) * a.npts + pt) * a.ngroups + grp) * n + o
-                     : nullptr;
+    const size_t o = static_cast<size_t>(g) * NN + l;  // this thread's G^ pencil (kx = g, kz = l), stride N
+    const bool two = jb.out2 >= 0;
+    const StridedPencil<T, N> G(a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + grp) * n + o);
+    const StridedPencil<T, N> G2(a.Ghat + ((static_cast<size_t>(two ? jb.out2 : jb.out1) * a.npts + pt) *
+                                               a.ngroups + grp) * n + o);
 #pragma unroll
     for (int k = 0; k < N; ++k) G[k] = cpx<T>{T(0), T(0)};
-    if (G2) {
+    if (two) {
 #pragma unroll
       for (int k = 0; k < N; ++k) G2[k] = cpx<T>{T(0), T(0)};
     }
@@ -340,31 +361,35 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
           e2[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(-sn)};
         }
         __syncthreads();
-        vss_rr1_inverse<T, N>(u, F1, e1, X, Xp, g, l);  // t3 (x-pencil at y = g, z = l)
+        vss_rr1_inverse<T, N>(u, F1, e1, X, Xp, g, l);  // t3 (x-pencil at y = l, z = g)
         __syncthreads();  // every x-pencil of t3 is read before the planes are rewritten
         vss_rr1_inverse<T, N>(v, F2, e2, X, Xp, g, l);  // t4
 #pragma unroll
         for (int xx = 0; xx < N; ++xx) u[xx] = u[xx] * v[xx];
-        FFT<T, N, -1>::run(u);  // x: (kx, y = g, z = l)
+        FFT<T, N, -1>::run(u);  // x: (kx, y = l, z = g)
         __syncthreads();
 #pragma unroll
         for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = u[kx];
         __syncthreads();
 #pragma unroll
-        for (int yy = 0; yy < N; ++yy) v[yy] = X[(g * N + yy) * N + l];
-        FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
+        for (int z = 0; z < N; ++z) v[z] = X[(g * N + z) * N + l];
+        FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
         __syncthreads();
 #pragma unroll
-        for (int ky = 0; ky < N; ++ky) Xp[ky * PP + l] = v[ky];
+        for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
         __syncthreads();
 #pragma unroll
-        for (int z = 0; z < N; ++z) v[z] = Xp[l * PP + z];
-        FFT<T, N, -1>::run(v);  // z: (kx = g, ky = l, kz)
-        const cpx<T>* b1 = jb.tab1 + (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n + o;
+        for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
+        FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+        {
+          const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
+                                       (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n + o);
 #pragma unroll
-        for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
-        if (G2) {
-          const cpx<T>* b2 = jb.tab2 + (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off + iy]) * n + o;
+          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
+        }
+        if (two) {
+          const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
+                                       (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off + iy]) * n + o);
 #pragma unroll
           for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
         }
