@@ -269,6 +269,10 @@ struct VssRR1Shape {
   static constexpr int PP = N + 1;          // padded row pitch of the kx planes
   static constexpr int field = N * N * PP;  // complex; holds the N^3 exchange too
   static_assert(field >= N * N * N, "exchange must fit the plane buffer");
+  // values of t3 parked in LDS while t4 is transformed (frees registers where
+  // the 576-thread workgroup's 168-VGPR limit would spill; the rest of the
+  // 160 KiB of LDS holds them)
+  static constexpr int park = (N == 24) ? 10 : 0;
 };
 
 // a pencil of stride N complex values in global memory addressed from two
@@ -318,9 +322,11 @@ __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __
 template <typename T, int N>
 __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs<T> a) {
   constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KP = VssRR1Shape<N>::park;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
   __shared__ cpx<T> e1[3][N], e2[3][N];
+  __shared__ cpx<T> upark[KP > 0 ? KP : 1][NT];
   const int tid = threadIdx.x;
   const int g = tid / N, l = tid - (tid / N) * N;
   cpx<T>* Xp = X + g * N * PP;  // kx plane g
@@ -362,8 +368,12 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
         }
         __syncthreads();
         vss_rr1_inverse<T, N>(u, F1, e1, X, Xp, g, l);  // t3 (x-pencil at y = l, z = g)
+#pragma unroll
+        for (int k = 0; k < KP; ++k) upark[k][tid] = u[N - KP + k];  // thread-private slots
         __syncthreads();  // every x-pencil of t3 is read before the planes are rewritten
         vss_rr1_inverse<T, N>(v, F2, e2, X, Xp, g, l);  // t4
+#pragma unroll
+        for (int k = 0; k < KP; ++k) u[N - KP + k] = upark[k][tid];
 #pragma unroll
         for (int xx = 0; xx < N; ++xx) u[xx] = u[xx] * v[xx];
         FFT<T, N, -1>::run(u);  // x: (kx, y = l, z = g)
