@@ -36,21 +36,27 @@ class DGFSResidualStdPlugin:
         st = self.stepper
         return [st.system.eles_scal_upts_full[st.soln_bank]]
 
-    def __call__(self, stepper=None):
+    def _norms(self):
+        """[(sum (c - p)^2, sum p^2)] per register of _regs()."""
         from dgfs_hip import fr
+        st = self.stepper
+        out = []
+        for p, c in zip(self._prev, self._regs()):
+            nupts, vsize, neles = c.ioshape
+            fr.residual_norms(nupts, neles, vsize, p, c, c.leaddim, self._out2, self._work,
+                              st.backend.stream_ptr)
+            st.backend.synchronize()
+            out.append(tuple(float(v) for v in self._out2.cpu()))
+        return out
+
+    def _row(self, norms):
+        return [self.stepper.tcurr, math.sqrt(sum(a for a, _ in norms) / sum(b for _, b in norms))]
+
+    def __call__(self, stepper=None):
         st = self.stepper
         n = st.nacptsteps
         if n % self.nsteps == 0 and n and self._prev is not None:
-            num = den = 0.0
-            for p, c in zip(self._prev, self._regs()):
-                nupts, vsize, neles = c.ioshape
-                fr.residual_norms(nupts, neles, vsize, p, c, c.leaddim, self._out2, self._work,
-                                  st.backend.stream_ptr)
-                st.backend.synchronize()
-                a, b = (float(v) for v in self._out2.cpu())
-                num += a
-                den += b
-            row = [st.tcurr, math.sqrt(num / den)]
+            row = self._row(self._norms())
             self.rows.append(row)
             if self.outf is not None:
                 print(",".join(str(r) for r in row), file=self.outf)
@@ -59,6 +65,21 @@ class DGFSResidualStdPlugin:
         if (n + 1) % self.nsteps == 0:
             st.backend.synchronize()
             self._prev = [r.data.clone() for r in self._regs()]
+
+
+class DGFSResidualBiPlugin(DGFSResidualStdPlugin):
+    """Per-species residual of the multi-species system
+    (plugins/dgfsresidualbi.py:9-95): row [t, r_1, ..., r_S] with r_p over
+    species p's solution register (register 0 of its bank)."""
+    name = "dgfsresidualbi"
+    systems = ["dgfsbi"]
+
+    def _regs(self):
+        sysm = self.stepper.system
+        return [sysm.species_register(p, 0) for p in range(sysm._nspcs)]
+
+    def _row(self, norms):
+        return [self.stepper.tcurr] + [math.sqrt(a / b) for a, b in norms]
 
 
 class DGFSDistWriterStdPlugin:

@@ -124,6 +124,66 @@ class DGFSBiSystem:
         fs = self.eles_scal_upts_full
         self.sm.fs_batch([fs[r] for r in regids[:S]], [fs[r] for r in regids[S:]])
 
+    def mixture_moments(self, bank=0):
+        """Per species the dimensional moment fields of
+        plugins/dgfsmomwriterbi.py:133-400 from register `bank` of every
+        species: mass density, bulk velocity, species temperature (about the
+        species' own velocity), heat flux and pressure tensor about the
+        mixture velocity, p and number density; 2D meshes take the mixture's
+        z velocity as 0, as the reference's 2D writer does.  Returns a list
+        over species of {name: (nupts, neles)}.
+
+        The raw sums come from the HIP moments kernel (central moments about
+        each species' own velocity); the shift to the mixture velocity W is
+        exact algebra, with d = U_p - W and M = sum f c' c' cw:
+        sum f (c-W)(c-W) cw = M + n d d,
+        sum f (c-W)|c-W|^2 cw = q + 2 M d + d tr M + n d |d|^2."""
+        vm = self.vm
+        S, nd = self._nspcs, self.ndims
+        raw = self.moments([p * self.nregs0 + bank for p in range(S)])   # (nupts, 15, neles) each
+        m = vm.masses()
+        rho0, u0, T0, M0 = vm.rho0(), vm.u0(), vm.T0(), vm.molarMass0()
+        n = [r[:, 0] for r in raw]
+        U = [r[:, 1:4] for r in raw]                                        # (nupts, 3, neles)
+        rho = [m[p] * n[p] for p in range(S)]                              # nondimensional
+        tot = sum(rho)
+        W = sum(rho[p][:, None] * U[p] for p in range(S)) / tot[:, None]
+        if nd < 3:
+            W[:, 2] = 0.0
+        iu = {(0, 0): 8, (0, 1): 9, (0, 2): 10, (1, 1): 11, (1, 2): 12, (2, 2): 13}
+        names = "xyz"
+        out = []
+        for p in range(S):
+            r = raw[p]
+            Mc = np.empty(r.shape[:1] + (3, 3) + r.shape[2:])
+            for (i, j), k in iu.items():
+                Mc[:, i, j] = Mc[:, j, i] = 0.5 * r[:, k]
+            # species temperature about (Ux, Uy, 0) in 2D: shifting the
+            # centre by Uz adds n Uz^2 to the sum of |c'|^2
+            uz = U[p][:, 2] if nd < 3 else 0.0
+            T = m[p] * (r[:, 4] + (2.0 / 3.0) * uz ** 2)
+            trM = Mc[:, 0, 0] + Mc[:, 1, 1] + Mc[:, 2, 2]
+            # M and q are about the species' own velocity: shift them to W
+            e = U[p] - W
+            e2 = (e ** 2).sum(1)
+            Me = np.einsum("aijb,ajb->aib", Mc, e)
+            qW = r[:, 5:8] + 2 * Me + e * trM[:, None] + n[p][:, None] * e * e2[:, None]
+            MW = Mc + n[p][:, None, None] * e[:, :, None] * e[:, None, :]
+            f = {"rho": rho[p] * rho0}
+            for i in range(nd):
+                f["U" + names[i]] = U[p][:, i] * u0
+            f["T"] = T * T0
+            for i in range(nd):
+                f["Q" + names[i]] = m[p] ** 2 * qW[:, i] * 0.5 * rho0 * u0 ** 3
+            for i in range(nd):
+                for j in range(i, nd):
+                    f["P" + names[i] + names[j]] = 2 * m[p] ** 2 * MW[:, i, j] * 0.5 * rho0 * u0 ** 2
+            # the reference's p and number density (dgfsmomwriterbi.py:248-256)
+            f["p"] = (m[p] * vm.R0 / M0) * f["rho"] * f["T"]
+            f["nden"] = (vm.NA / m[p] / M0) * f["rho"]
+            out.append(f)
+        return out
+
     def soln_arrays(self, bank=0):
         """{etype: (nupts, nspcs * vsize, neles)}: register `bank` of every
         species side by side (dgfsdistwriterbi.py:59-76)."""

@@ -238,3 +238,90 @@ def test_bi_fp32_steps_track_fp64(device):
         say("  species", p, "fp32 vs fp64", err, "number drift", abs(n1 - n0) / n0)
         assert err < 1e-5
         assert abs(n1 - n0) <= 1e-12 * n0
+
+
+def test_bi_residual_plugin(device):
+    """DGFSResidualBiPlugin (plugins/dgfsresidualbi.py:9-95): every nsteps
+    steps, per species sqrt(sum (f_n - f_{n-1})^2 / sum f_{n-1}^2), against
+    numpy on the unpacked registers (1e-12)."""
+    import math
+    from dgfs_hip.integrators.steppers import get_stepper
+    from dgfs_hip.plugins import DGFSResidualBiPlugin
+    be, sysm = build(6)
+    ov, _, _ = oracles(sysm)
+    f0 = initial(ov, 23)
+    for p in range(2):
+        sysm.species_register(p, 0).set(f0[p])
+    st = get_stepper("dgfsbi-tvd-rk2", be, sysm, 0.01)
+    seen = []
+    st.plugins.append(lambda s: seen.append([sysm.species_register(p, 0).get() for p in range(2)]))
+    pl = DGFSResidualBiPlugin(st, 2)
+    st.plugins.append(pl)
+    say("bi residual plugin, 4 steps")
+    st.advance_to(0.04)
+    assert len(pl.rows) == 2 and all(len(r) == 3 for r in pl.rows)
+    for k, (i, j) in enumerate([(0, 1), (2, 3)]):
+        for p in range(2):
+            want = math.sqrt(np.sum((seen[j][p] - seen[i][p]) ** 2) / np.sum(seen[i][p] ** 2))
+            say("  ", k, p, pl.rows[k][1 + p], want)
+            assert abs(pl.rows[k][1 + p] - want) <= 1e-12 * want
+
+
+def _writer_bi_2d(fs, cv, cw, masses, rho0, u0, T0, M0, R0, NA):
+    """plugins/dgfsmomwriterbi.py:133-256 on host arrays f_p (nupts, vsize, neles)."""
+    out, tot, mom = [], 0.0, 0.0
+    sp = []
+    for p, f in enumerate(fs):
+        mr = masses[p]
+        rho = f.sum(1) * mr * cw
+        ux = np.tensordot(f, cv[0], axes=(1, 0)) * mr * cw / rho
+        uy = np.tensordot(f, cv[1], axes=(1, 0)) * mr * cw / rho
+        cx = cv[0][None, :, None] - ux[:, None, :]
+        cy = cv[1][None, :, None] - uy[:, None, :]
+        cz = cv[2][None, :, None]
+        T = (f * (cx * cx + cy * cy + cz * cz)).sum(1) * (2.0 / 3.0 * mr * cw * mr) / rho
+        sp.append((rho, ux, uy, T))
+        tot = tot + rho
+        mom = mom + rho * np.stack([ux, uy])
+    W = mom / tot
+    for p, f in enumerate(fs):
+        mr = masses[p]
+        rho, ux, uy, T = sp[p]
+        cx = cv[0][None, :, None] - W[0][:, None, :]
+        cy = cv[1][None, :, None] - W[1][:, None, :]
+        cz = cv[2][None, :, None]
+        c2 = cx * cx + cy * cy + cz * cz
+        d = {"rho": rho * rho0, "Ux": ux * u0, "Uy": uy * u0, "T": T * T0,
+             "Qx": mr * (f * c2 * cx).sum(1) * mr * cw * 0.5 * rho0 * u0 ** 3,
+             "Qy": mr * (f * c2 * cy).sum(1) * mr * cw * 0.5 * rho0 * u0 ** 3,
+             "Pxx": 2 * mr * (f * cx * cx).sum(1) * mr * cw * 0.5 * rho0 * u0 ** 2,
+             "Pxy": 2 * mr * (f * cx * cy).sum(1) * mr * cw * 0.5 * rho0 * u0 ** 2,
+             "Pyy": 2 * mr * (f * cy * cy).sum(1) * mr * cw * 0.5 * rho0 * u0 ** 2}
+        d["p"] = (mr * R0 / M0) * d["rho"] * d["T"]
+        d["nden"] = (NA / mr / M0) * d["rho"]
+        out.append(d)
+    return out
+
+
+def test_bi_mixture_moments(device):
+    """DGFSBiSystem.mixture_moments against the reference writer's direct
+    sums (dgfsmomwriterbi.py, 2D) on drifting species states (1e-10)."""
+    be, sysm = build(4)
+    ov, _, _ = oracles(sysm)
+    rng = np.random.default_rng(29)
+    fs = []
+    for p, nd in enumerate((0.6, 0.4)):
+        f = V.species_maxwellian(ov, p, nd, [0.2, -0.1, 0.15], 1.1)[None, :, None] * \
+            (1 + 0.1 * rng.random((NUPTS, 1, 6))) * (1 + 0.05 * rng.random((1, ov.n, 1)))
+        fs.append(f)
+        sysm.species_register(p, 0).set(f)
+    vm = sysm.vm
+    got = sysm.mixture_moments(0)
+    want = _writer_bi_2d(fs, vm.cv(), vm.cw(), vm.masses(), vm.rho0(), vm.u0(), vm.T0(), vm.molarMass0(),
+                         vm.R0, vm.NA)
+    for p in range(2):
+        assert set(got[p]) == set(want[p])
+        for k, w in want[p].items():
+            err = np.abs(got[p][k] - w).max() / np.abs(w).max()
+            assert err < 1e-10, (p, k, err)
+    say("bi mixture moments ok", {k: float(got[0][k].mean()) for k in ("rho", "Ux", "T", "Qx", "Pxy")})
