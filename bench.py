@@ -218,8 +218,14 @@ def bench_vss(args, dev, stream):
     cfgv = op32.get_config()
     op32.set_timing(False)
     gain_ms, gain_n = tim["gain"]
+    algo = op32.get_algorithm()
+    kname = f"k_vss_gain_rr1<float,{nv}>" if algo == 4 else f"k_vss_gain<float,{nv}>"
     bq = vss_bytes(nv, K, S, 8)
     achieved = bq * P * args.vss_steps / (gain_ms / 1e3)
+    # HBM bytes per launch from the committed counter runs, scaled to this batch
+    tr = read_traffic(f"vss_gain_rr1_fp32_n{nv}_traffic.json" if algo == 4 else "")
+    traffic = (tr["bytes_per_dispatch"] * P * args.vss_steps / max(gain_n, 1) / tr["points_per_dispatch"]
+               if tr else None)
     out = {
         "workload": f"two-species VSS mixture: {P} points, Nv={nv} on [-{L:g},{L:g}]^3, M={M}, Nrho={Nrho}, "
                     f"Mpre={Mpre}, masses 1 / 2.1, omega 0.81 / 0.805 / 0.80, bimodal species states",
@@ -230,10 +236,10 @@ def bench_vss(args, dev, stream):
         "dtype": "f32+fp64-projection",
         "kernel_ms_per_step": {k: v[0] / args.vss_steps for k, v in tim.items()},
         "config": cfgv,
-        "roofline": {"bound": "hbm", "kernel": f"k_vss_gain<float,{nv}>", "achieved": achieved / 1e9,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "algorithmic_bytes_per_point": bq,
-                     "avg_launch_ms": gain_ms / max(gain_n, 1), "traffic": None,
+                     "avg_launch_ms": gain_ms / max(gain_n, 1), "traffic": traffic,
                      "note": "reference dataflow with every transform crossing HBM once; the kernel "
                              "keeps them in LDS/registers"},
     }
