@@ -5,9 +5,9 @@ frfs/readers/base.py:80-200: physical names select the fluid elements
 ('fluid'), the boundaries (every other name, which becomes the
 `soln-bcs-<name>` section) and periodic face pairs ('periodic_<id>_l' /
 'periodic_<id>_r'); faces are paired through their sorted vertex ids, and
-periodic faces through their centroids once the left side is translated
-onto the right.  Elements must be of one 2D type (triangles or quadrangles;
-higher-order elements keep their vertices, i.e. straight sides).  The result
+periodic faces in the order of their centroids.  Elements must be of one 2D
+type (triangles or quadrangles; higher-order elements keep their vertices,
+i.e. straight sides).  The result
 is the Mesh of dgfs_hip.mesh, with the reference's linear node ordering
 (gmsh quad nodes 0 1 2 3 -> tensor order 0 1 3 2) and face numbering
 (face k between gmsh vertices k and k+1).
@@ -49,8 +49,30 @@ def _sections(lines):
         yield name, body
 
 
+def _fuzzysort(pts, idx, dim=0, tol=1e-6):
+    """Indices idx ordered by pts[dim], values within tol counting as equal
+    and ordered by the next dimension (the ordering of frfs/nputil.py:84-100
+    that pairs periodic faces)."""
+    srt = sorted(idx, key=lambda i: pts[dim][i])
+    if dim + 1 >= len(pts) or len(srt) < 2:
+        return srt
+    out, i = [], 0
+    while i < len(srt):
+        j = i + 1
+        while j < len(srt) and pts[dim][srt[j]] - pts[dim][srt[i]] < tol:
+            j += 1
+        grp = srt[i:j]
+        out.extend(_fuzzysort(pts, grp, dim + 1, tol) if len(grp) > 1 else grp)
+        i = j
+    return out
+
+
 def read_gmsh(path_or_lines, ndims=2):
-    lines = open(path_or_lines).read().splitlines() if isinstance(path_or_lines, str) else list(path_or_lines)
+    if isinstance(path_or_lines, str):
+        with open(path_or_lines) as fh:
+            lines = fh.read().splitlines()
+    else:
+        lines = list(path_or_lines)
     sect = dict(_sections(lines))
     for req in ("MeshFormat", "PhysicalNames", "Nodes", "Elements"):
         if req not in sect:
@@ -124,8 +146,8 @@ def read_gmsh(path_or_lines, ndims=2):
     def centroid(key):
         return 0.5 * (pts[key[0]] + pts[key[1]])
 
-    # periodic pairs: translate the left side onto the right
-    # (readers/base.py:138-161 pairs them by sorted centroids)
+    # periodic pairs: both sides' faces ordered by their centroids, the
+    # i-th left face paired with the i-th right face (readers/base.py:138-161)
     for pid, tags in periodic.items():
         lk = [tuple(sorted(v)) for v in lines_by_tag[tags["l"]]]
         rk = [tuple(sorted(v)) for v in lines_by_tag[tags["r"]]]
@@ -133,18 +155,15 @@ def read_gmsh(path_or_lines, ndims=2):
             raise ValueError(f"periodic boundary {pid}: sides differ in face count")
         cl = np.array([centroid(k) for k in lk])
         cr = np.array([centroid(k) for k in rk])
-        shift = cr.mean(0) - cl.mean(0)
-        used = set()
-        for i, k in enumerate(lk):
-            d = np.sum((cr - (cl[i] + shift)) ** 2, axis=1)
-            j = int(np.argmin(d))
-            if j in used or d[j] > 1e-16 * max(1.0, float(np.sum(cl[i] ** 2))):
-                raise ValueError(f"periodic boundary {pid}: faces do not match under translation")
-            used.add(j)
-            eL, fL = open_faces.pop(k)
-            eR, fR = open_faces.pop(rk[j])
+        li = _fuzzysort(cl.T, range(len(lk)))
+        ri = _fuzzysort(cr.T, range(len(rk)))
+        for a, b in zip(li, ri):
+            if lk[a] not in open_faces or rk[b] not in open_faces:
+                raise ValueError(f"periodic boundary {pid}: face is not a free face of the fluid")
+            eL, fL = open_faces.pop(lk[a])
+            eR, fR = open_faces.pop(rk[b])
             # shift carries the right-hand face onto the left-hand one
-            inters.append((eL, fL, eR, fR, -shift))
+            inters.append((eL, fL, eR, fR, cl[a] - cr[b]))
 
     boundaries = {}
     for tag, name in bnames.items():
