@@ -409,6 +409,121 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident gain for isotropic scattering (eta = 0 for every pair,
+// the VHS mixture): B_yj = 1 makes b_pq(x, y) the same for every direction,
+// so G^_pq = sum_x w_x b_pq(x) FFT(sum_y t3 t4) -- the direction products
+// are summed in space and forward transformed once per radial node, as the
+// single-species operator sums before its forward transform.  Per product:
+// t3 (x-pencil) goes to this workgroup's scratch cube (thread-private),
+// t4 is formed, and acc += t3 t4 in registers; per radial
+// node the reverse sequence transforms acc to y-pencils (kx, kz) and adds
+// w b(x) FFT(acc) to G^ (both outputs of a shared cross-pair job: the sum
+// over all directions of the (p, q) products is also the (q, p) sum).  The
+// tables are the general ones read at the job's first direction row.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGainArgs<T> a) {
+  constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KP = VssRR1Shape<N>::park;  // accumulator values kept in the spare LDS
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> e1[3][N], e2[3][N];
+  __shared__ cpx<T> apark[KP > 0 ? KP : 1][NT];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  cpx<T>* Xp = X + g * N * PP;  // kx plane g
+  // t3 parking: thread-contiguous pencils (one base register, immediate offsets)
+  cpx<T>* U = a.Z1 + static_cast<size_t>(blockIdx.x) * n + static_cast<size_t>(tid) * N;
+  const int per_job = a.npts * a.ngroups;
+  cpx<T> acc[N], v[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int j = item / per_job, rem = item - j * per_job;
+    const int pt = rem / a.ngroups, grp = rem - pt * a.ngroups;
+    const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
+    const VssJob<T>& jb = a.job[j];
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const double mu1 = jb.mu1, mu2 = jb.mu2;
+    const size_t o = static_cast<size_t>(g) * NN + l;  // this thread's G^ pencil (kx = g, kz = l), stride N
+    const bool two = jb.out2 >= 0;
+    const StridedPencil<T, N> G(a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + grp) * n + o);
+    const StridedPencil<T, N> G2(a.Ghat + ((static_cast<size_t>(two ? jb.out2 : jb.out1) * a.npts + pt) *
+                                               a.ngroups + grp) * n + o);
+    for (int x = x0; x < x1; ++x) {
+      const double alpha = a.alpha[x];
+#pragma unroll
+      for (int k = 0; k < N - KP; ++k) acc[k] = cpx<T>{T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < KP; ++k) apark[k][tid] = cpx<T>{T(0), T(0)};  // thread-private slots
+      for (int iy = 0; iy < jb.ndir; ++iy) {
+        const int y = a.dirs[jb.dir_off + iy];
+        __syncthreads();  // readers of e1, e2 and the LDS field of the previous product are done
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double th = alpha * a.sz[y * 3 + d] * lfreq(k, N);
+          double sn, cs;
+          sincos(mu1 * th, &sn, &cs);
+          e1[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
+          sincos(mu2 * th, &sn, &cs);
+          e2[d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(-sn)};
+        }
+        __syncthreads();
+        vss_rr1_inverse<T, N>(v, F1, e1, X, Xp, g, l);  // t3 (x-pencil at y = l, z = g)
+#pragma unroll
+        for (int xx = 0; xx < N; ++xx) U[xx] = v[xx];
+        __syncthreads();  // every x-pencil of t3 is read before the planes are rewritten
+        vss_rr1_inverse<T, N>(v, F2, e2, X, Xp, g, l);  // t4
+#pragma unroll
+        for (int xx = 0; xx < N - KP; ++xx) acc[xx] = acc[xx] + U[xx] * v[xx];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) apark[k][tid] = apark[k][tid] + U[N - KP + k] * v[N - KP + k];
+      }
+#pragma unroll
+      for (int k = 0; k < KP; ++k) acc[N - KP + k] = apark[k][tid];
+      // G^ += w b(x) FFT(sum_y t3 t4)
+      FFT<T, N, -1>::run(acc);  // x: (kx, y = l, z = g)
+      __syncthreads();
+#pragma unroll
+      for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = acc[kx];
+      __syncthreads();
+#pragma unroll
+      for (int z = 0; z < N; ++z) v[z] = X[(g * N + z) * N + l];
+      FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
+      __syncthreads();
+#pragma unroll
+      for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
+      __syncthreads();
+#pragma unroll
+      for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
+      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+      const T w = static_cast<T>(a.wgt[x]);
+      {
+        const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
+                                     (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off]) * n + o);
+        if (x == x0) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G[k] = (w * b1[k]) * v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
+        }
+      }
+      if (two) {
+        const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
+                                     (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off]) * n + o);
+        if (x == x0) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G2[k] = (w * b2[k]) * v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // finalize: one item = (species p, point)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -589,6 +704,7 @@ struct VssHandle {
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent gain grid (0: occupancy)
   int algorithm = 0;       // 0: automatic, 1: multi-pass, 4: register-resident (FP32, Nv 16 / 24)
+  int iso = 0;             // eta = 0 for every pair: b_pq(x, y) independent of y
   int last_algorithm = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
   int timing = 0;
@@ -768,7 +884,8 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   if (vss_rr_kind<T, N>(h) == 4) {
     if constexpr (std::is_same<T, float>::value && (N == 16 || N == 24)) {
       const size_t lds = static_cast<size_t>(VssRR1Shape<N>::field) * sizeof(cpx<T>);
-      const void* kfn = reinterpret_cast<const void*>(k_vss_gain_rr1<T, N>);
+      const void* kfn = h->iso ? reinterpret_cast<const void*>(k_vss_gain_rr1_iso<T, N>)
+                               : reinterpret_cast<const void*>(k_vss_gain_rr1<T, N>);
       DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
       int occ = 1;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, VssRR1Shape<N>::NT, lds) != hipSuccess || occ < 1)
@@ -776,7 +893,11 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
       int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(occ) * cu_count()));
       if (h->slots_override > 0) grid = std::min(grid, h->slots_override);
       KTimerT<VssHandle> t1(h, 1, st);
-      hipLaunchKernelGGL((k_vss_gain_rr1<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
+      if (grid > h->cap_slots) grid = h->cap_slots;  // the isotropic kernel parks t3 in a scratch cube per workgroup
+      if (h->iso)
+        hipLaunchKernelGGL((k_vss_gain_rr1_iso<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
+      else
+        hipLaunchKernelGGL((k_vss_gain_rr1<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
       DGFS_LAUNCH_CHECK("vss_gain_rr1");
       t1.stop();
       h->last_slots = grid;
@@ -1100,6 +1221,9 @@ extern "C" int dgfs_vss_create(const dgfs_vss_desc* d, void** out) {
   DGFS_HIP_TRY(hipMemcpy(h->alpha, alpha.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->wgt, wgt.data(), h->Nrho * sizeof(double), hipMemcpyHostToDevice));
   DGFS_HIP_TRY(hipMemcpy(h->szd, d->sz, h->M * 3 * sizeof(double), hipMemcpyHostToDevice));
+  h->iso = 1;
+  for (int k = 0; k < h->npairs; ++k)
+    if (d->eta[k] != 0.0) h->iso = 0;
   int rc = vss_build_jobs(h.get(), d->sz);
   if (rc) return rc;
   rc = (h->prec == 64) ? vss_build_tables<double>(h.get(), d) : vss_build_tables<float>(h.get(), d);

@@ -232,19 +232,24 @@ def test_sharing_needs_antipodes(device):
         op.set_symmetry(True)
 
 
-@pytest.mark.parametrize("N,L,masses,sym", [(16, 5.0, (1.0, 2.1), True), (24, 6.0, (1.0, 2.1), True),
-                                            (24, 6.0, (1.0, 2.1), False), (16, 5.0, (1.0, 2.1, 0.7), True)])
-def test_register_resident_matches_multipass(N, L, masses, sym, device):
+@pytest.mark.parametrize("N,L,masses,sym,alpha", [(16, 5.0, (1.0, 2.1), True, 1.0), (24, 6.0, (1.0, 2.1), True, 1.0),
+                                                  (24, 6.0, (1.0, 2.1), False, 1.0),
+                                                  (16, 5.0, (1.0, 2.1, 0.7), True, 1.0),
+                                                  (24, 6.0, (1.0, 2.1), True, 1.3),
+                                                  (16, 5.0, (1.0, 2.1), False, 1.3)])
+def test_register_resident_matches_multipass(N, L, masses, sym, alpha, device):
     """The register-resident FP32 gain (algorithm 4) and the multi-pass FP32
     gain on the same handle and states, both against the FP64 operator: the
     same sums with other FP32 roundings, so the register-resident error is
     within 1e-4 and within twice the multi-pass error, and the two FP32
-    results agree to 5e-5."""
+    results agree to 5e-5.  alpha = 1 (eta = 0) takes the isotropic kernel,
+    which sums the direction products before one forward transform per
+    radial node; alpha = 1.3 the general one."""
     S = len(masses)
     Nrho, M, Mpre = 6, 12, 24
     omegas = tuple(0.81 - 0.005 * (p + q) for p in range(S) for q in range(S))
-    op, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=32)
-    op64, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=64)
+    op, _ = make_op(N, L, Nrho, M, Mpre, masses, alpha, omegas, (1.0,) * (S * S), precision=32)
+    op64, _ = make_op(N, L, Nrho, M, Mpre, masses, alpha, omegas, (1.0,) * (S * S), precision=64)
     op.set_symmetry(sym)
     fs = _mixture_states(V.BiVelocityMesh(N, L, Nrho, M, masses), 5, seed=31)
     Q64 = run_contig(op64, fs, device)
