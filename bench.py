@@ -219,11 +219,13 @@ def bench_vss(args, dev, stream):
     op32.set_timing(False)
     gain_ms, gain_n = tim["gain"]
     algo = op32.get_algorithm()
-    kname = f"k_vss_gain_rr1<float,{nv}>" if algo == 4 else f"k_vss_gain<float,{nv}>"
+    kname = {4: f"k_vss_gain_rr1<float,{nv}>", 5: f"k_vss_gain_rr1_iso<float,{nv}>"}.get(
+        algo, f"k_vss_gain<float,{nv}>")
     bq = vss_bytes(nv, K, S, 8)
     achieved = bq * P * args.vss_steps / (gain_ms / 1e3)
     # HBM bytes per launch from the committed counter runs, scaled to this batch
-    tr = read_traffic(f"vss_gain_rr1_fp32_n{nv}_traffic.json" if algo == 4 else "")
+    tr = read_traffic({4: f"vss_gain_rr1_fp32_n{nv}_traffic.json",
+                       5: f"vss_gain_rr1_iso_fp32_n{nv}_traffic.json"}.get(algo, ""))
     traffic = (tr["bytes_per_dispatch"] * P * args.vss_steps / max(gain_n, 1) / tr["points_per_dispatch"]
                if tr else None)
     out = {

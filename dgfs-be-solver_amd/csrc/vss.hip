@@ -128,6 +128,7 @@ struct VssGainArgs {
   cpx<T>* Ghat;      // [npairs][npts][ngroups][n], pair index p * nspcs + q
   cpx<T>* Z1;        // scratch [nslots][n]
   cpx<T>* Z2;        // scratch [nslots][n]
+  cpx<T>* Z3;        // scratch [nslots][n]: sum over directions (isotropic tables)
   const double* alpha;  // [Nrho] (pi/L) qz_x
   const double* wgt;    // [Nrho] qw_x sw / n
   const double* sz;     // [M][3]
@@ -136,9 +137,10 @@ struct VssGainArgs {
   const int* rows2;
   VssJob<T> job[kVssMaxPairs];
   int nitems, npts, ngroups, ppg, Nrho;
+  int iso;              // eta = 0 everywhere: sum the products over directions, one forward transform per x
 };
 
-template <typename T, int N>
+template <typename T, int N, bool ISO>
 __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
   constexpr int NT = Shape<N>::NT, PB = Shape<N>::PB, NN = N * N, n = N * N * N, h = N / 2;
   __shared__ cpx<T> lds[PB * N * (N + 1)];
@@ -146,6 +148,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
   const int tid = threadIdx.x;
   cpx<T>* Z1 = a.Z1 + static_cast<size_t>(blockIdx.x) * n;
   cpx<T>* Z2 = a.Z2 + static_cast<size_t>(blockIdx.x) * n;
+  cpx<T>* Z3 = a.Z3 + static_cast<size_t>(blockIdx.x) * n;
   const int per_job = a.npts * a.ngroups;
 
   for (int item = blockIdx.x; item < a.nitems; item += gridDim.x) {
@@ -205,6 +208,26 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
               },
               [&](int j, int yy, int z, cpx<T> v) { Z2[(kx0 + j) * NN + yy * N + z] = v; });
         }
+        if constexpr (ISO) {
+          // S_x += t3 t4 (thread-private pencils: the same idx mapping in every pass)
+          if (iy == 0) {
+            pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z2[kx * NN + idx]; },
+                                      [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                        for (int xx = 0; xx < N; ++xx) Z3[xx * NN + idx] = Z1[xx * NN + idx] * v[xx];
+                                      });
+          } else {
+            pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z2[kx * NN + idx]; },
+                                      [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                        for (int xx = 0; xx < N; ++xx) {
+                                          const cpx<T> t = Z1[xx * NN + idx] * v[xx];
+                                          Z3[xx * NN + idx] = Z3[xx * NN + idx] + t;
+                                        }
+                                      });
+          }
+          continue;
+        }
         pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z2[kx * NN + idx]; },
                                   [&](int idx, cpx<T>* v) {
 #pragma unroll
@@ -239,6 +262,26 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_gain(VssGainArgs<T> a) {
                   G[o] = G[o] + (w * bxy[o]) * v;
                 });
           }
+        }
+      }
+      if constexpr (ISO) {
+        // G^ += w b(x) FFT(S_x), the tables read at the job's first direction row
+        pencil_pass<T, N, NT, -1>([&](int xx, int idx) { return Z3[xx * NN + idx]; },
+                                  [&](int idx, cpx<T>* v) {
+#pragma unroll
+                                    for (int k = 0; k < N; ++k) Z2[k * NN + idx] = v[k];
+                                  });
+        const cpx<T>* bx = jb.tab1 + (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off]) * n;
+        const cpx<T>* bx2 = G2 ? jb.tab2 + (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off]) * n : nullptr;
+        for (int kx0 = 0; kx0 < N; kx0 += PB) {
+          const int cnt = min(PB, N - kx0);
+          slab_batch<T, N, NT, -1>(
+              lds, cnt, [&](int jj, int r, int c) { return Z2[(kx0 + jj) * NN + r * N + c]; },
+              [&](int jj, int r, int c, cpx<T> v) {
+                const int o = (kx0 + jj) * NN + r * N + c;
+                G[o] = G[o] + (w * bx[o]) * v;
+                if (bx2) G2[o] = G2[o] + (w * bx2[o]) * v;
+              });
         }
       }
     }
@@ -701,6 +744,7 @@ struct VssHandle {
   void* Ghat = nullptr;  // [cap_items][n] cpx<T>
   void* Z1 = nullptr;
   void* Z2 = nullptr;
+  void* Z3 = nullptr;
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent gain grid (0: occupancy)
   int algorithm = 0;       // 0: automatic, 1: multi-pass, 4: register-resident (FP32, Nv 16 / 24)
@@ -722,7 +766,7 @@ struct VssHandle {
     (void)hipFree(dirs_d); (void)hipFree(rows1_d); (void)hipFree(rows2_d);
     (void)hipFree(alpha); (void)hipFree(wgt); (void)hipFree(szd);
     (void)hipFree(fc); (void)hipFree(Qc); (void)hipFree(F); (void)hipFree(Ghat);
-    (void)hipFree(Z1); (void)hipFree(Z2);
+    (void)hipFree(Z1); (void)hipFree(Z2); (void)hipFree(Z3);
     for (auto& pr : pending) {
       (void)hipEventDestroy(pr.second.first);
       (void)hipEventDestroy(pr.second.second);
@@ -758,8 +802,12 @@ static int invert_n(const double* A, double* Ainv, int m) {
 template <typename T, int N>
 static int vss_slots() {
   int o1 = 1, o2 = 1, o3 = 1;
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, reinterpret_cast<const void*>(k_vss_gain<T, N>),
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, reinterpret_cast<const void*>(k_vss_gain<T, N, false>),
                                                      Shape<N>::NT, 0);
+  int o4 = 1;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, reinterpret_cast<const void*>(k_vss_gain<T, N, true>),
+                                                     Shape<N>::NT, 0);
+  o1 = std::max(o1, o4);
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, reinterpret_cast<const void*>(k_vhs_prep<T, N>),
                                                      Shape<N>::NT, 0);
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, reinterpret_cast<const void*>(k_vss_finalize<T, N>),
@@ -774,6 +822,7 @@ static int vss_ensure_ws(VssHandle* h, int pts) {
     const int slots = vss_slots<T, N>();
     DGFS_HIP_TRY(hipMalloc(&h->Z1, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
     DGFS_HIP_TRY(hipMalloc(&h->Z2, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
+    if (h->iso) DGFS_HIP_TRY(hipMalloc(&h->Z3, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
     h->cap_slots = slots;
   }
   if (static_cast<size_t>(pts) > h->cap_pts) {
@@ -854,6 +903,8 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   ga.Ghat = static_cast<cpx<T>*>(h->Ghat);
   ga.Z1 = static_cast<cpx<T>*>(h->Z1);
   ga.Z2 = static_cast<cpx<T>*>(h->Z2);
+  ga.Z3 = static_cast<cpx<T>*>(h->Z3);
+  ga.iso = h->iso;
   ga.alpha = h->alpha;
   ga.wgt = h->wgt;
   ga.sz = h->szd;
@@ -902,12 +953,16 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
       t1.stop();
       h->last_slots = grid;
       h->threads = VssRR1Shape<N>::NT;
-      h->last_algorithm = 4;
+      h->last_algorithm = h->iso ? 5 : 4;
     }
   } else {
     KTimerT<VssHandle> t1(h, 1, st);
-    hipLaunchKernelGGL((k_vss_gain<T, N>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
-                       dim3(Shape<N>::NT), 0, st, ga);
+    if (h->iso)
+      hipLaunchKernelGGL((k_vss_gain<T, N, true>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
+                         dim3(Shape<N>::NT), 0, st, ga);
+    else
+      hipLaunchKernelGGL((k_vss_gain<T, N, false>), dim3(static_cast<int>(std::min<size_t>(items, slots))),
+                         dim3(Shape<N>::NT), 0, st, ga);
     DGFS_LAUNCH_CHECK("vss_gain");
     t1.stop();
     h->last_algorithm = 1;

@@ -87,7 +87,8 @@ class VSSOperator:
         _lib.call("dgfs_vss_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):
-        """Gain kernel of the last call (1 or 4)."""
+        """Gain kernel of the last call: 1 multi-pass, 4 register-resident,
+        5 register-resident for isotropic scattering (eta = 0)."""
         v = C.c_int()
         _lib.call("dgfs_vss_get_algorithm", self._h, C.byref(v))
         return v.value

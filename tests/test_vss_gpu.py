@@ -255,7 +255,7 @@ def test_register_resident_matches_multipass(N, L, masses, sym, alpha, device):
     Q64 = run_contig(op64, fs, device)
     op.set_algorithm(4)
     Qr = run_contig(op, fs, device)
-    assert op.get_algorithm() == 4
+    assert op.get_algorithm() == (5 if alpha == 1.0 else 4)
     op.set_algorithm(1)
     Qm = run_contig(op, fs, device)
     assert op.get_algorithm() == 1
