@@ -477,11 +477,14 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGain
   cpx<T>* Xp = X + g * N * PP;  // kx plane g
   // t3 parking: thread-contiguous pencils (one base register, immediate offsets)
   cpx<T>* U = a.Z1 + static_cast<size_t>(blockIdx.x) * n + static_cast<size_t>(tid) * N;
-  const int per_job = a.npts * a.ngroups;
+  // items run point-major: the jobs of one point sit on consecutive slots,
+  // which xcd_slot() keeps on one XCD, so the point's spectra (read for
+  // every product) are shared in that L2
+  const int per_pt = (a.nitems / (a.npts * a.ngroups)) * a.ngroups;
   cpx<T> acc[N], v[N];
   for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
-    const int j = item / per_job, rem = item - j * per_job;
-    const int pt = rem / a.ngroups, grp = rem - pt * a.ngroups;
+    const int pt = item / per_pt, rem = item - pt * per_pt;
+    const int j = rem / a.ngroups, grp = rem - j * a.ngroups;
     const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
     const VssJob<T>& jb = a.job[j];
     const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
