@@ -435,7 +435,8 @@ def surrogate_eval(args, vm, sur, exact32, f, dev, stream):
     per_thr = {}
     ind_hi = mom[:, 6]
     cases = [(args.sur_threshold, args.sur_threshold_hi), (0.02, 0.0), (0.05, 0.0), (0.1, args.sur_threshold_hi),
-             (0.02, 1e-4), (0.005, 1e-4), (0.002, 0.0), (0.001, 0.0), (0.02, 1e-5), (0.005, 1e-5)]
+             (0.02, 1e-4), (0.005, 1e-4), (0.002, 0.0), (0.001, 0.0), (0.02, 1e-5), (0.005, 1e-5),
+             (0.02, 3e-6), (0.02, 1e-6), (0.02, 3e-7)]
     seen = []
     for thr, thh in cases:
         if (thr, thh) in seen:
