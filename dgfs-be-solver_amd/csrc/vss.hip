@@ -311,7 +311,12 @@ struct VssRR1Shape {
   static constexpr int NT = N * N;
   static constexpr int PP = N + 1;          // padded row pitch of the kx planes
   static constexpr int field = N * N * PP;  // complex; holds the N^3 exchange too
-  static_assert(field >= N * N * N, "exchange must fit the plane buffer");
+  // kx stride of the field exchange: N (N + 1) complex, so that the rows of
+  // consecutive kx that one wave touches land on different LDS banks (a
+  // stride of N^2, a multiple of 32 complex at N = 16 and 24, puts them on
+  // the same banks); the exchange then fills the plane buffer exactly
+  static constexpr int KS = N * PP;
+  static_assert(field >= (N - 1) * KS + N * N, "exchange must fit the plane buffer");
   // values of t3 parked in LDS while t4 is transformed (frees registers where
   // the 576-thread workgroup's 168-VGPR limit would spill; the rest of the
   // 160 KiB of LDS holds them)
@@ -339,7 +344,7 @@ struct StridedPencil {
 template <typename T, int N>
 __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Fp, const cpx<T> (*e)[N],
                                                 cpx<T>* X, cpx<T>* Xp, int g, int l) {
-  constexpr int PP = VssRR1Shape<N>::PP, NN = N * N;
+  constexpr int PP = VssRR1Shape<N>::PP, NN = N * N, KS = VssRR1Shape<N>::KS;
   const StridedPencil<T, N> src(const_cast<cpx<T>*>(Fp) + g * NN + l);
 #pragma unroll
   for (int ky = 0; ky < N; ++ky) v[ky] = src[ky];
@@ -355,16 +360,17 @@ __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __
   FFT<T, N, +1>::run(v);  // z: (kx = g, y = l, z)
   __syncthreads();
 #pragma unroll
-  for (int z = 0; z < N; ++z) X[(g * N + z) * N + l] = v[z];
+  for (int z = 0; z < N; ++z) X[g * KS + z * N + l] = v[z];
   __syncthreads();
 #pragma unroll
-  for (int kx = 0; kx < N; ++kx) v[kx] = X[(kx * N + g) * N + l];
+  for (int kx = 0; kx < N; ++kx) v[kx] = X[kx * KS + g * N + l];
   FFT<T, N, +1>::run(v);  // x: (x, y = l, z = g)
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs<T> a) {
   constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KS = VssRR1Shape<N>::KS;
   constexpr int KP = VssRR1Shape<N>::park;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -422,10 +428,10 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
         FFT<T, N, -1>::run(u);  // x: (kx, y = l, z = g)
         __syncthreads();
 #pragma unroll
-        for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = u[kx];
+        for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = u[kx];
         __syncthreads();
 #pragma unroll
-        for (int z = 0; z < N; ++z) v[z] = X[(g * N + z) * N + l];
+        for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
         FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
         __syncthreads();
 #pragma unroll
@@ -467,6 +473,7 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
 template <typename T, int N>
 __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGainArgs<T> a) {
   constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KS = VssRR1Shape<N>::KS;
   constexpr int KP = VssRR1Shape<N>::park;  // accumulator values kept in the spare LDS
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -530,10 +537,10 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGain
       FFT<T, N, -1>::run(acc);  // x: (kx, y = l, z = g)
       __syncthreads();
 #pragma unroll
-      for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = acc[kx];
+      for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = acc[kx];
       __syncthreads();
 #pragma unroll
-      for (int z = 0; z < N; ++z) v[z] = X[(g * N + z) * N + l];
+      for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
       FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
       __syncthreads();
 #pragma unroll
