@@ -721,12 +721,16 @@ struct RR1Shape {
   static constexpr int PP = N + 1;          // padded row pitch of the kx planes
   static constexpr int field = N * N * PP;  // complex; holds the N^3 exchange too
   static constexpr int wl = 3 * N * N;      // complex, Nyquist planes
-  static_assert(field >= N * N * N, "exchange must fit the plane buffer");
+  // kx stride of the field exchange: N (N + 1) complex, so that the rows of
+  // consecutive kx that one wave touches fall on different LDS banks
+  static constexpr int KS = N * PP;
+  static_assert(field >= (N - 1) * KS + N * N, "exchange must fit the plane buffer");
 };
 
 template <typename T, int N>
 __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> a) {
   constexpr int NT = RR1Shape<N>::NT, PP = RR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KS = RR1Shape<N>::KS;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* Wl = X + RR1Shape<N>::field;
@@ -790,10 +794,10 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
         FFT<T, N, +1>::run(v);  // y: (kx = g, y, z = l)
         __syncthreads();
 #pragma unroll
-        for (int y = 0; y < N; ++y) X[(g * N + y) * N + l] = v[y];
+        for (int y = 0; y < N; ++y) X[g * KS + y * N + l] = v[y];
         __syncthreads();
 #pragma unroll
-        for (int kx = 0; kx < N; ++kx) v[kx] = X[(kx * N + g) * N + l];
+        for (int kx = 0; kx < N; ++kx) v[kx] = X[kx * KS + g * N + l];
         FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
         {
           const cpx<T> w1 = Wl[g * N + l];
@@ -816,10 +820,10 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
       FFT<T, N, -1>::run(v);  // x: (kx, y = g, z = l)
       __syncthreads();
 #pragma unroll
-      for (int kx = 0; kx < N; ++kx) X[(kx * N + g) * N + l] = v[kx];
+      for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = v[kx];
       __syncthreads();
 #pragma unroll
-      for (int y = 0; y < N; ++y) v[y] = X[(g * N + y) * N + l];
+      for (int y = 0; y < N; ++y) v[y] = X[g * KS + y * N + l];
       FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
       __syncthreads();
 #pragma unroll
