@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--vss-nv", type=int, default=24)
     ap.add_argument("--vss-L", type=float, default=6.0)
     ap.add_argument("--vss-cpu-points", type=int, default=8)
+    ap.add_argument("--vss-algorithm", type=int, default=0,
+                    help="two-species gain kernel (0 automatic, 1 multi-pass, 4 register-resident, "
+                         "6 packed isotropic)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the bench, run 3 plain device copies (torch elementwise "
                          "multiply by 1) of this many MiB, so that a counter run sees a "
@@ -206,6 +209,8 @@ def bench_vss(args, dev, stream):
     ptr = lambda a: [x.data_ptr() for x in a]
     op32 = VSSOperator(nv, Nrho, M, vm.L(), vm.sw(), vm.cw(), vm.qz(), vm.qw(), vm.sz(), szpre,
                        vm.masses(), gamma, eta, prefac, precision=32, project=True)
+    if args.vss_algorithm:
+        op32.set_algorithm(args.vss_algorithm)
     op32.eval_contig(ptr(fs), ptr(Qs32), P, stream)
     torch.cuda.synchronize()
     op32.set_timing(True)
@@ -219,13 +224,14 @@ def bench_vss(args, dev, stream):
     op32.set_timing(False)
     gain_ms, gain_n = tim["gain"]
     algo = op32.get_algorithm()
-    kname = {4: f"k_vss_gain_rr1<float,{nv}>", 5: f"k_vss_gain_rr1_iso<float,{nv}>"}.get(
-        algo, f"k_vss_gain<float,{nv}>")
+    kname = {4: f"k_vss_gain_rr1<float,{nv}>", 5: f"k_vss_gain_rr1_iso<float,{nv}>",
+             6: f"k_vss_gain_rr1_isop<float,{nv}>"}.get(algo, f"k_vss_gain<float,{nv}>")
     bq = vss_bytes(nv, K, S, 8)
     achieved = bq * P * args.vss_steps / (gain_ms / 1e3)
     # HBM bytes per launch from the committed counter runs, scaled to this batch
     tr = read_traffic({4: f"vss_gain_rr1_fp32_n{nv}_traffic.json",
-                       5: f"vss_gain_rr1_iso_fp32_n{nv}_traffic.json"}.get(algo, ""))
+                       5: f"vss_gain_rr1_iso_fp32_n{nv}_traffic.json",
+                       6: f"vss_gain_rr1_isop_fp32_n{nv}_traffic.json"}.get(algo, ""))
     traffic = (tr["bytes_per_dispatch"] * P * args.vss_steps / max(gain_n, 1) / tr["points_per_dispatch"]
                if tr else None)
     out = {

@@ -341,16 +341,28 @@ struct StridedPencil {
   __device__ __forceinline__ cpx<T>& operator[](int k) const { return (k < KB) ? p0[k * N] : p1[(k - KB) * N]; }
 };
 
+// inverse transform of a phased y-pencil (kx = g, kz = l) already in v:
+// y-IFFT, plane exchange, z-IFFT, field exchange, x-IFFT; returns the
+// x-pencil (x, y = l, z = g)
+template <typename T, int N>
+__device__ __forceinline__ void vss_rr1_inverse_tail(cpx<T> (&v)[N], cpx<T>* X, cpx<T>* Xp, int g, int l);
+
 template <typename T, int N>
 __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Fp, const cpx<T> (*e)[N],
                                                 cpx<T>* X, cpx<T>* Xp, int g, int l) {
-  constexpr int PP = VssRR1Shape<N>::PP, NN = N * N, KS = VssRR1Shape<N>::KS;
+  constexpr int NN = N * N;
   const StridedPencil<T, N> src(const_cast<cpx<T>*>(Fp) + g * NN + l);
 #pragma unroll
   for (int ky = 0; ky < N; ++ky) v[ky] = src[ky];
   const cpx<T> exz = e[0][g] * e[2][l];
 #pragma unroll
   for (int ky = 0; ky < N; ++ky) v[ky] = v[ky] * (exz * e[1][ky]);
+  vss_rr1_inverse_tail<T, N>(v, X, Xp, g, l);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void vss_rr1_inverse_tail(cpx<T> (&v)[N], cpx<T>* X, cpx<T>* Xp, int g, int l) {
+  constexpr int PP = VssRR1Shape<N>::PP, KS = VssRR1Shape<N>::KS;
   FFT<T, N, +1>::run(v);  // y: (kx = g, y, kz = l)
 #pragma unroll
   for (int y = 0; y < N; ++y) Xp[y * PP + l] = v[y];
@@ -530,6 +542,193 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGain
         for (int xx = 0; xx < N - KP; ++xx) acc[xx] = acc[xx] + U[xx] * v[xx];
 #pragma unroll
         for (int k = 0; k < KP; ++k) apark[k][tid] = apark[k][tid] + U[N - KP + k] * v[N - KP + k];
+      }
+#pragma unroll
+      for (int k = 0; k < KP; ++k) acc[N - KP + k] = apark[k][tid];
+      // G^ += w b(x) FFT(sum_y t3 t4)
+      FFT<T, N, -1>::run(acc);  // x: (kx, y = l, z = g)
+      __syncthreads();
+#pragma unroll
+      for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = acc[kx];
+      __syncthreads();
+#pragma unroll
+      for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
+      FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
+      __syncthreads();
+#pragma unroll
+      for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
+      __syncthreads();
+#pragma unroll
+      for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
+      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+      const T w = static_cast<T>(a.wgt[x]);
+      {
+        const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
+                                     (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off]) * n + o);
+        if (x == x0) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G[k] = (w * b1[k]) * v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
+        }
+      }
+      if (two) {
+        const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
+                                     (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off]) * n + o);
+        if (x == x0) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G2[k] = (w * b2[k]) * v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Packed isotropic register-resident gain (FP32, N = 16, 24).  For real
+// f_p, f_q the fields t3 = IFFT(E1 F_p) and t4 = IFFT(E2 F_q) are real except
+// for what their Nyquist planes contribute: with phi the phase of E and
+// delta(k) = phi(k) + phi(-k) (zero unless a component of k is N/2, where
+// lfreq maps -k back to -N/2),
+//   Im t = IFFT(D),  D(k) = F(k) E(k) (1 - exp(-i delta(k))) / (2i),
+// which lives on the three Nyquist planes.  Each plane's part is a 2D
+// inverse transform broadcast along the third axis with the sign (-1)^x,
+// (-1)^y or (-1)^z (owner order x, then y, then z, so no k is counted
+// twice):  Im t(x, y, z) = (-1)^x W0(y, z) + (-1)^y W1(x, z) + (-1)^z W2(x, y).
+// One complex transform P = IFFT(E1 F_p + i E2 F_q) = t3 + i t4 then gives
+//   Re t3 = Re P + Im t4,   Re t4 = Im P - Im t3,
+// so a direction product costs one 3D inverse transform and six N x N 2D
+// transforms instead of two 3D inverse transforms, with the same t3 t4 (exact
+// in exact arithmetic).  The sum over directions, the forward transform per
+// radial node and the G^ accumulation are those of k_vss_gain_rr1_iso.
+// ---------------------------------------------------------------------------
+// the packed isotropic kernel is the automatic choice once it is measured faster
+constexpr bool kVssPackDefault = false;
+
+template <int N>
+struct VssPackShape {
+  static constexpr int park = (N == 24) ? 4 : 0;  // accumulator values kept in the spare LDS
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGainArgs<T> a) {
+  constexpr int NT = VssRR1Shape<N>::NT, PP = VssRR1Shape<N>::PP, NN = N * N, n = NN * N;
+  constexpr int KS = VssRR1Shape<N>::KS, h = N / 2, KP = VssPackShape<N>::park;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ cpx<T> e1[3][N], e2[3][N];
+  __shared__ cpx<T> cs[2][8];       // (1 - exp(-i delta_S)) / (2i), S the Nyquist components (bit 0 x, 1 y, 2 z)
+  constexpr int PW = N + 1;         // padded row pitch of the W planes (spreads the row reads over the banks)
+  __shared__ cpx<T> Wp[6][N * PW];  // W0, W1, W2 of t3, then of t4
+  __shared__ cpx<T> apark[KP > 0 ? KP : 1][NT];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  cpx<T>* Xp = X + g * KS;  // kx plane g
+  const int per_pt = (a.nitems / (a.npts * a.ngroups)) * a.ngroups;
+  cpx<T> acc[N], v[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int pt = item / per_pt, rem = item - pt * per_pt;
+    const int j = rem / a.ngroups, grp = rem - j * a.ngroups;
+    const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
+    const VssJob<T>& jb = a.job[j];
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const double mu1 = jb.mu1, mu2 = jb.mu2;
+    const size_t o = static_cast<size_t>(g) * NN + l;  // this thread's G^ pencil (kx = g, kz = l), stride N
+    const bool two = jb.out2 >= 0;
+    const StridedPencil<T, N> G(a.Ghat + ((static_cast<size_t>(jb.out1) * a.npts + pt) * a.ngroups + grp) * n + o);
+    const StridedPencil<T, N> G2(a.Ghat + ((static_cast<size_t>(two ? jb.out2 : jb.out1) * a.npts + pt) *
+                                               a.ngroups + grp) * n + o);
+    for (int x = x0; x < x1; ++x) {
+      const double alpha = a.alpha[x];
+#pragma unroll
+      for (int k = 0; k < N - KP; ++k) acc[k] = cpx<T>{T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < KP; ++k) apark[k][tid] = cpx<T>{T(0), T(0)};  // thread-private slots
+      for (int iy = 0; iy < jb.ndir; ++iy) {
+        const int y = a.dirs[jb.dir_off + iy];
+        __syncthreads();  // readers of the tables, W planes and LDS field of the previous product are done
+        if (tid < 3 * N) {
+          const int d = tid / N, k = tid - d * N;
+          const double th = alpha * a.sz[y * 3 + d] * lfreq(k, N);
+          double sn, c0;
+          sincos(mu1 * th, &sn, &c0);
+          e1[d][k] = cpx<T>{static_cast<T>(c0), static_cast<T>(sn)};
+          sincos(mu2 * th, &sn, &c0);
+          e2[d][k] = cpx<T>{static_cast<T>(c0), static_cast<T>(-sn)};
+        } else if (tid < 3 * N + 16) {
+          const int f = (tid - 3 * N) >> 3, S = (tid - 3 * N) & 7;
+          double sum = 0.0;
+          for (int d = 0; d < 3; ++d)
+            if (S & (1 << d)) sum += a.sz[y * 3 + d];
+          // delta = phi(k) + phi(-k): -mu1 alpha N sum for t3, +mu2 alpha N sum for t4
+          const double delta = (f == 0 ? -mu1 : mu2) * alpha * N * sum;
+          double sn, c0;
+          sincos(delta, &sn, &c0);
+          cs[f][S] = cpx<T>{static_cast<T>(0.5 * sn), static_cast<T>(-0.5 * (1.0 - c0))};
+        }
+        __syncthreads();
+        // D on the six Nyquist planes (plane q = 3 f + owner, element (i, jj)),
+        // every thread a few elements so that the F loads overlap
+        for (int e = tid; e < 6 * NN; e += NT) {
+          const int q = e / NN, ij = e - q * NN, i = ij / N, jj = ij - i * N, f = q / 3, ow = q - 3 * f;
+          int kx, ky, kz;
+          bool zero;
+          if (ow == 0) { kx = h; ky = i; kz = jj; zero = false; }
+          else if (ow == 1) { kx = i; ky = h; kz = jj; zero = (i == h); }
+          else { kx = i; ky = jj; kz = h; zero = (i == h) || (jj == h); }
+          const int S = (kx == h ? 1 : 0) | (ky == h ? 2 : 0) | (kz == h ? 4 : 0);
+          const cpx<T>* Fs = f ? F2 : F1;
+          const cpx<T> (*ef)[N] = f ? e2 : e1;
+          Wp[q][i * PW + jj] = zero ? cpx<T>{T(0), T(0)}
+                           : Fs[kx * NN + ky * N + kz] * ((ef[0][kx] * ef[1][ky] * ef[2][kz]) * cs[f][S]);
+        }
+        __syncthreads();
+        // W planes, rows
+        if (tid < 6 * N) {
+          const int q = tid / N, i = tid - q * N;
+#pragma unroll
+          for (int jj = 0; jj < N; ++jj) v[jj] = Wp[q][i * PW + jj];
+          FFT<T, N, +1>::run(v);
+#pragma unroll
+          for (int jj = 0; jj < N; ++jj) Wp[q][i * PW + jj] = v[jj];
+        }
+        __syncthreads();
+        // W planes, columns
+        if (tid < 6 * N) {
+          const int q = tid / N, jj = tid - q * N;
+#pragma unroll
+          for (int i = 0; i < N; ++i) v[i] = Wp[q][i * PW + jj];
+          FFT<T, N, +1>::run(v);
+#pragma unroll
+          for (int i = 0; i < N; ++i) Wp[q][i * PW + jj] = v[i];
+        }
+        // packed y-pencil (kx = g, kz = l): E1 F_p + i E2 F_q
+        {
+          const StridedPencil<T, N> s1(const_cast<cpx<T>*>(F1) + g * NN + l);
+          const StridedPencil<T, N> s2(const_cast<cpx<T>*>(F2) + g * NN + l);
+          const cpx<T> ea = e1[0][g] * e1[2][l], eb = e2[0][g] * e2[2][l];
+#pragma unroll
+          for (int ky = 0; ky < N; ++ky) v[ky] = s1[ky] * (ea * e1[1][ky]) + mul_i(s2[ky] * (eb * e2[1][ky]));
+        }
+        vss_rr1_inverse_tail<T, N>(v, X, Xp, g, l);  // P at (x, y = l, z = g); the tail's barriers order the W columns too
+        {
+          const T sl = (l & 1) ? T(-1) : T(1), sg = (g & 1) ? T(-1) : T(1);
+          const T w30 = Wp[0][l * PW + g].x, w40 = Wp[3][l * PW + g].x;
+#pragma unroll
+          for (int xx = 0; xx < N; ++xx) {
+            const T sx = (xx & 1) ? T(-1) : T(1);
+            const T i3 = sx * w30 + sl * Wp[1][xx * PW + g].x + sg * Wp[2][xx * PW + l].x;
+            const T i4 = sx * w40 + sl * Wp[4][xx * PW + g].x + sg * Wp[5][xx * PW + l].x;
+            const cpx<T> t3{v[xx].x + i4, i3}, t4{v[xx].y - i3, i4};
+            if (xx < N - KP) acc[xx] = acc[xx] + t3 * t4;
+            else apark[xx - (N - KP)][tid] = apark[xx - (N - KP)][tid] + t3 * t4;
+          }
+        }
       }
 #pragma unroll
       for (int k = 0; k < KP; ++k) acc[N - KP + k] = apark[k][tid];
@@ -865,7 +1064,7 @@ static int vss_chunk_points(const VssHandle* h, int total) {
 template <typename T, int N>
 static int vss_rr_kind(const VssHandle* h) {
   constexpr bool rr_ok = std::is_same<T, float>::value && (N == 16 || N == 24);
-  if (!rr_ok || h->algorithm == 1) return 1;
+  if (!rr_ok || h->algorithm == 1) return 1;  // algorithm 6 is validated to FP32 at 16 / 24 when set
   return 4;
 }
 
@@ -945,8 +1144,10 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   if (vss_rr_kind<T, N>(h) == 4) {
     if constexpr (std::is_same<T, float>::value && (N == 16 || N == 24)) {
       const size_t lds = static_cast<size_t>(VssRR1Shape<N>::field) * sizeof(cpx<T>);
-      const void* kfn = h->iso ? reinterpret_cast<const void*>(k_vss_gain_rr1_iso<T, N>)
-                               : reinterpret_cast<const void*>(k_vss_gain_rr1<T, N>);
+      const bool pack = h->iso && (h->algorithm == 6 || (h->algorithm == 0 && kVssPackDefault));
+      const void* kfn = pack ? reinterpret_cast<const void*>(k_vss_gain_rr1_isop<T, N>)
+                        : h->iso ? reinterpret_cast<const void*>(k_vss_gain_rr1_iso<T, N>)
+                                 : reinterpret_cast<const void*>(k_vss_gain_rr1<T, N>);
       DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
       int occ = 1;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, VssRR1Shape<N>::NT, lds) != hipSuccess || occ < 1)
@@ -955,7 +1156,9 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
       if (h->slots_override > 0) grid = std::min(grid, h->slots_override);
       KTimerT<VssHandle> t1(h, 1, st);
       if (grid > h->cap_slots) grid = h->cap_slots;  // the isotropic kernel parks t3 in a scratch cube per workgroup
-      if (h->iso)
+      if (pack)
+        hipLaunchKernelGGL((k_vss_gain_rr1_isop<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
+      else if (h->iso)
         hipLaunchKernelGGL((k_vss_gain_rr1_iso<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
       else
         hipLaunchKernelGGL((k_vss_gain_rr1<T, N>), dim3(grid), dim3(VssRR1Shape<N>::NT), lds, st, ga);
@@ -963,7 +1166,7 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
       t1.stop();
       h->last_slots = grid;
       h->threads = VssRR1Shape<N>::NT;
-      h->last_algorithm = h->iso ? 5 : 4;
+      h->last_algorithm = pack ? 6 : h->iso ? 5 : 4;
     }
   } else {
     KTimerT<VssHandle> t1(h, 1, st);
@@ -1391,10 +1594,13 @@ extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
 extern "C" int dgfs_vss_set_algorithm(void* p, int algorithm) {
   VssHandle* h = as_vss(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
-  if (algorithm != 0 && algorithm != 1 && algorithm != 4)
-    return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0 (automatic), 1 (multi-pass) or 4 (register-resident)");
-  if (algorithm == 4 && !(h->prec == 32 && (h->N == 16 || h->N == 24)))
+  if (algorithm != 0 && algorithm != 1 && algorithm != 4 && algorithm != 6)
+    return fail(DGFS_ERR_INVALID_VALUE,
+                "algorithm must be 0 (automatic), 1 (multi-pass), 4 (register-resident) or 6 (packed isotropic)");
+  if ((algorithm == 4 || algorithm == 6) && !(h->prec == 32 && (h->N == 16 || h->N == 24)))
     return fail(DGFS_ERR_UNSUPPORTED, "the register-resident VSS gain is built for FP32 at Nv 16 and 24");
+  if (algorithm == 6 && !h->iso)
+    return fail(DGFS_ERR_UNSUPPORTED, "the packed VSS gain needs isotropic scattering (eta = 0 for every pair)");
   h->algorithm = algorithm;
   return DGFS_OK;
 }

@@ -177,10 +177,12 @@ int dgfs_vss_set_groups(void* handle, int ngroups);
 int dgfs_vss_set_slots(void* handle, int nslots);
 /* Gain kernel: 0 automatic (register-resident where built, else multi-pass),
  * 1 multi-pass (any precision and grid), 4 register-resident with one pencil
- * per thread (FP32 at Nv 16 and 24).  With eta = 0 for every pair (isotropic
- * scattering) both sum the direction products in space before one forward
- * transform per radial node.  get_algorithm reports the kernel of the last
- * call: 1, 4, or 5 for the isotropic register-resident kernel. */
+ * per thread (FP32 at Nv 16 and 24), 6 its isotropic form with t3 and t4 from
+ * one packed complex transform plus Nyquist-plane corrections (not the
+ * automatic choice: slower than 4 as measured).  With eta = 0 for every pair
+ * (isotropic scattering) 1 and 4 sum the direction products in space before
+ * one forward transform per radial node.  get_algorithm reports the kernel of
+ * the last call: 1, 4, 5 for the isotropic register-resident kernel, or 6. */
 int dgfs_vss_set_algorithm(void* handle, int algorithm);
 int dgfs_vss_get_algorithm(void* handle, int* algorithm);
 

@@ -266,6 +266,20 @@ def test_register_resident_matches_multipass(N, L, masses, sym, alpha, device):
     for p in range(S):
         assert er[p] < 1e-4 and er[p] < 2 * em[p] + 1e-6
     assert max(e) < 5e-5
+    if alpha == 1.0:
+        # the packed kernel: one complex inverse transform for t3 and t4 plus
+        # the Nyquist-plane corrections
+        op.set_algorithm(6)
+        Qk = run_contig(op, fs, device)
+        assert op.get_algorithm() == 6
+        ek = [rel(Qk[p], Q64[p]) for p in range(S)]
+        say("  packed vs FP64", ek)
+        for p in range(S):
+            assert ek[p] < 1e-4 and ek[p] < 2 * em[p] + 1e-6
+    else:
+        from dgfs_hip import _lib
+        with pytest.raises(_lib.DGFSUnsupported):
+            op.set_algorithm(6)
 
 
 def test_register_resident_needs_fp32_and_grid(device):
