@@ -1,4 +1,5 @@
-"""Launch-shape autotuner of the collision operator.
+"""Launch-shape autotuner of the collision operators (single-species VHS and
+two-species VSS).
 
 For a given operator (velocity grid, precision, K = M Nrho) and batch size it
 times every candidate of
@@ -21,6 +22,9 @@ _DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "dgfs_hip", "tu
 
 def _key(op, npts):
     bucket = 1 << max(0, int(npts - 1).bit_length())
+    if getattr(op, "kind", "vhs") == "vss":
+        return (f"vss-N{op.Nv}-p{op.precision}-K{op.K}-S{op.nspcs}-iso{int(bool(op.iso))}"
+                f"-proj{int(bool(op.project))}-pts{bucket}")
     return f"vhs-N{op.Nv}-p{op.precision}-K{op.K}-proj{int(bool(op.project))}-pts{bucket}"
 
 
@@ -49,6 +53,8 @@ class Autotuner:
 
     @staticmethod
     def candidates(op, cus):
+        if getattr(op, "kind", "vhs") == "vss":
+            return Autotuner._vss_candidates(op, cus)
         algos = [1]
         if op.precision == 32 and op.Nv in (16, 24):
             algos[:0] = [4]
@@ -63,6 +69,27 @@ class Autotuner:
         for a in algos:
             for gr in groups:
                 slots = [0] if a >= 2 else [0, cus, 2 * cus, 4 * cus]
+                for sl in slots:
+                    out.append({"algorithm": a, "groups": gr, "slots": sl})
+        return out
+
+    @staticmethod
+    def _vss_candidates(op, cus):
+        """Two-species operator: the register-resident gain (and its packed
+        form for isotropic scattering) where built, the multi-pass gain with
+        its persistent-grid caps; radial groups 1, 2, 4, ... for each."""
+        algos = [1]
+        if op.precision == 32 and op.Nv in (16, 24):
+            algos[:0] = [4, 6] if op.iso else [4]
+        groups = [0]
+        g = 1
+        while g <= op.Nrho:
+            groups.append(g)
+            g *= 2
+        out = []
+        for a in algos:
+            for gr in groups:
+                slots = [0] if a >= 4 else [0, cus, 2 * cus, 4 * cus]
                 for sl in slots:
                     out.append({"algorithm": a, "groups": gr, "slots": sl})
         return out

@@ -65,6 +65,10 @@ class VSSOperator:
         _lib.call("dgfs_vss_create", C.byref(desc), C.byref(h))
         self._h = h
         self.nspcs, self.Nv, self.vsize = S, Nv, Nv ** 3
+        # what the autotuner keys and chooses on
+        self.kind, self.precision, self.project = "vss", int(precision), bool(project)
+        self.Nrho, self.K = Nrho, Nrho * M
+        self.iso = bool(np.all(np.asarray(eta, dtype=float) == 0.0))
 
     def _ptrs(self, arrs):
         if len(arrs) != self.nspcs:

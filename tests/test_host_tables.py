@@ -146,3 +146,22 @@ def test_npeval_restrictions():
     assert np.allclose(got, [2.0, 3.0])
     cfg.set("s", "c", "2.5")
     assert np.allclose(eval_at_points(cfg, "s", "c", np.zeros((4, 2))), 2.5)
+
+
+def test_autotune_candidates_for_both_operators():
+    import types
+    from dgfs_hip.autotune import Autotuner, _key
+    vss = types.SimpleNamespace(kind="vss", Nv=24, precision=32, K=144, nspcs=2, iso=True, project=True, Nrho=12)
+    c = Autotuner.candidates(vss, 256)
+    algos = sorted({x["algorithm"] for x in c})
+    assert algos == [1, 4, 6]
+    assert all(x["slots"] == 0 for x in c if x["algorithm"] >= 4)
+    assert {x["groups"] for x in c} == {0, 1, 2, 4, 8}
+    vss.iso = False
+    assert sorted({x["algorithm"] for x in Autotuner.candidates(vss, 256)}) == [1, 4]
+    vss.precision = 64
+    assert sorted({x["algorithm"] for x in Autotuner.candidates(vss, 256)}) == [1]
+    assert _key(vss, 2048).startswith("vss-N24-p64-K144-S2-iso0")
+    vhs = types.SimpleNamespace(Nv=32, precision=32, K=192, project=True, Nrho=16)
+    assert sorted({x["algorithm"] for x in Autotuner.candidates(vhs, 256)}) == [1, 2, 3]
+    assert _key(vhs, 4096) == "vhs-N32-p32-K192-proj1-pts4096"

@@ -292,3 +292,27 @@ def test_register_resident_needs_fp32_and_grid(device):
         op8.set_algorithm(4)
     with pytest.raises(_lib.DGFSInvalidValue):
         op8.set_algorithm(3)
+
+
+def test_autotune_two_species(device, tmp_path):
+    """The autotuner on the two-species operator: every candidate runs, the
+    fastest is applied and cached, and the tuned operator gives the result of
+    the default one (FP32, different summation orders: 1e-5)."""
+    import torch
+    from dgfs_hip.autotune import Autotuner
+    N, L, Nrho, M, Mpre, masses = 16, 5.0, 4, 12, 24, (1.0, 2.1)
+    op, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, (0.81, 0.805, 0.805, 0.80), (1.0,) * 4, precision=32,
+                    project=True)
+    fs = _mixture_states(V.BiVelocityMesh(N, L, Nrho, M, masses), 32, seed=41)
+    Q0 = run_contig(op, fs, device)
+    ft = [torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=device) for f in fs]
+    Qt = [torch.empty_like(f) for f in ft]
+    tuner = Autotuner(cache_path=str(tmp_path / "tune.json"), repeats=1)
+    best, ms = tuner.tune(op, [f.data_ptr() for f in ft], [q.data_ptr() for q in Qt], 32)
+    say("vss tune best", best, "of", len(ms))
+    assert len(ms) == len(Autotuner.candidates(op, torch.cuda.get_device_properties(0).multi_processor_count))
+    Q1 = run_contig(op, fs, device)
+    assert max(rel(Q1[p], Q0[p]) for p in range(2)) < 1e-5
+    best2, _ = Autotuner(cache_path=str(tmp_path / "tune.json")).tune(op, [f.data_ptr() for f in ft],
+                                                                     [q.data_ptr() for q in Qt], 32)
+    assert best2 == best
