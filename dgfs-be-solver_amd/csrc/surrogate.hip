@@ -1234,6 +1234,16 @@ extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const doub
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
   if (!vhs || !params || !f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null operand");
   if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  {
+    // the exact operator runs on row buffers sized for this surrogate's grid
+    int vN = 0;
+    double vL = 0.0;
+    const int rc0 = dgfs_vhs_get_dims(vhs, &vN, nullptr, nullptr, nullptr, &vL);
+    if (rc0) return rc0;
+    if (vN != h->N || std::fabs(vL - h->L) > 1e-12 * h->L)
+      return fail(DGFS_ERR_INVALID_VALUE, "surrogate (N=%d, L=%g) and exact operator (N=%d, L=%g) grids differ",
+                  h->N, h->L, vN, vL);
+  }
   if (npts == 0) {
     if (nfallback) *nfallback = 0;
     return DGFS_OK;

@@ -1654,6 +1654,17 @@ extern "C" int dgfs_vhs_eval_contig(void* p, const double* f, double* Q, int npt
                          : dispatch<float>(h, f, Q, false, 0, 0, 0, 0, 0, npts, st);
 }
 
+extern "C" int dgfs_vhs_get_dims(void* p, int* Nv, int* Nrho, int* M, int* precision, double* L) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (Nv) *Nv = h->N;
+  if (Nrho) *Nrho = h->Nrho;
+  if (M) *M = h->M;
+  if (precision) *precision = h->prec;
+  if (L) *L = h->L;
+  return DGFS_OK;
+}
+
 extern "C" int dgfs_vhs_get_config(void* p, int* ngroups, int* nslots, int* threads) {
   VhsHandle* h = as_handle(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");

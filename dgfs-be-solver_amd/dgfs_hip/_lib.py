@@ -91,6 +91,7 @@ _SIGNATURES = {
     "dgfs_vhs_eval_pair": [_P, _P, _P, _I, _P, _P, _P, _I, _P],
     "dgfs_vhs_destroy": [_P],
     "dgfs_vhs_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
+    "dgfs_vhs_get_dims": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)],
     "dgfs_vhs_set_groups": [_P, _I],
     "dgfs_vhs_set_slots": [_P, _I],
     "dgfs_vhs_set_timing": [_P, _I],
@@ -139,6 +140,11 @@ def load():
     _apply(lib, {**_SIGNATURES, **EXTRA_SIGNATURES})
     _lib = lib
     return lib
+
+
+def version() -> str:
+    """The library's build string (dgfs_version)."""
+    return load().dgfs_version().decode(errors="replace")
 
 
 def exported_symbols():

@@ -20,12 +20,24 @@ import time
 _DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "dgfs_hip", "tune.json")
 
 
-def _key(op, npts):
+def _platform():
+    """Device name, architecture and library version: a shape tuned on another
+    GPU or by another build of the library is not reused."""
+    import torch
+    from dgfs_hip import _lib
+    props = torch.cuda.get_device_properties(torch.cuda.current_device())
+    arch = getattr(props, "gcnArchName", "")
+    return f"{props.name}|{arch}|{_lib.version()}"
+
+
+def _key(op, npts, platform=""):
     bucket = 1 << max(0, int(npts - 1).bit_length())
     if getattr(op, "kind", "vhs") == "vss":
-        return (f"vss-N{op.Nv}-p{op.precision}-K{op.K}-S{op.nspcs}-iso{int(bool(op.iso))}"
-                f"-proj{int(bool(op.project))}-pts{bucket}")
-    return f"vhs-N{op.Nv}-p{op.precision}-K{op.K}-proj{int(bool(op.project))}-pts{bucket}"
+        k = (f"vss-N{op.Nv}-p{op.precision}-K{op.K}-S{op.nspcs}-iso{int(bool(op.iso))}"
+             f"-proj{int(bool(op.project))}-pts{bucket}")
+    else:
+        k = f"vhs-N{op.Nv}-p{op.precision}-K{op.K}-proj{int(bool(op.project))}-pts{bucket}"
+    return f"{platform}|{k}" if platform else k
 
 
 class Autotuner:
@@ -104,11 +116,16 @@ class Autotuner:
         """Pick and apply the fastest launch shape for eval_contig(f, Q, npts);
         returns (config, {candidate: ms})."""
         import torch
+        from dgfs_hip import _lib
         cache = self._load()
-        key = _key(op, npts)
+        key = _key(op, npts, _platform())
         if not force and key in cache:
-            self.apply(op, cache[key]["best"])
-            return cache[key]["best"], cache[key].get("ms", {})
+            try:
+                self.apply(op, cache[key]["best"])
+                return cache[key]["best"], cache[key].get("ms", {})
+            except (_lib.DGFSError, KeyError, TypeError):
+                # a stale or damaged entry: forget it and tune again
+                del cache[key]
         cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         timings = {}

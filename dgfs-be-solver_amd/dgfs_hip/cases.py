@@ -69,6 +69,22 @@ def cavity_cfg(Nv=32, L=6.0, Nrho=16, M=12, Kn=0.05, omega=0.81, order=3, ulid=0
     return cfg
 
 
+def bkw_cfg(Nv=16, L=6.0, Nrho=8, M=6, npts=4, t0=5.5, omega=1.0, Kn=None, precision="double",
+            project=None, NvBatchSize=None):
+    """Spatially homogeneous BKW relaxation of Maxwell molecules: `npts`
+    points at the BKW state of time t0 (frfs/solvers/dgfs/initcond.py:120-140).
+    The BKW solution holds for the kernel B = 1/(4 pi) (omega = 1), which the
+    prefactor formula gives at invKn = Gamma(3/2), i.e. Kn = 2/sqrt(pi)."""
+    if Kn is None:
+        Kn = 1.0 / math.gamma(1.5)
+    cfg = base_cfg(Nv, L, Nrho, M, omega=omega, Kn=Kn, NvBatchSize=NvBatchSize, precision=precision,
+                   project=project)
+    cfg.set("soln-ics", "type", "bkw")
+    cfg.set("soln-ics", "t0", t0)
+    cfg.set("mesh", "points", npts)
+    return cfg
+
+
 def _bi_base(Nv, L, Nrho, M, Mpre, masses, omega, alpha, prefactor, dRef, Tref, order, NvBatchSize,
              precision, project):
     """Common keys of the multi-species configurations (frfs/solvers/dgfsbi/*).

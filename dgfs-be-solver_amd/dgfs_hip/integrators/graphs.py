@@ -6,12 +6,18 @@ integrators/std/steppers.py:122-225 in the reference, each launched from
 Python).  GraphStepper records one step with HIP stream capture on the
 backend's own stream and replays it with a single graph launch per step.
 
-The recording holds the device pointers and scalars of that step, so replay
-is valid because a DGFS step reads and writes the same registers every time
-(it returns the register it started from) and its kernels take no time
-argument that changes their work; both are checked.  The first step runs
-eagerly so that every workspace of the library is allocated before the
-capture (no allocation may happen while capturing).
+The recording holds the device pointers and scalars of that step.  A replay
+is the same step because
+* a DGFS step reads and writes the same registers every time: it returns the
+  register it started from, which the capture checks;
+* no kernel of a step depends on the time t: boundary values are functions
+  of position only (the time argument of collide/rhs/updatebc is unused);
+* the time step is the recorded one: a shortened last step runs eagerly.
+The first step of every GraphStepper runs eagerly, whatever the stepper's
+step count (a restarted stepper included), so that every workspace of the
+library is allocated before the capture: no allocation may happen while
+capturing.  advance_to runs the stepper's plugins after every accepted step
+and counts the right-hand-side evaluations of the stepper class.
 """
 import ctypes as C
 
@@ -36,6 +42,8 @@ class GraphStepper:
         self.stepper = stepper
         self.backend = be
         self._exec = None
+        self._warm = False
+        self._dt = None
         self.nodes = 0
 
     def _capture(self, t, dt):
@@ -70,26 +78,31 @@ class GraphStepper:
         """One time step: eager the first time, captured the second time (and
         then launched), replayed afterwards.  Returns the solution register."""
         s = self.stepper
+        if self._exec is not None and dt != self._dt:
+            raise ValueError(f"the recorded step has dt = {self._dt}; step({dt}) must run eagerly")
         if self._exec is None:
-            if s.nacptsteps == 0 and not getattr(self, "_warm", False):
+            if not self._warm:
                 self._warm = True
                 return s.step(t, dt)
             self._capture(t, dt)
+            self._dt = dt
         _lib.call("dgfs_graph_launch", self._exec, C.c_void_p(self.backend.stream_ptr))
-        s.nfevals += 2 if s.stepper_name == "dgfs-tvd-rk2" else 1
+        s.nfevals += s._stepper_nfevals
         return s._idxcurr
 
     def advance_to(self, tend):
         s = self.stepper
         while s.tcurr < tend - 1e-12 * max(1.0, abs(tend)):
             dt = min(s.dt, tend - s.tcurr)
-            if self._exec is not None and abs(dt - s.dt) > 1e-15 * s.dt:
+            if self._exec is not None and dt != self._dt:
                 # a shortened last step differs from the recording: run it eagerly
                 s._idxcurr = s.step(s.tcurr, dt)
             else:
                 s._idxcurr = self.step(s.tcurr, dt)
             s.tcurr += dt
             s.nacptsteps += 1
+            for pl in s.plugins:
+                pl(s)
 
     def close(self):
         if self._exec is not None and self._exec.value:

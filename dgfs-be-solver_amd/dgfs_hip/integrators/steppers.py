@@ -75,6 +75,7 @@ class StdEulerDGFSStepper(BaseDGFSStepper):
     stepper_name = "dgfs-euler"
     _stepper_nregs = 2
     _stepper_order = 1
+    _stepper_nfevals = 1     # right-hand-side evaluations per step
 
     def step(self, t, dt):
         add, rhs = self._add_dgfs, self.system.rhs
@@ -98,7 +99,7 @@ class StdEulerDGFSStepper(BaseDGFSStepper):
             copyto(r1, r1, sidx, eidx - 1, 0, NvBS - 1)
             add(1.0, r0, dt, r1)
             copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
-        self.nfevals += 1
+        self.nfevals += self._stepper_nfevals
         return r0
 
 
@@ -106,6 +107,7 @@ class StdTVDRK2DGFSStepper(BaseDGFSStepper):
     stepper_name = "dgfs-tvd-rk2"
     _stepper_nregs = 3
     _stepper_order = 2
+    _stepper_nfevals = 2     # right-hand-side evaluations per step
 
     def step(self, t, dt):
         add, rhs = self._add_dgfs, self.system.rhs
@@ -145,7 +147,7 @@ class StdTVDRK2DGFSStepper(BaseDGFSStepper):
             rhs(t + dt, r1, r2, batch, sidx)
             add(1.0, r0, 0.5, r1, 0.5 * dt, r2)
             copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
-        self.nfevals += 2
+        self.nfevals += self._stepper_nfevals
         return r0
 
 
@@ -172,6 +174,7 @@ class StdEulerDGFSBiStepper(BaseDGFSBiStepper):
     stepper_name = "dgfsbi-euler"
     _stepper_nregs_orig = 2
     _stepper_order = 1
+    _stepper_nfevals = 1     # right-hand-side evaluations per step
 
     def step(self, t, dt):
         add, rhs = self._add_dgfs, self.system.rhs
@@ -197,7 +200,7 @@ class StdEulerDGFSBiStepper(BaseDGFSBiStepper):
                 copyto(r0 + 1, r0 + 1, sidx, eidx - 1, 0, NvBS - 1)
                 add(1.0, r0, dt, r0 + 1)
                 copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
-        self.nfevals += 1
+        self.nfevals += self._stepper_nfevals
         return 0
 
 
@@ -205,6 +208,7 @@ class StdTVDRK2DGFSBiStepper(BaseDGFSBiStepper):
     stepper_name = "dgfsbi-tvd-rk2"
     _stepper_nregs_orig = 3
     _stepper_order = 2
+    _stepper_nfevals = 2     # right-hand-side evaluations per step
 
     def step(self, t, dt):
         add, rhs = self._add_dgfs, self.system.rhs
@@ -250,7 +254,7 @@ class StdTVDRK2DGFSBiStepper(BaseDGFSBiStepper):
                 rhs(t + dt, str(p), r0 + 1, r0 + 2, batch, sidx)
                 add(1.0, r0, 0.5, r0 + 1, 0.5 * dt, r0 + 2)
                 copyfrm(r0, r0, 0, NvBS - 1, sidx, eidx - 1)
-        self.nfevals += 2
+        self.nfevals += self._stepper_nfevals
         return 0
 
 

@@ -187,7 +187,7 @@ def soln_for_system(initsoln, cfg_now, ele, fields, etype):
     same = solnb.nupts == ele.nupts and np.array_equal(solnb.upts, ele.basis.upts)
     if same:
         return soln
-    interp = solnb.ubasis.nodal_basis_at(ele.basis.upts)       # (nupts_now, nupts_soln)
+    interp = solnb.ubasis.at(ele.basis.upts)       # (nupts_now, nupts_soln)
     out = interp @ soln.reshape(solnb.nupts, -1)
     return out.reshape(ele.nupts, soln.shape[1], soln.shape[2])
 

@@ -1,144 +1,130 @@
-"""Orthonormal polynomial bases on the reference line, quadrilateral and
-triangle (mirrors frfs/polys.py:12-208: Jacobi recurrences, tensor Legendre
-on quads, Dubiner/Koornwinder on triangles; nodal bases by solving with the
-Vandermonde matrix)."""
-import math
+"""Polynomial spaces and nodal bases on the reference line, quadrilateral and
+triangle.
 
+The flux-reconstruction operators need two things of a solution space
+(the role of frfs/polys.py in the reference):
+* a nodal basis on the solution points (values and gradients anywhere), for
+  the interpolation and differentiation operators;
+* an L2-orthonormal basis of the same space on the reference element, for
+  the DG lift of the face corrections (a projection, so any orthonormal basis
+  gives the same operator).
+
+Construction used here: the space is spanned by products of Legendre
+polynomials P_i(x) P_j(y) (i, j <= p on the square, i + j <= p on the
+triangle, i <= p on the line).  Their Gram matrix under an exact quadrature
+rule of the reference element is factorised G = R^T R (Cholesky), and
+phi = P R^-1 is orthonormal.  The nodal basis of a point set is the inverse
+of the (square) matrix of any basis at the points.  Legendre values and
+derivatives come from numpy.polynomial.legendre.
+
+Reference elements: line [-1, 1]; quadrilateral [-1, 1]^2; triangle with
+vertices (-1, -1), (1, -1), (-1, 1) (area 2).
+"""
 import numpy as np
+from numpy.polynomial import legendre as _leg
+
+_NDIMS = {"line": 1, "quad": 2, "tri": 2}
 
 
-def jacobi(n, a, b, z):
-    """[P_0 .. P_n] of the Jacobi polynomials P^(a,b) at z (polys.py:12-31)."""
-    z = np.asarray(z, dtype=float)
-    j = [np.ones_like(z)]
-    if n >= 1:
-        j.append(((a + b + 2) * z + a - b) / 2)
-    if n >= 2:
-        apb, bbmaa = a + b, b * b - a * a
-        for q in range(2, n + 1):
-            qapbpq, apbp2q = q * (apb + q), apb + 2 * q
-            apbp2qm1, apbp2qm2 = apbp2q - 1, apbp2q - 2
-            aq = apbp2q * apbp2qm1 / (2 * qapbpq)
-            bq = apbp2qm1 * bbmaa / (2 * qapbpq * apbp2qm2)
-            cq = apbp2q * (a + q - 1) * (b + q - 1) / (qapbpq * apbp2qm2)
-            j.append((aq * z - bq) * j[-1] - cq * j[-2])
-    return j
+def _legendre(x, deg):
+    """Values and first derivatives of P_0..P_deg at x: two (npts, deg+1)."""
+    x = np.asarray(x, dtype=float).reshape(-1)
+    val = _leg.legvander(x, deg)
+    der = np.zeros_like(val)
+    for i in range(1, deg + 1):
+        c = np.zeros(i + 1)
+        c[i] = 1.0
+        der[:, i] = _leg.legval(x, _leg.legder(c))
+    return val, der
 
 
-def jacobi_diff(n, a, b, z):
-    z = np.asarray(z, dtype=float)
-    dj = [np.zeros_like(z)]
-    if n >= 1:
-        dj.extend(jp * (i + a + b + 2) / 2 for i, jp in enumerate(jacobi(n - 1, a + 1, b + 1, z)))
-    return dj
+def element_rule(shape, n):
+    """Gauss rule with n points per direction on a reference element: exact
+    for degree 2n - 1 per direction on the line and square, and for total
+    degree 2n - 2 on the triangle (collapsed coordinates)."""
+    z, w = _leg.leggauss(n)
+    if shape == "line":
+        return z[:, None], w
+    A, B = np.meshgrid(z, z, indexing="ij")
+    W = np.outer(w, w)
+    if shape == "quad":
+        return np.stack([A.ravel(), B.ravel()], axis=1), W.ravel()
+    if shape == "tri":
+        xi = (1.0 + A) * (1.0 - B) / 2.0 - 1.0
+        return np.stack([xi.ravel(), B.ravel()], axis=1), (W * (1.0 - B) / 2.0).ravel()
+    raise ValueError(f"no reference element {shape!r}")
 
 
-class BasePolyBasis:
-    name = None
+class PolySpace:
+    """P_p on the line or triangle, Q_p on the square, with an orthonormal
+    basis phi_b (b = 0 .. size-1)."""
 
-    def __init__(self, order, pts):
-        self.order = order
-        self.pts = np.asarray(pts, dtype=float)
+    def __init__(self, shape, degree):
+        if shape not in _NDIMS:
+            raise ValueError(f"no polynomial space on {shape!r}")
+        self.shape, self.degree, self.ndims = shape, int(degree), _NDIMS[shape]
+        p = self.degree
+        if shape == "line":
+            self.exps = [(i,) for i in range(p + 1)]
+        elif shape == "quad":
+            self.exps = [(i, j) for i in range(p + 1) for j in range(p + 1)]
+        else:
+            self.exps = [(i, j) for i in range(p + 1) for j in range(p + 1 - i)]
+        self.size = len(self.exps)
+        qp, qw = element_rule(shape, p + 2)
+        raw = self._raw(qp)[0]
+        gram = (raw * qw[:, None]).T @ raw
+        R = np.linalg.cholesky(gram).T
+        self._coef = np.linalg.inv(R)             # phi = raw @ coef
 
-    @property
-    def vdm(self):
-        return self.ortho_basis_at(self.pts)
+    def _raw(self, pts):
+        """Legendre products and their gradients: (npts, size) and
+        (ndims, npts, size)."""
+        pts = np.asarray(pts, dtype=float).reshape(-1, self.ndims)
+        axes = [_legendre(pts[:, d], self.degree) for d in range(self.ndims)]
+        val = np.ones((len(pts), self.size))
+        grad = np.ones((self.ndims, len(pts), self.size))
+        for b, e in enumerate(self.exps):
+            for d, i in enumerate(e):
+                v, dv = axes[d][0][:, i], axes[d][1][:, i]
+                val[:, b] *= v
+                for g in range(self.ndims):
+                    grad[g, :, b] *= dv if g == d else v
+        return val, grad
 
-    def nodal_basis_at(self, epts):
-        """L[p][j]: the nodal basis of self.pts evaluated at epts."""
-        return np.linalg.solve(self.vdm, self.ortho_basis_at(epts)).T
+    def ortho_at(self, pts):
+        """(size, npts): phi_b at the points."""
+        return (self._raw(pts)[0] @ self._coef).T
 
-    def jac_nodal_basis_at(self, epts):
-        """dL[d][j][p]: derivative along reference axis d of nodal basis j at p."""
-        J = self.jac_ortho_basis_at(epts)  # (ndims, nbasis, npts)
-        return np.stack([np.linalg.solve(self.vdm, J[d]) for d in range(J.shape[0])])
-
-
-class LinePolyBasis(BasePolyBasis):
-    name = "line"
-
-    def ortho_basis_at(self, pts):
-        p = np.asarray(pts, dtype=float).reshape(-1)
-        jp = jacobi(self.order - 1, 0, 0, p)
-        return np.array([math.sqrt(i + 0.5) * v for i, v in enumerate(jp)])
-
-    def jac_ortho_basis_at(self, pts):
-        p = np.asarray(pts, dtype=float).reshape(-1)
-        djp = jacobi_diff(self.order - 1, 0, 0, p)
-        return np.array([[math.sqrt(i + 0.5) * v for i, v in enumerate(djp)]])
-
-    @property
-    def degrees(self):
-        return [(i,) for i in range(self.order)]
-
-
-class QuadPolyBasis(BasePolyBasis):
-    name = "quad"
-
-    def _pab(self, pts):
-        pts = np.atleast_2d(np.asarray(pts, dtype=float))
-        p, q = pts[:, 0], pts[:, 1]
-        sk = [math.sqrt(k + 0.5) for k in range(self.order)]
-        pa = [c * jp for c, jp in zip(sk, jacobi(self.order - 1, 0, 0, p))]
-        pb = [c * jp for c, jp in zip(sk, jacobi(self.order - 1, 0, 0, q))]
-        dpa = [c * jp for c, jp in zip(sk, jacobi_diff(self.order - 1, 0, 0, p))]
-        dpb = [c * jp for c, jp in zip(sk, jacobi_diff(self.order - 1, 0, 0, q))]
-        return pa, pb, dpa, dpb
-
-    def ortho_basis_at(self, pts):
-        pa, pb, _, _ = self._pab(pts)
-        return np.array([pi * pj for pi in pa for pj in pb])
-
-    def jac_ortho_basis_at(self, pts):
-        pa, pb, dpa, dpb = self._pab(pts)
-        d0 = [dpi * pj for pi, dpi in zip(pa, dpa) for pj, dpj in zip(pb, dpb)]
-        d1 = [pi * dpj for pi, dpi in zip(pa, dpa) for pj, dpj in zip(pb, dpb)]
-        return np.array([d0, d1])
-
-    @property
-    def degrees(self):
-        return [(i, j) for i in range(self.order) for j in range(self.order)]
+    def grad_ortho_at(self, pts):
+        """(ndims, size, npts): d phi_b / d x_d at the points."""
+        g = self._raw(pts)[1]
+        return np.stack([(g[d] @ self._coef).T for d in range(self.ndims)])
 
 
-class TriPolyBasis(BasePolyBasis):
-    name = "tri"
+class NodalBasis:
+    """Lagrange basis of a PolySpace on a unisolvent point set: l_j(x_i) = delta_ij."""
 
-    def _ab(self, pts):
-        pts = np.atleast_2d(np.asarray(pts, dtype=float))
-        p, q = pts[:, 0], pts[:, 1]
-        with np.errstate(divide="ignore", invalid="ignore"):
-            a = np.where(q != 1, 2 * (1 + p) / (1 - q) - 1, -1.0)
-        return a, q
+    def __init__(self, space, nodes):
+        self.space = space
+        self.nodes = np.asarray(nodes, dtype=float).reshape(-1, space.ndims)
+        if len(self.nodes) != space.size:
+            raise ValueError(f"{len(self.nodes)} nodes for a space of dimension {space.size}")
+        V = space.ortho_at(self.nodes)            # V[b, i] = phi_b(x_i)
+        self._C = np.linalg.inv(V)                # l_j = sum_b C[j, b] phi_b, as C V = I
 
-    def ortho_basis_at(self, pts):
-        a, b = self._ab(pts)
-        ob = []
-        for i, pi in enumerate(jacobi(self.order - 1, 0, 0, a)):
-            pa = pi * (1 - b) ** i
-            for j, pj in enumerate(jacobi(self.order - i - 1, 2 * i + 1, 0, b)):
-                cij = math.sqrt((2 * i + 1) * (2 * i + 2 * j + 2)) / 2 ** (i + 1)
-                ob.append(cij * pa * pj)
-        return np.array(ob)
+    def at(self, pts):
+        """(npts, nnodes): l_j at every point."""
+        return (self._C @ self.space.ortho_at(pts)).T
 
-    def jac_ortho_basis_at(self, pts):
-        a, b = self._ab(pts)
-        f = jacobi(self.order - 1, 0, 0, a)
-        df = jacobi_diff(self.order - 1, 0, 0, a)
-        d0, d1 = [], []
-        for i, (fi, dfi) in enumerate(zip(f, df)):
-            g = jacobi(self.order - i - 1, 2 * i + 1, 0, b)
-            dg = jacobi_diff(self.order - i - 1, 2 * i + 1, 0, b)
-            for j, (gj, dgj) in enumerate(zip(g, dg)):
-                cij = math.sqrt((2 * i + 1) * (2 * i + 2 * j + 2)) / 2 ** (i + 1)
-                tmp = (1 - b) ** (i - 1) if i > 0 else np.ones_like(b)
-                d0.append(cij * 2 * tmp * dfi * gj)
-                d1.append(cij * (tmp * (-i * fi + (1 + a) * dfi) * gj + (1 - b) ** i * fi * dgj))
-        return np.array([d0, d1])
+    def grad_at(self, pts):
+        """(ndims, nnodes, npts): gradient of l_j along reference axis d."""
+        G = self.space.grad_ortho_at(pts)
+        return np.stack([self._C @ G[d] for d in range(self.space.ndims)])
 
-    @property
-    def degrees(self):
-        return [(i, j) for i in range(self.order) for j in range(self.order - i)]
+    def ortho_at(self, pts):
+        return self.space.ortho_at(pts)
 
 
-def get_polybasis(name, order, pts=()):
-    return {"line": LinePolyBasis, "quad": QuadPolyBasis, "tri": TriPolyBasis}[name](order, pts)
+def nodal_basis(shape, degree, nodes):
+    return NodalBasis(PolySpace(shape, degree), nodes)

@@ -1,34 +1,45 @@
-"""Reference elements and FR operator matrices (mirrors frfs/shapes.py:25-425).
+"""Reference elements and the FR operator matrices of the DGFS advection.
 
-Operators, as opmat() returns them (shapes.py:83-118):
-  M0 = nodal basis of the solution points evaluated at the flux points
-  M1 = derivatives of that basis at the solution points, [upt][d*nupts + j]
-  M2 = reference face normal times M0, [fpt][d*nupts + j]
-  M3 = correction lift: sum_j phi_j(upt) * S[fpt][j], S the face integral of
-       face-nodal basis times orthonormal basis on a (p+1)-point
-       Gauss-Lobatto-Legendre face rule (gbasis_coeffs, shapes.py:243-285)
-  M12 = vdm^T vdm (used by invmass)
-Point sets are generated here: tensor Gauss(-Lobatto)-Legendre points on the
-quadrilateral in x-fastest order (the order of the reference's tabulated
-quad rules), Gauss-Legendre flux points on each face in ascending face
-coordinate, and the Williams-Shunn-Jameson rules (J. Comput. Appl. Math. 266,
-2014) on the triangle.
+The operators and their layouts are the reference's (frfs/shapes.py:93-118):
+  M0  (nfpts, nupts)          solution -> flux points: l_j(x_f)
+  M1  (nupts, ndims*nupts)    reference gradient at the solution points,
+                              column d*nupts + j = d l_j / d xi_d (x_u)
+  M2  (nfpts, ndims*nupts)    reference face normal (unnormalised) times M0
+  M3  (nupts, nfpts)          correction lift of the face fluxes
+and opmat() evaluates expressions of them such as 'M1 - M3*M2'.
+
+The lift M3 is the DG lift of shapes.py:243-289: column (face f, point k)
+is the L2 projection onto the solution space of the face functional
+q -> int_f l_k q ds, with the face integral taken by the (p+1)-point
+Gauss-Lobatto rule.  With an orthonormal basis phi_b of the space,
+M3[u, (f, k)] = sum_b phi_b(x_u) sum_q w_q l_k(s_q) phi_b(x_f(s_q)).
+
+Point sets (conventions the meshes and the reference's files rely on):
+* quadrilateral: tensor Gauss-Legendre (or -Lobatto) points, x fastest;
+* triangle: the Williams-Shunn-Jameson points (J. Comput. Appl. Math. 266,
+  2014), the tables of frfs/quadrules/tri/williams-shunn-n{3,6,10}-*.txt;
+* faces: Gauss-Legendre points in ascending face coordinate, faces in the
+  reference's order (quad: bottom, right, top, left; triangle: (s, -1),
+  (-s, s), (-1, s)), with the reference's unnormalised normals.
 """
-import itertools as it
 import re
 
 import numpy as np
 
-from dgfs_hip.polys import get_polybasis
+from dgfs_hip.polys import PolySpace, NodalBasis, nodal_basis
 from dgfs_hip.quadratures import zwgj, zwglj
 
-# Williams-Shunn-Jameson symmetric triangle rules (points on [-1,1]^2 reference triangle)
 _WS_TRI = {
     3: [(-0.666666666666667, 0.333333333333333), (0.333333333333333, -0.666666666666667),
         (-0.666666666666667, -0.666666666666667)],
     6: [(-0.81684757298044, 0.63369514596088), (0.63369514596088, -0.81684757298044),
         (-0.81684757298044, -0.81684757298044), (-0.108103018168072, -0.783793963663856),
         (-0.783793963663856, -0.108103018168072), (-0.108103018168072, -0.108103018168072)],
+    10: [(-0.333333333333333, -0.333333333333333), (-0.888871894660414, 0.777743789320828),
+         (0.777743789320828, -0.888871894660414), (-0.888871894660414, -0.888871894660414),
+         (0.268421495491446, -0.408932576528214), (-0.859488918963232, -0.408932576528214),
+         (-0.408932576528214, -0.859488918963232), (0.268421495491446, -0.859488918963232),
+         (-0.859488918963232, 0.268421495491446), (-0.408932576528214, 0.268421495491446)],
 }
 
 
@@ -40,14 +51,78 @@ def line_rule(rule, npts):
     raise ValueError(f"unknown line rule {rule!r}")
 
 
-def _proj_pts(proj, pts):
-    pts = np.atleast_2d(np.asarray(pts, dtype=float).T)
-    return np.vstack(np.broadcast_arrays(*proj(*pts))).T
+def _on_face(proj, s):
+    """Reference-element points of the face parametrisation at s."""
+    s = np.asarray(s, dtype=float)
+    return np.stack([np.broadcast_to(c, s.shape).astype(float) for c in proj(s)], axis=1)
+
+
+# -- operator expressions ----------------------------------------------------
+_TOKEN = re.compile(r"\s*(M\d+|[-+*()])")
+
+
+def eval_opmat(expr, lookup):
+    """Value of an operator expression over matrices M<n>: '+', '-' (binary
+    and unary), '*' (matrix product) and parentheses.  lookup(name) returns
+    the matrix of a name."""
+    if not re.fullmatch(r"[M0-9\-+*() ]+", expr):
+        raise ValueError("Invalid operator matrix expression")
+    toks, pos = [], 0
+    while pos < len(expr):
+        m = _TOKEN.match(expr, pos)
+        if not m:
+            if expr[pos:].strip():
+                raise ValueError(f"Invalid operator matrix expression at {expr[pos:]!r}")
+            break
+        toks.append(m.group(1))
+        pos = m.end()
+    toks.append(None)
+    at = [0]
+
+    def peek():
+        return toks[at[0]]
+
+    def take():
+        t = toks[at[0]]
+        at[0] = min(at[0] + 1, len(toks) - 1)
+        return t
+
+    def factor():
+        t = take()
+        if t == "-":
+            return -factor()
+        if t == "(":
+            v = sum_()
+            if take() != ")":
+                raise ValueError("unbalanced parentheses in operator expression")
+            return v
+        if t is not None and t.startswith("M"):
+            return np.asarray(lookup(t), dtype=float)
+        raise ValueError(f"unexpected {t!r} in operator expression")
+
+    def product():
+        v = factor()
+        while peek() == "*":
+            take()
+            v = v @ factor()
+        return v
+
+    def sum_():
+        v = product()
+        while peek() in ("+", "-"):
+            v = v + product() if take() == "+" else v - product()
+        return v
+
+    out = sum_()
+    if peek() is not None:
+        raise ValueError(f"trailing {peek()!r} in operator expression")
+    return out
 
 
 class BaseShape:
     name = None
     ndims = -1
+    faces = ()
 
     def __init__(self, order, upts_rule="gauss-legendre", fpts_rule="gauss-legendre"):
         self.order = order
@@ -55,99 +130,96 @@ class BaseShape:
         self.fpts_rule = fpts_rule
         self.upts = self._upts()
         self.nupts = len(self.upts)
-        self.ubasis = get_polybasis(self.name, order + 1, self.upts)
-        # faces
-        ppts, pnorms, nfacefpts = [], [], []
-        z, _ = line_rule(fpts_rule, order + 1)
-        self.face_pts1d = z
-        for kind, proj, norm in self.faces:
-            fp = _proj_pts(proj, z)
-            ppts.append(fp)
-            pnorms.append(np.tile(norm, (len(fp), 1)))
-            nfacefpts.append(len(fp))
-        self.fpts = np.vstack(ppts)
-        self.norm_fpts = np.vstack(pnorms).astype(float)
-        self.nfacefpts = nfacefpts
+        self.space = PolySpace(self.name, order)
+        self.ubasis = NodalBasis(self.space, self.upts)
+        self.face_pts1d, _ = line_rule(fpts_rule, order + 1)
+        pts, nrm, counts = [], [], []
+        for _, proj, norm in self.faces:
+            fp = _on_face(proj, self.face_pts1d)
+            pts.append(fp)
+            nrm.append(np.tile(np.asarray(norm, dtype=float), (len(fp), 1)))
+            counts.append(len(fp))
+        self.fpts = np.vstack(pts)
+        self.norm_fpts = np.vstack(nrm)
+        self.nfacefpts = counts
         self.nfpts = len(self.fpts)
-        nf = np.cumsum([0] + nfacefpts)
-        self.facefpts = [list(range(nf[i], nf[i + 1])) for i in range(len(nfacefpts))]
+        edges = np.cumsum([0] + counts)
+        self.facefpts = [list(range(edges[i], edges[i + 1])) for i in range(len(counts))]
+        self._ops = {}
 
     # operator matrices ----------------------------------------------------
+    def _op(self, name, build):
+        if name not in self._ops:
+            self._ops[name] = build()
+        return self._ops[name]
+
     @property
     def m0(self):
-        return self.ubasis.nodal_basis_at(self.fpts)
+        return self._op("m0", lambda: self.ubasis.at(self.fpts))
 
     @property
     def m1(self):
-        m = np.rollaxis(self.ubasis.jac_nodal_basis_at(self.upts), 2)
-        return m.reshape(self.nupts, -1)
+        def build():
+            g = self.ubasis.grad_at(self.upts)            # (ndims, nupts_j, nupts_u)
+            return np.concatenate([g[d].T for d in range(self.ndims)], axis=1)
+        return self._op("m1", build)
 
     @property
     def m2(self):
-        m = self.norm_fpts[..., None] * self.m0[:, None, :]
-        return m.reshape(self.nfpts, -1)
-
-    @property
-    def gbasis_coeffs(self):
-        coeffs = []
-        zq, wq = line_rule("gauss-legendre-lobatto", self.order + 1)
-        fb = get_polybasis("line", self.order + 1, self.face_pts1d)
-        for kind, proj, norm in self.faces:
-            L = fb.nodal_basis_at(zq)                       # [q][k]
-            M = self.ubasis.ortho_basis_at(_proj_pts(proj, zq))  # [j][q]
-            coeffs.append(np.einsum("i,ik,ji->kj", wq, L, M))
-        return np.vstack(coeffs)
+        def build():
+            return np.concatenate([self.norm_fpts[:, d:d + 1] * self.m0 for d in range(self.ndims)], axis=1)
+        return self._op("m2", build)
 
     @property
     def m3(self):
-        return (self.gbasis_coeffs @ self.ubasis.ortho_basis_at(self.upts)).T
-
-    @property
-    def m12(self):
-        v = self.ubasis.vdm
-        return (v.T @ v).reshape(self.nupts, -1)
+        def build():
+            zq, wq = line_rule("gauss-legendre-lobatto", self.order + 1)
+            lk = nodal_basis("line", self.order, self.face_pts1d).at(zq)   # (nq, nk)
+            phi_u = self.space.ortho_at(self.upts)                          # (nb, nupts)
+            blocks = []
+            for _, proj, _ in self.faces:
+                phi_q = self.space.ortho_at(_on_face(proj, zq))             # (nb, nq)
+                blocks.append(phi_u.T @ (phi_q @ (wq[:, None] * lk)))       # (nupts, nk)
+            return np.hstack(blocks)
+        return self._op("m3", build)
 
     def opmat(self, expr):
-        """Evaluate an operator expression such as 'M1 - M3*M2' (shapes.py:83-91)."""
-        if not re.match(r"[M0-9\-+*() ]+$", expr):
-            raise ValueError("Invalid operator matrix expression")
-        mats = {m: np.asarray(getattr(self, m.lower())) for m in re.findall(r"M\d+", expr)}
-        py = re.sub(r"(M\d+)\s*\*\s*(M\d+)", r"(\1 @ \2)", expr)
-        return np.asarray(eval(py, {"__builtins__": None}, mats))
+        """Operator expression such as 'M1 - M3*M2' (the syntax of shapes.py:83-91)."""
+        return eval_opmat(expr, lambda m: getattr(self, m.lower()))
 
-    # geometry: linear shape functions ------------------------------------
+    # geometry: the linear shape functions --------------------------------
     def shape_basis(self):
-        return get_polybasis(self.name, 2, self.std_ele_lin())
+        return nodal_basis(self.name, 1, self.std_ele_lin())
 
 
 class QuadShape(BaseShape):
     name = "quad"
     ndims = 2
-    faces = [
-        ("line", lambda s: (s, -1), (0, -1)),
-        ("line", lambda s: (1, s), (1, 0)),
-        ("line", lambda s: (s, 1), (0, 1)),
-        ("line", lambda s: (-1, s), (-1, 0)),
-    ]
+    faces = (
+        ("line", lambda s: (s, -1.0), (0, -1)),
+        ("line", lambda s: (1.0, s), (1, 0)),
+        ("line", lambda s: (s, 1.0), (0, 1)),
+        ("line", lambda s: (-1.0, s), (-1, 0)),
+    )
 
     def _upts(self):
         z, _ = line_rule(self.upts_rule, self.order + 1)
-        return np.array([(z[i], z[j]) for j in range(len(z)) for i in range(len(z))])
+        Y, X = np.meshgrid(z, z, indexing="ij")
+        return np.stack([X.ravel(), Y.ravel()], axis=1)
 
     @staticmethod
     def std_ele_lin():
-        pts1d = np.linspace(-1, 1, 2)
-        return np.array([p[::-1] for p in it.product(pts1d, repeat=2)])
+        return np.array([(-1.0, -1.0), (1.0, -1.0), (-1.0, 1.0), (1.0, 1.0)])
 
 
 class TriShape(BaseShape):
     name = "tri"
     ndims = 2
-    faces = [
-        ("line", lambda s: (s, -1), (0, -1)),
+    faces = (
+        ("line", lambda s: (s, -1.0), (0, -1)),
         ("line", lambda s: (-s, s), (1, 1)),
-        ("line", lambda s: (-1, s), (-1, 0)),
-    ]
+        ("line", lambda s: (-1.0, s), (-1, 0)),
+    )
 
     def _upts(self):
         n = (self.order + 1) * (self.order + 2) // 2

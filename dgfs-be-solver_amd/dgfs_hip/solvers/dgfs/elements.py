@@ -9,8 +9,6 @@ points.
 """
 import numpy as np
 
-from dgfs_hip.polys import get_polybasis
-
 
 class DGFSElements:
     def __init__(self, backend, shape, nodes, vm, nvars, nregs):
@@ -21,7 +19,7 @@ class DGFSElements:
         self.nspts, self.neles, self.ndims = self.nodes.shape
         self.nupts, self.nfpts = shape.nupts, shape.nfpts
         self.nvars = nvars
-        self._sbasis = get_polybasis(shape.name, 2, shape.std_ele_lin())
+        self._sbasis = shape.shape_basis()
         self._geometry()
 
         be = backend
@@ -41,8 +39,8 @@ class DGFSElements:
 
     # geometry ------------------------------------------------------------
     def _map(self, pts):
-        L = self._sbasis.nodal_basis_at(pts)                    # (npts, nspts)
-        dL = self._sbasis.jac_nodal_basis_at(pts)               # (ndims, nspts, npts)
+        L = self._sbasis.at(pts)                                # (npts, nspts)
+        dL = self._sbasis.grad_at(pts)                          # (ndims, nspts, npts)
         ploc = np.einsum("pa,aed->ped", L, self.nodes)          # (npts, neles, ndims)
         jac = np.einsum("sap,aer->pesr", dL, self.nodes)        # d x_r / d xi_s
         return ploc, jac

@@ -91,6 +91,13 @@ class VHSOperator:
         """Cap the multi-pass persistent grid (0: occupancy limit)."""
         _lib.call("dgfs_vhs_set_slots", self._h, int(nslots))
 
+    def get_dims(self):
+        """(Nv, Nrho, M, precision, L) the handle was built for."""
+        v = [C.c_int() for _ in range(4)]
+        L = C.c_double()
+        _lib.call("dgfs_vhs_get_dims", self._h, *[C.byref(x) for x in v], C.byref(L))
+        return tuple(x.value for x in v) + (L.value,)
+
     def get_config(self):
         g, s, t = C.c_int(), C.c_int(), C.c_int()
         _lib.call("dgfs_vhs_get_config", self._h, C.byref(g), C.byref(s), C.byref(t))

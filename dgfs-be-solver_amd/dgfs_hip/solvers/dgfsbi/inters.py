@@ -110,6 +110,16 @@ class DGFSBiInletBCInters(DGFSBiBCInters):
         self._species_maxwellian(b, p, wall=False)
 
 
+class DGFSBiInletNormalShockBCInters(DGFSBiBCInters):
+    """inters.py:498-560, bcs/dgfsbi-inlet-normalshock.mako: each species'
+    section Maxwellian (nden<p>) on the velocities with c.n < 0."""
+    type = "dgfsbi-inlet-normalshock"
+    single = I1.DGFSInletNormalShockBCInters
+
+    def _setup_species(self, b, p):
+        self._species_maxwellian(b, p, wall=False)
+
+
 class DGFSBiOutletBCInters(DGFSBiBCInters):
     """inters.py:563-617."""
     type = "dgfsbi-outlet"

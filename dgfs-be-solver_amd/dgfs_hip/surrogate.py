@@ -229,6 +229,10 @@ class SpectralSurrogate:
         <= threshold (and the high-mode indicator <= threshold_hi when that is
         > 0), the exact operator `vhs_op` (a VHSOperator of the same grid)
         elsewhere.  Returns the number of fallback points."""
+        Nv, _, _, _, L = vhs_op.get_dims()
+        if Nv != self.Nv or abs(L - self.L) > 1e-12 * self.L:
+            raise ValueError(f"surrogate grid (N={self.Nv}, L={self.L}) differs from the exact operator's "
+                             f"(N={Nv}, L={L})")
         nfb = C.c_int()
         _lib.call("dgfs_sur_eval", self._h, vhs_op._h, _ptr(self.params_bf16), _ptr(f), _ptr(Q), int(npts),
                   float(threshold), float(threshold_hi), int(bool(bf16)), C.byref(nfb),

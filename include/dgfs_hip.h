@@ -92,6 +92,11 @@ int dgfs_vhs_destroy(void* handle);
 /* Work split chosen for the last eval: points per launch, radial groups per
  * point, persistent workgroups.  Used by the autotuner and the benchmark. */
 int dgfs_vhs_get_config(void* handle, int* ngroups, int* nslots, int* threads);
+/* The grid a handle was built for: Nv (points per axis), Nrho, M,
+ * precision (64 or 32) and the half-width L of the velocity box.  Lets a
+ * caller that shares a handle (the surrogate's fallback) check shapes before
+ * any launch; no reference counterpart (the reference holds one mesh). */
+int dgfs_vhs_get_dims(void* handle, int* Nv, int* Nrho, int* M, int* precision, double* L);
 /* Override the radial-group split (0 = automatic). */
 int dgfs_vhs_set_groups(void* handle, int ngroups);
 /* Cap the persistent grid of the multi-pass kernels at nslots workgroups

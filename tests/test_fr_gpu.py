@@ -155,13 +155,22 @@ def test_specular_and_outlet_walls(device):
 
 
 def test_moments_kernel(device):
+    """All 15 outputs of the HIP moments kernel against direct numpy sums of
+    the reference's definitions (plugins/dgfsmomwriterstd.py:140-214, before
+    its dimensional scaling): rho, U, T, q_i = sum f c'_i |c'|^2 cw,
+    P_ij = 2 sum f c'_i c'_j cw and p = rho T.  The states are sheared,
+    anisotropic and skewed so that every heat-flux and stress component is
+    non-zero."""
     be, sysm, _ = build_system()
     vm = sysm.vm
     rng = np.random.default_rng(8)
-    f = OF.maxwellian(vm.cv(), 1.0, [0.2, -0.1, 0.05], 0.9)[None, :, None] * (1 + 0.1 * rng.random((9, 1, 6)))
+    cv, cw = vm.cv(), vm.cw()
+    base = OF.maxwellian(cv, 1.0, [0.2, -0.1, 0.05], 0.9)
+    skew = (1 + 0.3 * np.tanh(cv[0] * cv[1]) + 0.2 * np.tanh(cv[1] * cv[2]) + 0.25 * np.tanh(cv[0] * cv[2])
+            + 0.1 * np.tanh(cv[0] ** 3) + 0.1 * np.tanh(cv[1] ** 3) + 0.1 * np.tanh(cv[2] ** 3))
+    f = (base * skew)[None, :, None] * (1 + 0.1 * rng.random((9, 1, 6)))
     sysm.eles_scal_upts_full[0].set(f)
     mom = sysm.moments(0)
-    cv, cw = vm.cv(), vm.cw()
     rho = f.sum(1) * cw
     U = np.stack([(f * cv[d][None, :, None]).sum(1) * cw / rho for d in range(3)])
     c = cv[:, None, :, None] - U[:, :, None, :]
@@ -170,8 +179,16 @@ def test_moments_kernel(device):
     assert np.allclose(mom[:, 0], rho, rtol=1e-13)
     assert np.allclose(mom[:, 1:4], U.transpose(1, 0, 2), rtol=1e-12, atol=1e-14)
     assert np.allclose(mom[:, 4], T, rtol=1e-12)
-    qx = (f * c[0] * c2).sum(1) * cw
-    assert np.allclose(mom[:, 5], qx, rtol=1e-10, atol=1e-13)
+    for i in range(3):
+        qi = (f * c[i] * c2).sum(1) * cw
+        assert np.abs(qi).min() > 1e-4
+        assert np.allclose(mom[:, 5 + i], qi, rtol=1e-10, atol=1e-13), i
+    for m, (i, j) in enumerate(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))):
+        pij = 2 * (f * c[i] * c[j]).sum(1) * cw
+        assert np.abs(pij).min() > 1e-4
+        assert np.allclose(mom[:, 8 + m], pij, rtol=1e-11, atol=1e-13), (i, j)
+    # the trace of P is 3 rho T (T = (2/3) <|c'|^2>)
+    assert np.allclose(mom[:, 8] + mom[:, 11] + mom[:, 13], 3 * rho * T, rtol=1e-12)
     assert np.allclose(mom[:, 14], rho * T, rtol=1e-12)
 
 
@@ -227,7 +244,7 @@ def _tri_nodal_weights(ele):
     WA = np.outer(wa, wa) * (1 - B) / 2
     xi = (1 + A) * (1 - B) / 2 - 1
     pts = np.stack([xi.ravel(), B.ravel()], axis=1)
-    L = ele.basis.ubasis.nodal_basis_at(pts)     # (npts, nupts)
+    L = ele.basis.ubasis.at(pts)                 # (npts, nupts)
     return WA.ravel() @ L
 
 
@@ -399,3 +416,63 @@ def test_triangle_advection_converges(order, device):
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     say("triangle advection p =", order, "errors", errs, "rates", rates)
     assert np.all(rates > order - 0.3), (errs, rates)
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_system_rhs_vs_oracle_orders(order, device):
+    """The RHS at p = 1 and at p = 3 (the cavity's order: 16 solution points,
+    M1 - M3 M2 of 16 x 32) against the quadrilateral oracle (1e-12)."""
+    be, sysm, _ = build_system(order=order)
+    vm = sysm.vm
+    ov = O.VelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M())
+    omesh = OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True)
+    walls = {"bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+             "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))}
+    orc = OF.AdvectionOracle(order, omesh, ov.cv, ov.cw, walls)
+    nupts = (order + 1) ** 2
+    rng = np.random.default_rng(30 + order)
+    f = OF.maxwellian(ov.cv, 1.0, [0.1, -0.05, 0], 0.95)[None, :, None] * \
+        (1 + 0.1 * rng.random((nupts, 1, 6))) * (1 + 0.05 * rng.random((1, ov.n, 1)))
+    say("system rhs p =", order)
+    r = _rhs_of(be, sysm, f)
+    err = rel(r, orc.rhs(f))
+    say("  rel L2", err)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("order,walls", [(1, "diffuse"), (2, "diffuse"), (2, "farfield")])
+def test_triangle_rhs_vs_oracle(order, walls, device):
+    """Triangles (Williams-Shunn points) on the x-periodic channel, against the
+    triangle oracle built from monomials and the definition of the DG lift
+    (oracle/fr.py TriFR): diffuse moving walls, or an inlet at the bottom and
+    an outlet at the top (1e-12)."""
+    from dgfs_hip.backend import HIPBackend
+    from dgfs_hip.cases import couette_cfg, maxwellian_section
+    from dgfs_hip.mesh import rect_tris
+    from dgfs_hip.solvers.dgfs.system import DGFSSystem
+    cfg = couette_cfg(Nv=16, L=5.0, Nrho=4, M=6, Kn=1.0, omega=0.5, order=order, uwall=0.3)
+    ov = O.VelocityMesh(16, 5.0, 4, 6)
+    if walls == "farfield":
+        maxwellian_section(cfg, "soln-bcs-bottom", 1.2, 0.9, (0.1, 0.6, 0.0), "dgfs-inlet")
+        cfg._cp.remove_section("soln-bcs-top")
+        cfg.set("soln-bcs-top", "type", "dgfs-outlet")
+        owalls = {"bottom": ("inlet", np.array([0.1, 0.6, 0.0]), OF.maxwellian(ov.cv, 1.2, [0.1, 0.6, 0.0], 0.9)),
+                  "top": ("outlet",)}
+    else:
+        owalls = {"bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+                  "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))}
+    be = HIPBackend(cfg)
+    mesh = rect_tris(3, 2, 0.0, 1.5, 0.0, 1.0, periodic_x=True)
+    sysm = DGFSSystem(be, mesh, cfg, 2)
+    verts = np.transpose(mesh.nodes, (1, 0, 2))
+    orc = OF.TriMeshAdvectionOracle(order, verts, ov.cv, ov.cw, walls=owalls, period=1.5,
+                                    boundary_name=lambda m: "bottom" if abs(m[1]) < 1e-9 else "top")
+    nupts, neles = sysm.ele.nupts, sysm.ele.neles
+    rng = np.random.default_rng(40 + order)
+    f = OF.maxwellian(ov.cv, 1.0, [0.1, -0.05, 0], 0.95)[None, :, None] * \
+        (1 + 0.1 * rng.random((nupts, 1, neles))) * (1 + 0.2 * np.tanh(ov.cv[0] - ov.cv[1]))[None, :, None]
+    say("triangle rhs p =", order, walls)
+    r = _rhs_of(be, sysm, f)
+    err = rel(r, orc.rhs(f))
+    say("  rel L2", err)
+    assert err < 1e-12

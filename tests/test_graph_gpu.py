@@ -58,3 +58,34 @@ def test_graph_replay_matches_eager(stepper, nbs, device):
     assert gs.nodes > 10
     assert float(scratch[0]) == 5.0
     gs.close()
+
+
+@pytest.mark.parametrize("stepper", ["dgfs-tvd-rk2", "dgfs-euler"])
+def test_graph_advance_runs_plugins_and_restarts(stepper, device):
+    """GraphStepper.advance_to calls every plugin after each accepted step,
+    counts the stepper's right-hand-side evaluations, and a stepper that
+    resumes at a non-zero step count (a restart) still warms up eagerly before
+    the capture; its states equal the eager stepper's."""
+    from dgfs_hip.integrators.graphs import GraphStepper
+    from dgfs_hip.integrators.steppers import get_stepper
+    be0, s0 = _system(False, None)
+    be1, s1 = _system(True, None)
+    dt = 0.01
+    eager = get_stepper(stepper, be0, s0, dt)
+    inner = get_stepper(stepper, be1, s1, dt)
+    inner.tcurr, inner.nacptsteps = 0.3, 30          # as restart() leaves them
+    eager.tcurr, eager.nacptsteps = 0.3, 30
+    seen = []
+    inner.plugins.append(lambda s: seen.append((s.nacptsteps, s.nfevals)))
+    gs = GraphStepper(inner)
+    say("graph advance", stepper)
+    gs.advance_to(0.3 + 4 * dt)
+    eager.advance_to(0.3 + 4 * dt)
+    per = 2 if stepper == "dgfs-tvd-rk2" else 1
+    assert seen == [(31 + k, per * (k + 1)) for k in range(4)]
+    assert inner.nfevals == eager.nfevals == 4 * per
+    a = s0.eles_scal_upts_full[eager.soln_bank].get()
+    b = s1.eles_scal_upts_full[inner.soln_bank].get()
+    assert float(np.abs(a - b).max() / np.abs(a).max()) <= 1e-15
+    assert gs.nodes > 10
+    gs.close()

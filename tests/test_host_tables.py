@@ -165,3 +165,43 @@ def test_autotune_candidates_for_both_operators():
     vhs = types.SimpleNamespace(Nv=32, precision=32, K=192, project=True, Nrho=16)
     assert sorted({x["algorithm"] for x in Autotuner.candidates(vhs, 256)}) == [1, 2, 3]
     assert _key(vhs, 4096) == "vhs-N32-p32-K192-proj1-pts4096"
+
+
+def test_autotune_cache_is_keyed_by_platform_and_drops_stale_entries(tmp_path, monkeypatch):
+    """A cached launch shape is reused only on the same device and library
+    build; an entry the operator rejects is dropped and the problem re-tuned."""
+    import types
+    from dgfs_hip import _lib, autotune
+    from dgfs_hip.autotune import Autotuner, _key
+    vhs = types.SimpleNamespace(Nv=16, precision=32, K=48, project=True, Nrho=8)
+    assert _key(vhs, 64, "MI355X|gfx950|v1") != _key(vhs, 64, "MI300X|gfx942|v1")
+    assert _key(vhs, 64, "MI355X|gfx950|v1") != _key(vhs, 64, "MI355X|gfx950|v2")
+    monkeypatch.setattr(autotune, "_platform", lambda: "dev|arch|ver")
+    tuner = Autotuner(cache_path=str(tmp_path / "tune.json"))
+    key = _key(vhs, 64, "dev|arch|ver")
+    tuner._cache = {key: {"best": {"algorithm": 9, "groups": 0, "slots": 0}, "ms": {}}}
+    applied = []
+
+    class Op(types.SimpleNamespace):
+        def set_algorithm(self, a):
+            if a == 9:
+                raise _lib.DGFSUnsupported("algorithm 9 is not built")
+            applied.append(a)
+
+        def set_groups(self, g):
+            pass
+
+        def set_slots(self, s):
+            pass
+
+        def eval_contig(self, *a):
+            pass
+
+    import torch
+    monkeypatch.setattr(torch.cuda, "get_device_properties", lambda i: types.SimpleNamespace(multi_processor_count=4))
+    monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a: None)
+    op = Op(**vars(vhs))
+    best, _ = tuner.tune(op, 0, 0, 64, stream=0)
+    assert best["algorithm"] in (1, 2, 3, 4)
+    assert tuner._cache[key]["best"] == best

@@ -88,7 +88,7 @@ def test_restart_raises_order(tmp_path, device):
     say("restart p=1 -> p=2")
     be2, s2 = _single(initsoln=NativeReader(p), order=2)
     got = s2.eles_scal_upts_full[0].get()
-    B = get_shape("quad", 1, "gauss-legendre").ubasis.nodal_basis_at(get_shape("quad", 2, "gauss-legendre").upts)
+    B = get_shape("quad", 1, "gauss-legendre").ubasis.at(get_shape("quad", 2, "gauss-legendre").upts)
     want = np.einsum("ij,jve->ive", B, f1)
     assert np.abs(got - want).max() <= 1e-14 * np.abs(want).max()
 

@@ -249,3 +249,29 @@ def test_eval_with_fallback_routes_points(device):
     assert np.array_equal(Qm_h[hi], Qx_h[hi])
     assert rel(Qm_h[~hi], Qc_h[~hi]) < 1e-12
     say("fallback routing ok", nfb)
+
+
+def test_fallback_rejects_an_operator_of_another_grid(device):
+    """The surrogate's fallback shares the caller's exact operator: a handle of
+    another grid (24^3 against the surrogate's 16^3, or another box) is
+    refused before any launch, by the wrapper and by the library itself."""
+    import ctypes as C
+    import torch
+    from dgfs_hip import _lib
+    from dgfs_hip.solvers.dgfs.scattering import VHSOperator
+    from dgfs_hip.solvers.dgfs.velocitymesh import DGFSVelocityMesh
+    sur = _sur(device)
+    f = torch.zeros((2, 16 ** 3), dtype=torch.float64, device=device)
+    Q = torch.full_like(f, 7.0)
+    for Nv, L in ((24, 6.0), (16, 5.0)):
+        op = VHSOperator.from_mesh(DGFSVelocityMesh.from_params(None, Nv, L, 4, 6), 0.38, 1.0)
+        assert op.get_dims()[0] == Nv and op.get_dims()[4] == L
+        with pytest.raises(ValueError):
+            sur.eval_with_fallback(op, f.data_ptr(), Q.data_ptr(), 2)
+        nfb = C.c_int()
+        with pytest.raises(_lib.DGFSError):
+            _lib.call("dgfs_sur_eval", sur.handle, op._h, C.c_void_p(sur.params_bf16.data_ptr()),
+                      C.c_void_p(f.data_ptr()), C.c_void_p(Q.data_ptr()), 2, 0.05, 1e-3, 0, C.byref(nfb),
+                      C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert bool(torch.all(Q == 7.0))
