@@ -3,39 +3,60 @@
 Mirrors the contract of frfs/backends/base/backend.py:28-176: kernels are
 looked up by name across the providers in order, a provider that cannot
 serve the arguments raises NotSuitableError and the next one is tried
-(backend.py:161-170).  Kernels returned here are bound HIP launches on the
-backend stream; ``queue << kern`` and ``runall`` keep the reference's queue
-vocabulary with stream-ordered (immediate) execution.
+(backend.py:161-170).  The providers (backend/kernels.py) are registered
+when the backend is made: BLAS extensions, the MFMA operator product and
+the pointwise kernels.  Queues take the reference's queue items
+``kernel(*args, **kwargs) -> (kernel, args, kwargs)`` with ``<<``, ``%`` and
+``runall`` (backends/base/types.py:281-334); execution is stream-ordered on
+the backend stream, so a queue run is a sequence of HIP launches.
 """
 import numpy as np
 
-from dgfs_hip.backend.types import ConstMatrix, Matrix, MatrixBank
+from dgfs_hip.backend.kernels import (HIPBlasExtKernels, HIPMFMAKernels, HIPPointwiseKernels,
+                                      NotSuitableError)
+from dgfs_hip.backend.types import ConstMatrix, Matrix, MatrixBank, View
+
+__all__ = ["HIPBackend", "NotSuitableError", "Queue"]
 
 
-class NotSuitableError(Exception):
-    pass
-
-
-class Queue(list):
-    """Kernel queue: kernels run in insertion order on the backend stream."""
+class Queue:
+    """Kernel queue: items run in insertion order on the backend stream."""
 
     def __init__(self, backend):
-        super().__init__()
         self.backend = backend
+        self._items = []
 
-    def __lshift__(self, kern):
-        self.append(kern)
+    @staticmethod
+    def _as_items(items):
+        if isinstance(items, tuple) and len(items) == 3 and hasattr(items[0], "run"):
+            return [items]
+        if hasattr(items, "run"):
+            return [(items, (), {})]
+        out = []
+        for it in items:
+            out.extend(Queue._as_items(it))
+        return out
+
+    def __lshift__(self, items):
+        self._items.extend(self._as_items(items))
         return self
 
-    def __mod__(self, kern):
-        self.append(kern)
+    def __mod__(self, items):
+        self.run()
+        self << items
         self.run()
         return self
 
+    def __bool__(self):
+        return bool(self._items)
+
+    def __len__(self):
+        return len(self._items)
+
     def run(self):
-        for k in self:
-            k.run(self)
-        self.clear()
+        items, self._items = self._items, []
+        for kern, args, kwargs in items:
+            kern.run(self, *args, **kwargs)
 
 
 class HIPBackend:
@@ -59,7 +80,8 @@ class HIPBackend:
         if prec not in ("double",):
             raise ValueError("the distribution registers are FP64 (precision = double)")
         self.fpdtype = np.float64
-        self._providers = []
+        self.pointwise = HIPPointwiseKernels(self)
+        self._providers = [HIPBlasExtKernels(self), HIPMFMAKernels(self), self.pointwise]
         # kernels run on torch's current stream, or on a stream of their own
         # ([backend-hip] stream = own), which HIP graph capture of the time
         # steps needs (the legacy default stream cannot be captured)
@@ -91,6 +113,10 @@ class HIPBackend:
     def matrix_bank(self, mats, initbank=0):
         return MatrixBank(self, mats, initbank)
 
+    def view(self, base, offsets):
+        """Flux points of one interface side: base matrix + element offsets."""
+        return View(self, base, offsets)
+
     def const_array(self, ary, dtype=np.float64):
         import torch
         return torch.as_tensor(np.ascontiguousarray(ary, dtype=dtype), device=self.device)
@@ -111,10 +137,14 @@ class HIPBackend:
                 return kern(*args, **kwargs)
             except NotSuitableError:
                 continue
-        raise KeyError(f"kernel not found: {name}")
+        raise KeyError(f"'{name}' has no providers")
 
-    def register_provider(self, prov):
-        self._providers.append(prov)
+    def register_provider(self, prov, first=False):
+        """Add a kernel provider (first=True: ahead of the built-in ones)."""
+        if first:
+            self._providers.insert(0, prov)
+        else:
+            self._providers.append(prov)
 
     def synchronize(self):
         import torch

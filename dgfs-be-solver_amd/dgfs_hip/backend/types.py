@@ -130,3 +130,18 @@ class MatrixBank(list):
     @active.setter
     def active(self, idx):
         self._curr_idx = idx
+
+
+class View:
+    """One side of a set of interface points (the reference's view of the
+    flux-point matrix, backends/base/types.py:211-264): interface point i of
+    variable v lives at base + offsets[i] + 32 v."""
+
+    def __init__(self, backend, base, offsets):
+        self.backend = backend
+        self.base = base
+        offsets = np.asarray(offsets, dtype=np.int64).reshape(-1)
+        if offsets.size and (offsets.min() < 0 or offsets.max() >= base.nrow * base.leaddim):
+            raise ValueError("view offsets outside the base matrix")
+        self.n = int(offsets.size)
+        self.offsets = backend.const_array(offsets if offsets.size else np.zeros(1, np.int64), np.int64)

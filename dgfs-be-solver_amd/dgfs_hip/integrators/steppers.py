@@ -3,9 +3,10 @@ register plumbing of integrators/base.py:152-214).
 
 The sequence of collide / updatebc / rhs / axnpby / register copies of each
 stage is the reference's; every operation is a HIP launch on the backend
-stream.
+stream.  The stage kernels come from the backend's provider walk
+(backend.kernel('axnpby' | 'copy_to_reg' | 'copy_from_reg', ...)), as the
+reference's integrators get them (integrators/base.py:152-214).
 """
-from dgfs_hip.backend.kernels import HIPBlasExtKernels
 
 
 class BaseDGFSStepper:
@@ -19,7 +20,6 @@ class BaseDGFSStepper:
         self.nacptsteps = 0
         self._idxcurr = 0
         self._regidx = list(range(self._stepper_nregs))
-        self._blas = HIPBlasExtKernels(backend)
         self._axk = {}
         self._cpk = {}
         self.nfevals = 0
@@ -34,21 +34,21 @@ class BaseDGFSStepper:
         idx = tuple(args[1::2])
         if idx not in self._axk:
             regs = self.system.ele.scal_upts
-            self._axk[idx] = self._blas.axnpby(*[regs[i] for i in idx])
+            self._axk[idx] = self.backend.kernel("axnpby", *[regs[i] for i in idx])
         self._axk[idx].run(None, *args[::2])
 
     def _copy_to_reg_dgfs(self, inp, out, sidx, eidx, rsidx, reidx):
         key = ("to", inp, out)
         if key not in self._cpk:
-            self._cpk[key] = self._blas.copy_to_reg(self.system.eles_scal_upts_full[inp],
-                                                    self.system.ele.scal_upts[out])
+            self._cpk[key] = self.backend.kernel("copy_to_reg", self.system.eles_scal_upts_full[inp],
+                                                 self.system.ele.scal_upts[out])
         self._cpk[key].run(None, sidx, eidx, rsidx, reidx)
 
     def _copy_from_reg_dgfs(self, inp, out, rsidx, reidx, sidx, eidx):
         key = ("from", inp, out)
         if key not in self._cpk:
-            self._cpk[key] = self._blas.copy_from_reg(self.system.ele.scal_upts[inp],
-                                                      self.system.eles_scal_upts_full[out])
+            self._cpk[key] = self.backend.kernel("copy_from_reg", self.system.ele.scal_upts[inp],
+                                                 self.system.eles_scal_upts_full[out])
         self._cpk[key].run(None, rsidx, reidx, sidx, eidx)
 
     def advance_to(self, tend):

@@ -1,11 +1,16 @@
 """FR element data and kernels of the DGFS system on the HIP path.
 
 Mirrors frfs/solvers/dgfs/elements.py:26-164 (disu, tdisf, tdivtpcorf,
-tdivtconf, negdivconf, invmass) and the geometry of
+tdivtconf, negdivconf) and the geometry of
 frfs/solvers/base/elements.py:189-354 (physical points, smats = |J| J^-1,
 djac, physical normals at flux points).  The mapping is the linear
 (straight-sided) one of the mesh nodes, evaluated directly at the target
 points.
+
+The kernels are obtained the way the reference's elements obtain them: the
+pointwise modules are registered with the backend and every kernel comes
+from the provider walk ``backend.kernel(name, ...)``; ``self.kernels`` maps
+the reference's names to factories, and the system instantiates them once.
 """
 import numpy as np
 
@@ -30,12 +35,37 @@ class DGFSElements:
         self.scal_upts_outb = be.matrix_bank(self.scal_upts)
         self._scal_fpts = mk((nfpts, nvars, neles))
         self._vect_upts = mk((ndims, nupts, nvars, neles))
-
-        self._M0 = be.const_array(shape.m0)
-        self._M132 = be.const_array(shape.opmat("M1 - M3*M2"))
-        self._M3 = be.const_array(shape.m3)
         self._smats_upts = be.const_matrix(self.smat_at_np("upts"), tags={"align"})
         self._rcpdjac_upts = be.const_matrix(self.rcpdjac_at_np("upts"), tags={"align"})
+        self._opmats = {}
+        self._set_kernels()
+
+    def opmat(self, expr):
+        """The operator expression as a device constant matrix (cached)."""
+        if expr not in self._opmats:
+            self._opmats[expr] = self.be.const_matrix(self.basis.opmat(expr))
+        return self._opmats[expr]
+
+    def _set_kernels(self):
+        """elements.py:26-164: disu, tdisf, tdivtpcorf, tdivtconf, negdivconf."""
+        be, vm = self.be, self.vm
+        be.pointwise.register("frfs.solvers.dgfs.kernels.negdivconf")
+        be.pointwise.register("frfs.solvers.dgfs.kernels.tflux")
+        tplargs = dict(ndims=self.ndims, nvars=self.nvars, vsize=vm.vsize())
+        srctplargs = dict(ndims=self.ndims, nvars=self.nvars, srcex=[])
+        self.kernels = {
+            "disu": lambda: be.kernel("mul", self.opmat("M0"), self.scal_upts_inb, out=self._scal_fpts),
+            "tdisf": lambda: be.kernel("tflux", tplargs=tplargs, dims=[self.nupts, self.neles],
+                                       u=self.scal_upts_inb, smats=self._smats_upts, f=self._vect_upts,
+                                       cvx=vm.d_cvx(), cvy=vm.d_cvy(), cvz=vm.d_cvz()),
+            "tdivtpcorf": lambda: be.kernel("mul", self.opmat("M1 - M3*M2"), self._vect_upts,
+                                            out=self.scal_upts_outb),
+            "tdivtconf": lambda: be.kernel("mul", self.opmat("M3"), self._scal_fpts, out=self.scal_upts_outb,
+                                           beta=1.0),
+            "negdivconf": lambda: be.kernel("negdivconf", tplargs=srctplargs, dims=[self.nupts, self.neles],
+                                            tdivtconf=self.scal_upts_outb, rcpdjac=self._rcpdjac_upts,
+                                            ploc=None, u=None),
+        }
 
     # geometry ------------------------------------------------------------
     def _map(self, pts):
@@ -87,34 +117,3 @@ class DGFSElements:
         """Offset of (fpt, var 0, e) in the flux-point matrix; var v adds 32 v."""
         m = self._scal_fpts
         return fpt * m.leaddim + (e // 32) * self.nvars * 32 + (e % 32)
-
-    # kernels (elements.py:56-164) -----------------------------------------
-    def disu(self, uin):
-        u = self.scal_upts[uin]
-        self._mul(self._M0, u, self._scal_fpts)
-
-    def tdisf(self, uin, vidx):
-        from dgfs_hip import fr
-        u = self.scal_upts[uin]
-        fr.tflux(self.ndims, self.nupts, self.neles, self.nvars, vidx, u, u.leaddim,
-                 self._smats_upts, self._smats_upts.leaddim, self._vect_upts,
-                 self._vect_upts.leaddim, self.vm.d_cv(), self.vm.vsize(), self.be.stream_ptr)
-
-    def tdivtpcorf(self, fout):
-        self._mul(self._M132, self._vect_upts, self.scal_upts[fout])
-
-    def tdivtconf(self, fout):
-        self._mul(self._M3, self._scal_fpts, self.scal_upts[fout], beta=1.0)
-
-    def negdivconf(self, fout):
-        from dgfs_hip import fr
-        out = self.scal_upts[fout]
-        fr.negdivconf(self.nupts, self.neles, self.nvars, out, out.leaddim, self._rcpdjac_upts,
-                      self._rcpdjac_upts.leaddim, self.be.stream_ptr)
-
-    def _mul(self, A, b, out, alpha=1.0, beta=0.0):
-        from dgfs_hip import fr
-        m, k = A.shape
-        if b.nrow != k or out.nrow != m:
-            raise ValueError("operator shape mismatch")
-        fr.mul(A, m, k, k, b, b.leaddim, out, out.leaddim, b.ncol, alpha, beta, self.be.stream_ptr)

@@ -1,16 +1,21 @@
 """Interface and boundary fluxes of the DGFS system on the HIP path.
 
-Mirrors frfs/solvers/dgfs/inters.py:17-492.  An interface side is addressed
-the way the reference's views address it (backends/base/types.py View):
-each face flux point maps to (row = element flux point, column = element),
-and here to one offset into the flux-point matrix (variable v at +32 v).
-Flux points of the two sides are paired by physical location
+Mirrors frfs/solvers/dgfs/inters.py:17-492.  An interface side is a view of
+the element flux-point matrix, as in the reference (backends/base/types.py
+View): each interface point maps to (row = element flux point, column =
+element), here to one offset into the flux-point matrix (variable v at
++32 v).  Flux points of the two sides are paired by physical location
 (BaseElements._srtd_face_fpts), and the interface points are ordered so
 consecutive points touch consecutive left elements (get_opt_view_perm).
+
+Each class registers its pointwise module and keeps kernel factories in
+``self.kernels`` ('comm_flux', and 'update_bc' for walls that accumulate),
+built from ``backend.kernel('intcflux' | 'bccflux' | 'bccupdate',
+tplargs, dims, ul=..., nl=..., magnl=..., cvx/cvy/cvz=..., bnd_f0=...,
+bc_vals=...)`` with the reference's argument names.
 """
 import numpy as np
 
-from dgfs_hip import fr
 from dgfs_hip.solvers.dgfs.initcond import DGFSMaxwellianInitCondition
 
 
@@ -43,24 +48,25 @@ class DGFSIntInters:
         roff = np.array([ele.fpt_offset(r[5], r[4]) for r in recs], dtype=np.int64)
         nl = np.array([ele.norm_pnorm_fpts[r[3], r[2]] for r in recs]).reshape(n, ele.ndims).T
         magnl = np.array([ele.mag_pnorm_fpts[r[3], r[2]] for r in recs])
-        self._loff = be.const_array(loff, np.int64)
-        self._roff = be.const_array(roff, np.int64)
-        self._nl = be.const_array(np.ascontiguousarray(nl))
-        self._magnl = be.const_array(magnl)
+        self._scal_lhs = be.view(ele._scal_fpts, loff)
+        self._scal_rhs = be.view(ele._scal_fpts, roff)
+        self._nl = be.const_array(np.ascontiguousarray(nl) if n else np.zeros((ele.ndims, 1)))
+        self._magnl = be.const_array(magnl if n else np.zeros(1))
         self._recs = recs
-
-    def comm_flux(self, vidx):
-        ele, vm = self.ele, self.ele.vm
-        if self.ninterfpts == 0:
-            return
-        m = ele._scal_fpts
-        fr.intcflux(ele.ndims, self.ninterfpts, ele.nvars, vidx, m, self._loff, m, self._roff,
-                    self._nl, self._magnl, vm.d_cv(), vm.vsize(), self.be.stream_ptr)
+        vm = ele.vm
+        be.pointwise.register("frfs.solvers.dgfs.kernels.intcflux")
+        tplargs = dict(ndims=ele.ndims, nvars=ele.nvars, rsolver="rusanov", vsize=vm.vsize())
+        self.kernels = {
+            "comm_flux": lambda: be.kernel("intcflux", tplargs=tplargs, dims=[self.ninterfpts],
+                                           ul=self._scal_lhs, ur=self._scal_rhs, magnl=self._magnl,
+                                           nl=self._nl, cvx=vm.d_cvx(), cvy=vm.d_cvy(), cvz=vm.d_cvz()),
+        }
 
 
 class DGFSBCInters:
     type = None
-    bctype = None
+    # accumulating walls have an update kernel (bccupdate)
+    updates = False
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         self.be = be
@@ -74,17 +80,19 @@ class DGFSBCInters:
         recs.sort()
         self.ninterfpts = n = len(recs)
         self._recs = recs
-        self._loff = be.const_array(np.array([ele.fpt_offset(r[3], r[2]) for r in recs], dtype=np.int64),
-                                    np.int64)
+        self._scal_lhs = be.view(ele._scal_fpts, [ele.fpt_offset(r[3], r[2]) for r in recs])
         nl = np.array([ele.norm_pnorm_fpts[r[3], r[2]] for r in recs]).reshape(n, ele.ndims)
         self.nl_np = nl
-        self._nl = be.const_array(np.ascontiguousarray(nl.T))
-        self._magnl = be.const_array(np.array([ele.mag_pnorm_fpts[r[3], r[2]] for r in recs]))
+        self._nl = be.const_array(np.ascontiguousarray(nl.T) if n else np.zeros((ele.ndims, 1)))
+        self._magnl = be.const_array(np.array([ele.mag_pnorm_fpts[r[3], r[2]] for r in recs]) if n
+                                     else np.zeros(1))
         self.ploc = np.array([ele.plocfpts[r[3], r[2]] for r in recs])
         self._f0 = None
         self._bcvals = None
         self._rmap = None
         self._U = (0.0, 0.0, 0.0)
+        self.kernels = {}
+        self._set_kernels()
 
     def _wall_maxwellian(self):
         bc = DGFSMaxwellianInitCondition(self.be, self.cfg, self.ele.vm, self.cfgsect)
@@ -92,21 +100,33 @@ class DGFSBCInters:
         u = bc.unondim()
         self._U = (float(u[0, 0]), float(u[1, 0]), float(u[2, 0]))
 
-    def update_bc(self, t, vidx):
-        pass
+    def _tplargs(self):
+        vm = self.ele.vm
+        c = dict(Ux=self._U[0], Uy=self._U[1], Uz=self._U[2])
+        return dict(ndims=self.ele.ndims, nvars=self.ele.nvars, rsolver="rusanov", c=c, bctype=self.type,
+                    vsize=vm.vsize(), cw=vm.cw())
 
-    def comm_flux(self, t, vidx):
-        ele, vm = self.ele, self.ele.vm
-        if self.ninterfpts == 0:
-            return
-        fr.bccflux(self.bctype, ele.ndims, self.ninterfpts, ele.nvars, vidx, ele._scal_fpts, self._loff,
-                   self._nl, self._magnl, self._f0, self._bcvals, self._rmap, self._U, vm.d_cv(),
-                   vm.vsize(), self.be.stream_ptr)
+    def _set_kernels(self):
+        """inters.py:232-443: register bccflux (and bccupdate) and keep the
+        kernel factories.  The factories read the boundary data (inflow or
+        wall distribution, velocity, balance storage, reflection map) when the
+        system instantiates them, after every subclass has set it."""
+        be, vm = self.be, self.ele.vm
+        be.pointwise.register("frfs.solvers.dgfs.kernels.bccflux")
+        cvs = lambda: dict(cvx=vm.d_cvx(), cvy=vm.d_cvy(), cvz=vm.d_cvz())
+        self.kernels["comm_flux"] = lambda: be.kernel(
+            "bccflux", tplargs=self._tplargs(), dims=[self.ninterfpts], ul=self._scal_lhs, magnl=self._magnl,
+            nl=self._nl, bnd_f0=self._f0, bc_vals=self._bcvals, rmap=self._rmap, **cvs())
+        if self.updates:
+            be.pointwise.register("frfs.solvers.dgfs.kernels.bccupdate")
+            self.kernels["update_bc"] = lambda: be.kernel(
+                "bccupdate", tplargs=self._tplargs(), dims=[self.ninterfpts], ul=self._scal_lhs, nl=self._nl,
+                bnd_f0=self._f0, bc_vals=self._bcvals, **cvs())
 
 
 class DGFSWallDiffuseBCInters(DGFSBCInters):
     type = "dgfs-wall-diffuse"
-    bctype = fr.BC_DIFFUSE
+    updates = True
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         super().__init__(be, ele, faces, cfg, cfgsect)
@@ -114,18 +134,10 @@ class DGFSWallDiffuseBCInters(DGFSBCInters):
         import torch
         self._bcvals = torch.zeros(2 * max(self.ninterfpts, 1), dtype=torch.float64, device=be.device)
 
-    def update_bc(self, t, vidx):
-        ele, vm = self.ele, self.ele.vm
-        if self.ninterfpts == 0:
-            return
-        fr.bccupdate(self.bctype, ele.ndims, self.ninterfpts, ele.nvars, vidx, ele._scal_fpts, self._loff,
-                     self._nl, self._f0, self._bcvals, self._U, vm.cw(), vm.d_cv(), vm.vsize(),
-                     self.be.stream_ptr)
-
 
 class DGFSWallSpecularBCInters(DGFSBCInters):
     type = "dgfs-wall-specular"
-    bctype = fr.BC_SPECULAR
+    updates = True
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         super().__init__(be, ele, faces, cfg, cfgsect)
@@ -145,18 +157,9 @@ class DGFSWallSpecularBCInters(DGFSBCInters):
         self._bcvals = torch.zeros(ele.vm.vsize() * max(self.ninterfpts, 1), dtype=torch.float64,
                                    device=be.device)
 
-    def update_bc(self, t, vidx):
-        ele, vm = self.ele, self.ele.vm
-        if self.ninterfpts == 0:
-            return
-        fr.bccupdate(self.bctype, ele.ndims, self.ninterfpts, ele.nvars, vidx, ele._scal_fpts, self._loff,
-                     self._nl, None, self._bcvals, self._U, vm.cw(), vm.d_cv(), vm.vsize(),
-                     self.be.stream_ptr)
-
 
 class DGFSInletBCInters(DGFSBCInters):
     type = "dgfs-inlet"
-    bctype = fr.BC_INLET
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         super().__init__(be, ele, faces, cfg, cfgsect)
@@ -165,7 +168,6 @@ class DGFSInletBCInters(DGFSBCInters):
 
 class DGFSInletNormalShockBCInters(DGFSBCInters):
     type = "dgfs-inlet-normalshock"
-    bctype = fr.BC_INLET_NS
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         super().__init__(be, ele, faces, cfg, cfgsect)
@@ -174,12 +176,10 @@ class DGFSInletNormalShockBCInters(DGFSBCInters):
 
 class DGFSOutletBCInters(DGFSBCInters):
     type = "dgfs-outlet"
-    bctype = fr.BC_OUTLET
 
 
 class DGFSVanishBCInters(DGFSBCInters):
     type = "dgfs-vanish"
-    bctype = fr.BC_VANISH
 
 
 class DGFSWallDiffuseCylBCInters(DGFSBCInters):
@@ -189,7 +189,7 @@ class DGFSWallDiffuseCylBCInters(DGFSBCInters):
     point; the wall emits the unnormalised Maxwellian exp(-|c - U|^2 / T)
     scaled by the per-point flux balance."""
     type = "dgfs-wall-diffuse-cyl"
-    bctype = fr.BC_DIFFUSE_CYL
+    updates = True
 
     def __init__(self, be, ele, faces, cfg, cfgsect):
         super().__init__(be, ele, faces, cfg, cfgsect)
@@ -207,11 +207,3 @@ class DGFSWallDiffuseCylBCInters(DGFSBCInters):
         self._f0 = be.const_array(st.ravel())
         import torch
         self._bcvals = torch.zeros(2 * max(n, 1), dtype=torch.float64, device=be.device)
-
-    def update_bc(self, t, vidx):
-        ele, vm = self.ele, self.ele.vm
-        if self.ninterfpts == 0:
-            return
-        fr.bccupdate(self.bctype, ele.ndims, self.ninterfpts, ele.nvars, vidx, ele._scal_fpts, self._loff,
-                     self._nl, self._f0, self._bcvals, self._U, vm.cw(), vm.d_cv(), vm.vsize(),
-                     self.be.stream_ptr)

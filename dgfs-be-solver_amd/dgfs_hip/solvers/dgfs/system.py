@@ -5,6 +5,10 @@ interfaces, the scattering model and the full-velocity registers
 (nupts, vsize, neles).  The element registers hold one velocity batch of
 NvBatchSize velocities; rhs/updatebc act on the batch starting at varidx,
 collide on the full registers, exactly as the reference's steppers expect.
+
+Kernels are instantiated once from the factories of the elements and
+interfaces (frfs/solvers/base/system.py:167-183, ``_gen_kernels``) and
+run through a backend queue in the reference's order.
 """
 from collections.abc import Mapping
 
@@ -17,6 +21,24 @@ from dgfs_hip.solvers.dgfs.inters import DGFSBCInters, DGFSIntInters
 from dgfs_hip.solvers.dgfs.scattering import DGFSScatteringModel
 from dgfs_hip.solvers.dgfs.velocitymesh import DGFSVelocityMesh
 from dgfs_hip.util import subclass_where
+
+
+class KernelList(list):
+    """Kernels of one name across several objects: calling the list gives
+    the queue items of all of them (the reference's proxylist)."""
+
+    def __call__(self, *args, **kwargs):
+        return [k(*args, **kwargs) for k in self]
+
+
+def gen_kernels(ele, iint, bcints, bcnames=("comm_flux", "update_bc")):
+    """{(kind, name): kernel or KernelList}: every factory of the elements and
+    interfaces instantiated once (frfs/solvers/base/system.py:167-183)."""
+    out = {("eles", n): f() for n, f in ele.kernels.items()}
+    out["iint", "comm_flux"] = KernelList([iint.kernels["comm_flux"]()])
+    for n in bcnames:
+        out["bcint", n] = KernelList(b.kernels[n]() for b in bcints if n in b.kernels)
+    return out
 
 
 class DGFSSystem:
@@ -74,24 +96,31 @@ class DGFSSystem:
         self.eles_scal_upts_full = [be.matrix((nupts, vsize, neles), f0, tags={"align"})
                                     for _ in range(nregs)]
         self.eles_scal_upts_inb_full = be.matrix_bank(self.eles_scal_upts_full)
+        self._kernels = gen_kernels(self.ele, self.iint, self.bcints)
+        self._queue = be.queue()
 
     def rhs(self, t, uinbank, foutbank, batch, varidx):
         """Advection of one velocity batch: fout = -div F(uin) (system.py:140-177)."""
-        ele = self.ele
-        ele.disu(uinbank)
-        ele.tdisf(uinbank, varidx)
-        ele.tdivtpcorf(foutbank)
-        self.iint.comm_flux(varidx)
-        for b in self.bcints:
-            b.comm_flux(t, varidx)
-        ele.tdivtconf(foutbank)
-        ele.negdivconf(foutbank)
+        k, q = self._kernels, self._queue
+        self.ele.scal_upts_inb.active = uinbank
+        self.ele.scal_upts_outb.active = foutbank
+        q << k["eles", "disu"]()
+        q << k["eles", "tdisf"](varidx=varidx)
+        q << k["eles", "tdivtpcorf"]()
+        q << k["iint", "comm_flux"](varidx=varidx)
+        q << k["bcint", "comm_flux"](t=t, varidx=varidx)
+        q << k["eles", "tdivtconf"]()
+        q << k["eles", "negdivconf"](t=t)
+        self.backend.runall([q])
 
     def updatebc(self, t, uinbank, foutbank, batch, varidx):
         """Boundary accumulations over one velocity batch (system.py:180-196)."""
-        self.ele.disu(uinbank)
-        for b in self.bcints:
-            b.update_bc(t, varidx)
+        k, q = self._kernels, self._queue
+        self.ele.scal_upts_inb.active = uinbank
+        self.ele.scal_upts_outb.active = foutbank
+        q << k["eles", "disu"]()
+        q << k["bcint", "update_bc"](t=t, varidx=varidx)
+        self.backend.runall([q])
 
     def collide(self, t, uinbank, foutbank):
         """Q at every solution point: one batched call (system.py:199-218)."""

@@ -124,3 +124,8 @@ class DGFSVelocityMesh:
         if self._d_cv is None:
             self._d_cv = self.backend.const_array(self._cv)
         return self._d_cv
+
+    # the three rows of d_cv(), the per-axis arrays of the reference's meshes
+    def d_cvx(self): return self.d_cv()[0]
+    def d_cvy(self): return self.d_cv()[1]
+    def d_cvz(self): return self.d_cv()[2]

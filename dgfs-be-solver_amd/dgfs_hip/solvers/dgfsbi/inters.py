@@ -43,11 +43,15 @@ class DGFSBiBCInters:
         u = bc.unondim()
         b._U = (float(u[0, 0]), float(u[1, 0]), float(u[2, 0]))
 
-    def comm_flux(self, t, idx, vidx):
-        self.parts[int(idx)].comm_flux(t, vidx)
-
-    def update_bc(self, t, idx, vidx):
-        self.parts[int(idx)].update_bc(t, vidx)
+    @property
+    def kernels(self):
+        """Per-species factories under the reference's names comm_flux<p>
+        and update_bc<p> (dgfsbi/inters.py: comm_flux0, comm_flux1, ...)."""
+        out = {}
+        for p, b in enumerate(self.parts):
+            for n, f in b.kernels.items():
+                out[f"{n}{p}"] = f
+        return out
 
 
 class DGFSBiWallDiffuseBCInters(DGFSBiBCInters):

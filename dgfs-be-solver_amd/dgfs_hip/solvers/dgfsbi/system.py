@@ -17,6 +17,7 @@ import numpy as np
 from dgfs_hip.shapes import get_shape
 from dgfs_hip.solvers.dgfs.elements import DGFSElements
 from dgfs_hip.solvers.dgfs.inters import DGFSIntInters
+from dgfs_hip.solvers.dgfs.system import gen_kernels
 from dgfs_hip.solvers.dgfsbi.initcond import DGFSBiInitCondition
 from dgfs_hip.solvers.dgfsbi.inters import DGFSBiBCInters
 from dgfs_hip.solvers.dgfsbi.scattering import DGFSBiScatteringModel
@@ -94,26 +95,34 @@ class DGFSBiSystem:
         self.eles_scal_upts_full = [be.matrix((nupts, vsize, neles), f0[p], tags={"align"})
                                     for p in range(nspcs) for _ in range(self.nregs0)]
         self.eles_scal_upts_inb_full = be.matrix_bank(self.eles_scal_upts_full)
+        names = [f"{n}{p}" for p in range(nspcs) for n in ("comm_flux", "update_bc")]
+        self._kernels = gen_kernels(self.ele, self.iint, self.bcints, names)
+        self._queue = be.queue()
 
     def rhs(self, t, idx, uinbank, foutbank, batch, varidx):
         """Advection of one velocity batch of species idx (system.py:160-200)."""
         p = int(idx)
-        ele = self.ele
-        ele.disu(uinbank)
-        ele.tdisf(uinbank, varidx)
-        ele.tdivtpcorf(foutbank)
-        self.iint.comm_flux(varidx)
-        for b in self.bcints:
-            b.comm_flux(t, p, varidx)
-        ele.tdivtconf(foutbank)
-        ele.negdivconf(foutbank)
+        k, q = self._kernels, self._queue
+        self.ele.scal_upts_inb.active = uinbank
+        self.ele.scal_upts_outb.active = foutbank
+        q << k["eles", "disu"]()
+        q << k["eles", "tdisf"](varidx=varidx)
+        q << k["eles", "tdivtpcorf"]()
+        q << k["iint", "comm_flux"](varidx=varidx)
+        q << k["bcint", f"comm_flux{p}"](t=t, varidx=varidx)
+        q << k["eles", "tdivtconf"]()
+        q << k["eles", "negdivconf"](t=t)
+        self.backend.runall([q])
 
     def updatebc(self, t, idx, uinbank, foutbank, batch, varidx):
         """Boundary accumulations of species idx over one batch (system.py:203-220)."""
         p = int(idx)
-        self.ele.disu(uinbank)
-        for b in self.bcints:
-            b.update_bc(t, p, varidx)
+        k, q = self._kernels, self._queue
+        self.ele.scal_upts_inb.active = uinbank
+        self.ele.scal_upts_outb.active = foutbank
+        q << k["eles", "disu"]()
+        q << k["bcint", f"update_bc{p}"](t=t, varidx=varidx)
+        self.backend.runall([q])
 
     def collide(self, t, regids):
         """regids = (in_0, ..., in_{S-1}, out_0, ..., out_{S-1}) full-register

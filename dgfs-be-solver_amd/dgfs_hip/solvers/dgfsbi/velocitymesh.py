@@ -126,3 +126,8 @@ class DGFSBiVelocityMesh:
         if getattr(self, "_d_cv", None) is None:
             self._d_cv = self.backend.const_array(self._cv)
         return self._d_cv
+
+    # the three rows of d_cv(), the per-axis arrays of the reference's meshes
+    def d_cvx(self): return self.d_cv()[0]
+    def d_cvy(self): return self.d_cv()[1]
+    def d_cvz(self): return self.d_cv()[2]

@@ -28,6 +28,11 @@ How each function is pinned (tests/test_oracle_*.py):
   q_spectral -- brute-force DFT evaluation (q_bruteforce, explicit n x n
       matrices) at N=4 and 6; the BKW closed-form time derivative for Maxwell
       molecules; Q(Maxwellian) ~ 0; translation equivariance; conservation.
+  vhs_prefactor -- the VHS kernel derived from Bird's cross-section in SI
+      units and the hard-sphere mean free path (tests/test_prefactor.py).
+  moments5 -- Maxwellian and BKW closed-form moments (tests/test_oracle_bkw.py).
+  project_invariants -- constraints, idempotence, correction in the row space
+      of the constraints (tests/test_oracle_vhs.py).
 """
 from __future__ import annotations
 
@@ -361,3 +366,19 @@ def collision_invariants(vm: VelocityMesh, f):
     c2 = np.sum(vm.cv ** 2, axis=0)
     return np.stack([f.sum(-1), (f * vm.cv[0]).sum(-1), (f * vm.cv[1]).sum(-1),
                      (f * vm.cv[2]).sum(-1), (f * c2).sum(-1)], axis=-1) * vm.cw
+
+
+def project_invariants(vm: VelocityMesh, Q):
+    """Orthogonal projection of Q (..., n) onto {Q : sum_r Q_r phi_j(c_r) cw = 0,
+    j = 1..5} with phi = (1, cx, cy, cz, |c|^2): the minimum-norm correction
+    Q - C^+ C Q, C the 5 x n constraint matrix (pseudo-inverse by SVD).
+    This is what the FP64 projection of the HIP operators computes (by the
+    normal equations).  Pinned by its defining properties: the constraints hold,
+    the map is idempotent, the correction lies in the row space of C (so no
+    other point of the constraint set is closer)."""
+    Q = np.asarray(Q, dtype=np.float64)
+    c = vm.cv
+    C = np.stack([np.ones(vm.n), c[0], c[1], c[2], (c * c).sum(0)]) * vm.cw
+    flat = Q.reshape(-1, vm.n)
+    out = flat - (np.linalg.pinv(C) @ (C @ flat.T)).T
+    return out.reshape(Q.shape)

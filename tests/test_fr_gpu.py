@@ -476,3 +476,60 @@ def test_triangle_rhs_vs_oracle(order, walls, device):
     err = rel(r, orc.rhs(f))
     say("  rel L2", err)
     assert err < 1e-12
+
+
+def test_rhs_through_the_provider_walk(device):
+    """The advection RHS assembled only from backend.kernel(...) calls with
+    the reference's kernel names and argument names (elements.py:56-164,
+    inters.py:17-58 and 232-309: 'mul', 'tflux', 'intcflux', 'bccupdate',
+    'bccflux', 'negdivconf'), run through a backend queue, equals the
+    oracle (1e-12) and the system's own rhs (bitwise)."""
+    be, sysm, _ = build_system()
+    ele, vm, iint = sysm.ele, sysm.vm, sysm.iint
+    for mod in ("tflux", "negdivconf", "intcflux", "bccflux", "bccupdate"):
+        be.pointwise.register(f"frfs.solvers.dgfs.kernels.{mod}")
+    cvs = dict(cvx=vm.d_cvx(), cvy=vm.d_cvy(), cvz=vm.d_cvz())
+    nupts, neles, nvars = ele.nupts, ele.neles, ele.nvars
+    disu = be.kernel("mul", ele.opmat("M0"), ele.scal_upts_inb, out=ele._scal_fpts)
+    tdisf = be.kernel("tflux", tplargs=dict(ndims=2, nvars=nvars, vsize=vm.vsize()), dims=[nupts, neles],
+                      u=ele.scal_upts_inb, smats=ele._smats_upts, f=ele._vect_upts, **cvs)
+    tdivtpcorf = be.kernel("mul", ele.opmat("M1 - M3*M2"), ele._vect_upts, out=ele.scal_upts_outb)
+    tdivtconf = be.kernel("mul", ele.opmat("M3"), ele._scal_fpts, out=ele.scal_upts_outb, beta=1.0)
+    negdiv = be.kernel("negdivconf", tplargs=dict(ndims=2, nvars=nvars, srcex=[]), dims=[nupts, neles],
+                       tdivtconf=ele.scal_upts_outb, rcpdjac=ele._rcpdjac_upts)
+    icomm = be.kernel("intcflux", tplargs=dict(ndims=2, nvars=nvars, rsolver="rusanov", vsize=vm.vsize()),
+                      dims=[iint.ninterfpts], ul=iint._scal_lhs, ur=iint._scal_rhs, magnl=iint._magnl,
+                      nl=iint._nl, **cvs)
+    bcomm, bupd = [], []
+    for b in sysm.bcints:
+        tpl = dict(ndims=2, nvars=nvars, rsolver="rusanov", bctype=b.type, vsize=vm.vsize(), cw=vm.cw(),
+                   c=dict(Ux=b._U[0], Uy=b._U[1], Uz=b._U[2]))
+        bcomm.append(be.kernel("bccflux", tplargs=tpl, dims=[b.ninterfpts], ul=b._scal_lhs, magnl=b._magnl,
+                               nl=b._nl, bnd_f0=b._f0, bc_vals=b._bcvals, **cvs))
+        bupd.append(be.kernel("bccupdate", tplargs=tpl, dims=[b.ninterfpts], ul=b._scal_lhs, nl=b._nl,
+                              bnd_f0=b._f0, bc_vals=b._bcvals, **cvs))
+    ov = O.VelocityMesh(vm.Nv(), vm.L(), vm.Nrho(), vm.M())
+    orc = OF.AdvectionOracle(2, OF.RectMesh(3, 2, 0, 1.5, 0, 1.0, periodic_x=True), ov.cv, ov.cw, {
+        "bottom": (np.array([-0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [-0.3, 0, 0], 1)),
+        "top": (np.array([0.3, 0, 0]), OF.maxwellian(ov.cv, 1, [0.3, 0, 0], 1))})
+    f = _perturbed(ov, 21)
+    ele.scal_upts[0].set(f)
+    ele.scal_upts_inb.active, ele.scal_upts_outb.active = 0, 1
+    q = be.queue()
+    q << disu()
+    q << [k(t=0.0, varidx=0) for k in bupd]
+    be.runall([q])
+    q << disu()
+    q << tdisf(varidx=0)
+    q << tdivtpcorf()
+    q << icomm(varidx=0)
+    q << [k(t=0.0, varidx=0) for k in bcomm]
+    q << tdivtconf()
+    q << negdiv(t=0.0)
+    be.runall([q])
+    be.synchronize()
+    walk = ele.scal_upts[1].get()
+    err = rel(walk, orc.rhs(f))
+    say("rhs through the provider walk: rel L2 vs oracle", err)
+    assert err < 1e-12
+    assert np.array_equal(walk, _rhs_of(be, sysm, f))

@@ -99,20 +99,11 @@ def test_group_split_invariance(device):
     assert rel(outs[1], outs[0]) < 1e-13 and rel(outs[2], outs[0]) < 1e-13
 
 
-def project_np(ov, Q):
-    """Minimum-norm correction onto the invariants {1, c, |c|^2}: the
-    restatement of the FP64 projection used to pin the GPU kernel."""
-    phi = np.stack([np.ones(ov.n), ov.cv[0], ov.cv[1], ov.cv[2], (ov.cv ** 2).sum(0)])
-    C = phi * ov.cw
-    lam = np.linalg.solve(C @ C.T, (C @ Q.T))
-    return Q - (C.T @ lam).T
-
-
 def test_projection_conserves(device):
     ov = O.VelocityMesh(24, 7.0, 12, 12)
     f = smooth_states(ov, 4, seed=11)
     Qref = O.VHSOracle(ov, 0.38, 1.0).q(f)
-    Qref_p = project_np(ov, Qref)
+    Qref_p = O.project_invariants(ov, Qref)
     say("projection size relative to Q:", rel(Qref_p, Qref))
     outs = {}
     for prec in (64, 32):
