@@ -79,10 +79,14 @@ int dgfs_vhs_eval(void* handle, const double* f, double* Q, int nupts, int ldim,
 int dgfs_vhs_eval_contig(void* handle, const double* f, double* Q, int npts, void* stream);
 
 /* Two independent contiguous batches on two velocity grids (32+24, 32+16 or
- * 24+16 points per axis, same precision) with their gain work items in one
- * persistent launch (multi-pass gain for both).  Batches beyond one workspace
- * chunk run as separate launches.  No reference counterpart (the reference has
- * one velocity grid per run). */
+ * 24+16 points per axis, same precision) evaluated together.  When both
+ * handles take a register-resident gain (FP32), the two pipelines run on the
+ * caller's stream and a second stream of handle_b, forked and joined by
+ * events, with the two persistent gain grids splitting the CUs in proportion
+ * to their work; Q equals two separate calls bitwise.  Otherwise the gain
+ * work items of both grids share one multi-pass persistent launch.  Batches
+ * beyond one workspace chunk run as separate launches.  No reference
+ * counterpart (the reference has one velocity grid per run). */
 int dgfs_vhs_eval_pair(void* handle_a, const double* f_a, double* Q_a, int npts_a, void* handle_b,
                        const double* f_b, double* Q_b, int npts_b, void* stream);
 

@@ -235,6 +235,36 @@ def test_pair_launch_matches_separate(precision, device):
     assert ea < tol and eb < tol
 
 
+def test_pair_register_resident_c3_grids(device):
+    """The C3 shapes (32^3 on [-6, 6]^3 and a 24^3 batch, M = 12, Nrho = 16,
+    FP32 with FP64 projection) in one paired call: both register-resident
+    gains run side by side on split CUs.  Q equals the separate calls bitwise
+    (same work items and summation order), and each grid meets the FP32 bar
+    against the FP64 oracle with the same projection."""
+    import torch
+    opa, _ = make_op(32, 6.0, 16, 12, 0.38, 1.0, precision=32, project=True)
+    opb, _ = make_op(24, 6.0, 16, 12, 0.38, 1.0, precision=32, project=True)
+    ova, ovb = O.VelocityMesh(32, 6.0, 16, 12), O.VelocityMesh(24, 6.0, 16, 12)
+    fa = smooth_states(ova, 6, seed=31)
+    fb = smooth_states(ovb, 5, seed=32)
+    Qa = run_contig(opa, fa, device)
+    Qb = run_contig(opb, fb, device)
+    fa_t, fb_t = (torch.as_tensor(x, device=device) for x in (fa, fb))
+    for rep in range(2):
+        Qa_t, Qb_t = torch.full_like(fa_t, float("nan")), torch.full_like(fb_t, float("nan"))
+        say("register-resident pair, call", rep)
+        opa.eval_pair(fa_t.data_ptr(), Qa_t.data_ptr(), 6, opb, fb_t.data_ptr(), Qb_t.data_ptr(), 5,
+                      torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert opa.get_algorithm() in (2, 3) and opb.get_algorithm() == 4
+        assert np.array_equal(Qa_t.cpu().numpy(), Qa) and np.array_equal(Qb_t.cpu().numpy(), Qb)
+    for ov, f, Q in ((ova, fa, Qa), (ovb, fb, Qb)):
+        ref = O.project_invariants(ov, O.VHSOracle(ov, 0.38, 1.0).q(f))
+        e = rel(Q, ref)
+        say("  pair grid", ov.N, "FP32 vs FP64 oracle (both projected)", e)
+        assert e < 1e-3
+
+
 def test_autotuner_picks_a_valid_shape(tmp_path, device):
     """The tuner sweeps algorithm x radial groups x persistent grid on a
     sample batch, applies the fastest, caches it, and Q is unchanged."""
