@@ -401,9 +401,11 @@ def bench_pair(args, dev, stream):
     """The batching workload: the 4096 cavity-shape points at 32^3 and a second
     batch of 1024 homogeneous bimodal points at 24^3 (same box, M, Nrho,
     omega; FP32 + FP64 projection) evaluated by one paired call, in which the
-    two register-resident gains run side by side on split CUs; against the two
-    batches evaluated one after the other, with per-grid errors against the
-    FP64 oracle on a sample."""
+    two register-resident gains run back to back (their kernels need different
+    workgroup sizes); against the two batches evaluated by separate calls, with
+    per-grid errors against the FP64 oracle on a sample.  The one-launch form
+    (both grids' multi-pass gain items in one persistent launch) is timed
+    beside it."""
     import numpy as np
     import torch
     from dgfs_hip.solvers.dgfs.scattering import VHSOperator
@@ -440,6 +442,21 @@ def bench_pair(args, dev, stream):
         out[name] = {"value": (Pa + Pb) / w, "ms_per_step": 1e3 * w}
     out["value"] = out["pair"]["value"]
     out["algorithms"] = [opa.get_algorithm(), opb.get_algorithm()]
+    # the literal one-launch form: multi-pass gain items of both grids in one
+    # persistent launch
+    opa.set_algorithm(1)
+    opb.set_algorithm(1)
+    pair()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pair()
+    torch.cuda.synchronize()
+    w = time.perf_counter() - t0
+    out["one_launch_multipass"] = {"value": (Pa + Pb) / w, "ms_per_step": 1e3 * w}
+    opa.set_algorithm(0)
+    opb.set_algorithm(0)
+    pair()
+    torch.cuda.synchronize()
     Qa_p, Qb_p = Qa.clone(), Qb.clone()
     sep()
     torch.cuda.synchronize()

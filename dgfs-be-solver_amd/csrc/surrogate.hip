@@ -860,6 +860,56 @@ __global__ __launch_bounds__(256) void k_rows_copy_f(const float* __restrict__ s
     dst[static_cast<long>(i) * n + r] = src[static_cast<long>(idx[i]) * n + r];
 }
 
+// out[2i] = ||Q[idx[i]] - R[i]||^2, out[2i+1] = ||R[i]||^2 over the n
+// velocities of row i (one workgroup per row)
+__global__ __launch_bounds__(256) void k_row_err(const double* __restrict__ Q, const int* __restrict__ idx,
+                                                 const double* __restrict__ R, int nrows, int n,
+                                                 double* __restrict__ out) {
+  __shared__ double red[2][256];
+  const int i = blockIdx.x;
+  if (i >= nrows) return;
+  const double* q = Q + static_cast<long>(idx[i]) * n;
+  const double* r = R + static_cast<long>(i) * n;
+  double e = 0.0, m = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    const double d = q[k] - r[k];
+    e += d * d;
+    m += r[k] * r[k];
+  }
+  red[0][threadIdx.x] = e;
+  red[1][threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * i] = red[0][0];
+    out[2 * i + 1] = red[1][0];
+  }
+}
+
+// out[i] = ||Q[i]||^2 (one workgroup per row)
+__global__ __launch_bounds__(256) void k_row_norm2(const double* __restrict__ Q, int nrows, int n,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  const int i = blockIdx.x;
+  if (i >= nrows) return;
+  const double* q = Q + static_cast<long>(i) * n;
+  double m = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) m += q[k] * q[k];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -894,6 +944,10 @@ struct SurHandle {
   double* fb_Q = nullptr;
   size_t cap_fb = 0;
   double* mom_h = nullptr;  // pinned copy of the moments
+  // sampled spectral check: per-row squared errors and norms, and their
+  // pinned host copy ([3][cap_pts])
+  double* chk_d = nullptr;
+  double* chk_h = nullptr;
   ~SurHandle() {
     for (auto* p : Z) (void)hipFree(p);
     for (auto* p : U) (void)hipFree(p);
@@ -902,6 +956,8 @@ struct SurHandle {
     (void)hipFree(mom); (void)hipFree(xin); (void)hipFree(ltin); (void)hipFree(yout);
     (void)hipFree(xg); (void)hipFree(ltg); (void)hipFree(idx_d); (void)hipFree(fb_f); (void)hipFree(fb_Q);
     (void)hipHostFree(mom_h);
+    (void)hipFree(chk_d);
+    (void)hipHostFree(chk_h);
   }
 };
 
@@ -1083,7 +1139,9 @@ static int ensure_eval_ws(SurHandle* h, int npts) {
   if (static_cast<size_t>(npts) <= h->cap_pts) return DGFS_OK;
   (void)hipFree(h->mom); (void)hipFree(h->xin); (void)hipFree(h->ltin); (void)hipFree(h->yout);
   (void)hipFree(h->xg); (void)hipFree(h->ltg); (void)hipFree(h->idx_d); (void)hipHostFree(h->mom_h);
+  (void)hipFree(h->chk_d); (void)hipHostFree(h->chk_h);
   h->mom = nullptr; h->xin = h->ltin = h->yout = h->xg = h->ltg = nullptr; h->idx_d = nullptr; h->mom_h = nullptr;
+  h->chk_d = h->chk_h = nullptr;
   h->cap_pts = 0;
   DGFS_HIP_TRY(hipMalloc(&h->mom, static_cast<size_t>(npts) * 8 * sizeof(double)));
   DGFS_HIP_TRY(hipMalloc(&h->xin, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
@@ -1091,8 +1149,10 @@ static int ensure_eval_ws(SurHandle* h, int npts) {
   DGFS_HIP_TRY(hipMalloc(&h->yout, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
   DGFS_HIP_TRY(hipMalloc(&h->xg, static_cast<size_t>(npts) * kSurModes * sizeof(float)));
   DGFS_HIP_TRY(hipMalloc(&h->ltg, static_cast<size_t>(npts) * sizeof(float)));
-  DGFS_HIP_TRY(hipMalloc(&h->idx_d, static_cast<size_t>(npts) * 2 * sizeof(int)));
+  DGFS_HIP_TRY(hipMalloc(&h->idx_d, static_cast<size_t>(npts) * 4 * sizeof(int)));
   DGFS_HIP_TRY(hipHostMalloc(&h->mom_h, static_cast<size_t>(npts) * 8 * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->chk_d, static_cast<size_t>(npts) * 3 * sizeof(double)));
+  DGFS_HIP_TRY(hipHostMalloc(&h->chk_h, static_cast<size_t>(npts) * 3 * sizeof(double)));
   h->cap_pts = npts;
   return DGFS_OK;
 }
@@ -1223,6 +1283,115 @@ extern "C" int dgfs_sur_decode(void* p, const float* y, const double* mom, const
   return rc;
 }
 
+namespace dgfs {
+
+// checks shared by the evaluation entry points; the exact operator runs on
+// row buffers sized for this surrogate's grid
+static int check_eval_args(SurHandle* h, void* vhs, const float* params, const double* f, double* Q, int npts) {
+  if (!vhs || !params || !f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null operand");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
+  int vN = 0;
+  double vL = 0.0;
+  const int rc0 = dgfs_vhs_get_dims(vhs, &vN, nullptr, nullptr, nullptr, &vL);
+  if (rc0) return rc0;
+  if (vN != h->N || std::fabs(vL - h->L) > 1e-12 * h->L)
+    return fail(DGFS_ERR_INVALID_VALUE, "surrogate (N=%d, L=%g) and exact operator (N=%d, L=%g) grids differ",
+                h->N, h->L, vN, vL);
+  return DGFS_OK;
+}
+
+// encode every point and read the moments and indicators on the host
+static int encode_and_read(SurHandle* h, const double* f, int npts, hipStream_t st) {
+  int rc = ensure_eval_ws(h, npts);
+  if (rc) return rc;
+  EncArgs ea{f, h->mom, h->xin, h->ltin, h->L, h->cw, npts};
+  rc = do_encode(h, ea, st);
+  if (rc) return rc;
+  DGFS_HIP_TRY(hipMemcpyAsync(h->mom_h, h->mom, static_cast<size_t>(npts) * 8 * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+  DGFS_HIP_TRY(hipStreamSynchronize(st));
+  return DGFS_OK;
+}
+
+// row buffers for `rows` exact/projected rows, sized before any launch that
+// reads them (the stream is drained before a buffer is replaced)
+static int ensure_rows(SurHandle* h, int rows, hipStream_t st) {
+  const size_t need = static_cast<size_t>(std::max(rows, 1)) * h->n;
+  if (need <= h->cap_fb) return DGFS_OK;
+  DGFS_HIP_TRY(hipStreamSynchronize(st));
+  (void)hipFree(h->fb_f); (void)hipFree(h->fb_Q);
+  h->fb_f = h->fb_Q = nullptr;
+  h->cap_fb = 0;
+  DGFS_HIP_TRY(hipMalloc(&h->fb_f, need * sizeof(double)));
+  DGFS_HIP_TRY(hipMalloc(&h->fb_Q, need * sizeof(double)));
+  h->cap_fb = need;
+  return DGFS_OK;
+}
+
+static dim3 rows_grid(const SurHandle* h, int nrows) { return dim3(std::max(1, std::min(64, h->n / 256)), nrows); }
+
+// exact operator on the rows idx[0..nr) of f: results in fb_Q[0..nr), and
+// scattered into Q when scatter is set
+static int exact_rows(SurHandle* h, void* vhs, const double* f, double* Q, const int* idx, int nr, bool scatter,
+                      hipStream_t st) {
+  if (nr == 0) return DGFS_OK;
+  hipLaunchKernelGGL(k_rows_copy, rows_grid(h, nr), dim3(256), 0, st, f, h->fb_f, idx, nr, h->n, 1);
+  DGFS_LAUNCH_CHECK("rows_gather");
+  int rc = dgfs_vhs_eval_contig(vhs, h->fb_f, h->fb_Q, nr, st);
+  if (rc) return rc;
+  if (scatter) {
+    hipLaunchKernelGGL(k_rows_copy, rows_grid(h, nr), dim3(256), 0, st, h->fb_Q, Q, idx, nr, h->n, 0);
+    DGFS_LAUNCH_CHECK("rows_scatter");
+  }
+  return DGFS_OK;
+}
+
+// network, decoder and FP64 projection on the rows idx[0..ns) of Q (uses
+// fb_Q as the projection's row buffer)
+static int network_rows(SurHandle* h, const float* params, double* Q, const int* idx, int ns, bool bf16,
+                        hipStream_t st) {
+  if (ns == 0) return DGFS_OK;
+  for (int b0 = 0; b0 < ns; b0 += h->max_batch) {
+    const int nb = std::min(h->max_batch, ns - b0);
+    hipLaunchKernelGGL(k_rows_copy_f, dim3(2, nb), dim3(256), 0, st, h->xin, h->xg, idx + b0, nb, kSurModes);
+    DGFS_LAUNCH_CHECK("x_gather");
+    hipLaunchKernelGGL(k_rows_copy_f, dim3(1, nb), dim3(256), 0, st, h->ltin, h->ltg, idx + b0, nb, 1);
+    DGFS_LAUNCH_CHECK("lt_gather");
+    int rc = sur_forward(h, params, h->xg, h->ltg, nb, h->yout, bf16, st);
+    if (rc) return rc;
+    DecArgs da{h->yout, h->mom, idx + b0, Q, h->L, h->gamma, h->prefac, nb};
+    rc = do_decode(h, da, st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_rows_copy, rows_grid(h, ns), dim3(256), 0, st, Q, h->fb_Q, idx, ns, h->n, 1);
+  DGFS_LAUNCH_CHECK("rows_gather");
+  int rc = launch_invariant_projection(h->fb_Q, h->Ainv, h->L, h->cw, h->N, ns, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rows_copy, rows_grid(h, ns), dim3(256), 0, st, h->fb_Q, Q, idx, ns, h->n, 0);
+  DGFS_LAUNCH_CHECK("rows_scatter");
+  return DGFS_OK;
+}
+
+// indicator routing: points within both thresholds are surrogate candidates
+static void route(const SurHandle* h, int npts, double threshold, double threshold_hi, std::vector<int>& sur,
+                  std::vector<int>& fb) {
+  sur.clear();
+  fb.clear();
+  for (int i = 0; i < npts; ++i) {
+    const double ind = h->mom_h[static_cast<size_t>(i) * 8 + 5];
+    const double ind_hi = h->mom_h[static_cast<size_t>(i) * 8 + 6];
+    const bool ok = ind <= threshold && std::isfinite(ind) && (threshold_hi <= 0.0 || ind_hi <= threshold_hi);
+    (ok ? sur : fb).push_back(i);
+  }
+}
+
+static int upload_idx(int* dst, const std::vector<int>& v, hipStream_t st) {
+  if (!v.empty()) DGFS_HIP_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  return DGFS_OK;
+}
+
+}  // namespace dgfs
+
 // Surrogate with fallback: encode every point, evaluate the network on the
 // points whose indicator is <= threshold (and, if threshold_hi > 0, whose
 // high-mode indicator is <= threshold_hi), decode and project them; run the
@@ -1232,94 +1401,172 @@ extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const doub
                              double threshold, double threshold_hi, int bf16, int* nfallback, void* stream) {
   SurHandle* h = as_sur(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
-  if (!vhs || !params || !f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null operand");
-  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts must be >= 0");
-  {
-    // the exact operator runs on row buffers sized for this surrogate's grid
-    int vN = 0;
-    double vL = 0.0;
-    const int rc0 = dgfs_vhs_get_dims(vhs, &vN, nullptr, nullptr, nullptr, &vL);
-    if (rc0) return rc0;
-    if (vN != h->N || std::fabs(vL - h->L) > 1e-12 * h->L)
-      return fail(DGFS_ERR_INVALID_VALUE, "surrogate (N=%d, L=%g) and exact operator (N=%d, L=%g) grids differ",
-                  h->N, h->L, vN, vL);
-  }
+  int rc = check_eval_args(h, vhs, params, f, Q, npts);
+  if (rc) return rc;
   if (npts == 0) {
     if (nfallback) *nfallback = 0;
     return DGFS_OK;
   }
   auto st = static_cast<hipStream_t>(stream);
-  int rc = ensure_eval_ws(h, npts);
+  rc = encode_and_read(h, f, npts, st);
   if (rc) return rc;
-  EncArgs ea{f, h->mom, h->xin, h->ltin, h->L, h->cw, npts};
-  rc = do_encode(h, ea, st);
-  if (rc) return rc;
-  DGFS_HIP_TRY(hipMemcpyAsync(h->mom_h, h->mom, static_cast<size_t>(npts) * 8 * sizeof(double),
-                              hipMemcpyDeviceToHost, st));
-  DGFS_HIP_TRY(hipStreamSynchronize(st));
   std::vector<int> sur, fb;
-  sur.reserve(npts);
-  for (int i = 0; i < npts; ++i) {
-    const double ind = h->mom_h[static_cast<size_t>(i) * 8 + 5];
-    const double ind_hi = h->mom_h[static_cast<size_t>(i) * 8 + 6];
-    const bool ok = ind <= threshold && std::isfinite(ind) && (threshold_hi <= 0.0 || ind_hi <= threshold_hi);
-    if (ok) sur.push_back(i); else fb.push_back(i);
-  }
+  route(h, npts, threshold, threshold_hi, sur, fb);
   if (nfallback) *nfallback = static_cast<int>(fb.size());
   const int ns = static_cast<int>(sur.size()), nf = static_cast<int>(fb.size());
   int* idx_s = h->idx_d;
   int* idx_f = h->idx_d + npts;
-  if (ns) DGFS_HIP_TRY(hipMemcpyAsync(idx_s, sur.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
-  if (nf) DGFS_HIP_TRY(hipMemcpyAsync(idx_f, fb.data(), nf * sizeof(int), hipMemcpyHostToDevice, st));
-  // one row buffer for the fallback inputs/outputs and the projection of the
-  // surrogate rows, sized before any launch of this call
-  {
-    const size_t need = static_cast<size_t>(std::max(nf, ns)) * h->n;
-    if (need > h->cap_fb) {
-      DGFS_HIP_TRY(hipStreamSynchronize(st));
-      (void)hipFree(h->fb_f); (void)hipFree(h->fb_Q);
-      h->fb_f = h->fb_Q = nullptr;
-      h->cap_fb = 0;
-      DGFS_HIP_TRY(hipMalloc(&h->fb_f, need * sizeof(double)));
-      DGFS_HIP_TRY(hipMalloc(&h->fb_Q, need * sizeof(double)));
-      h->cap_fb = need;
+  rc = upload_idx(idx_s, sur, st);
+  if (!rc) rc = upload_idx(idx_f, fb, st);
+  if (!rc) rc = ensure_rows(h, std::max(nf, ns), st);
+  if (!rc) rc = exact_rows(h, vhs, f, Q, idx_f, nf, true, st);
+  if (!rc) rc = network_rows(h, params, Q, idx_s, ns, bf16 != 0, st);
+  return rc;
+}
+
+// Surrogate with fallback and a sampled spectral check.  After the indicator
+// routing, the surrogate candidates are ranked by indicator and split into
+// bins of equal count; a stratified sample of each bin is also evaluated by
+// the exact operator, which gives each bin's squared surrogate error per
+// point.  Whole bins, highest estimated error per point first, then fall back
+// until the estimated error of what the surrogate keeps is within
+// `safety * tol` of ||Q|| (relative L2 over the batch).  Sampled points keep
+// their exact Q.  Two host synchronisations: the indicators, the sample.
+extern "C" int dgfs_sur_eval_checked(void* p, void* vhs, const float* params, const double* f, double* Q,
+                                     int npts, double threshold, double threshold_hi, const dgfs_sur_check* chk,
+                                     int bf16, dgfs_sur_check_stats* stats, void* stream) {
+  SurHandle* h = as_sur(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid surrogate handle");
+  if (!chk) return fail(DGFS_ERR_INVALID_VALUE, "null check settings");
+  if (!(chk->tol > 0) || !(chk->safety > 0) || chk->safety > 1 || chk->nbins < 1 || chk->min_per_bin < 1 ||
+      !(chk->sample_frac >= 0) || chk->sample_frac > 1)
+    return fail(DGFS_ERR_INVALID_VALUE, "check settings: tol > 0, 0 < safety <= 1, nbins >= 1, min_per_bin >= 1, "
+                                        "0 <= sample_frac <= 1");
+  int rc = check_eval_args(h, vhs, params, f, Q, npts);
+  if (rc) return rc;
+  dgfs_sur_check_stats out{};
+  if (npts == 0) {
+    if (stats) *stats = out;
+    return DGFS_OK;
+  }
+  auto st = static_cast<hipStream_t>(stream);
+  rc = encode_and_read(h, f, npts, st);
+  if (rc) return rc;
+  std::vector<int> cand, fb;
+  route(h, npts, threshold, threshold_hi, cand, fb);
+  // rank the candidates by indicator, largest first, and cut equal bins
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    return h->mom_h[static_cast<size_t>(a) * 8 + 5] > h->mom_h[static_cast<size_t>(b) * 8 + 5];
+  });
+  const int nc = static_cast<int>(cand.size());
+  const int nbins = std::max(1, std::min(chk->nbins, nc));
+  std::vector<int> bin_lo(nbins + 1);
+  for (int b = 0; b <= nbins; ++b) bin_lo[b] = static_cast<int>((static_cast<long>(nc) * b) / nbins);
+  // stratified sample: s_b evenly spaced ranks in each bin, offset by a
+  // seeded fraction of the spacing
+  std::vector<int> samp, samp_bin;
+  std::vector<char> in_sample(nc, 0);
+  unsigned long long x = chk->seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  auto unif = [&]() {  // splitmix64 -> [0, 1)
+    x += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int b = 0; b < nbins; ++b) {
+    const int nb = bin_lo[b + 1] - bin_lo[b];
+    if (nb == 0) continue;
+    const int sb = std::min(nb, std::max(chk->min_per_bin, static_cast<int>(std::ceil(chk->sample_frac * nb))));
+    const double u = unif();
+    for (int j = 0; j < sb; ++j) {
+      const int r = bin_lo[b] + std::min(nb - 1, static_cast<int>((j + u) * nb / sb));
+      if (in_sample[r]) continue;
+      in_sample[r] = 1;
+      samp.push_back(cand[r]);
+      samp_bin.push_back(b);
     }
   }
-  // exact operator on the fallback rows
-  if (nf) {
-    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), nf), dim3(256), 0, st, f, h->fb_f,
-                       idx_f, nf, h->n, 1);
-    DGFS_LAUNCH_CHECK("rows_gather");
-    rc = dgfs_vhs_eval_contig(vhs, h->fb_f, h->fb_Q, nf, stream);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), nf), dim3(256), 0, st, h->fb_Q, Q,
-                       idx_f, nf, h->n, 0);
+  const int nsamp = static_cast<int>(samp.size()), nf = static_cast<int>(fb.size());
+  // device index lists: [candidates | fallback + sample | sample | rejected]
+  int* idx_c = h->idx_d;
+  int* idx_e = h->idx_d + npts;
+  int* idx_s = h->idx_d + 2 * npts;
+  int* idx_r = h->idx_d + 3 * npts;
+  std::vector<int> exact(fb);
+  exact.insert(exact.end(), samp.begin(), samp.end());
+  const int ne = static_cast<int>(exact.size());
+  rc = upload_idx(idx_c, cand, st);
+  if (!rc) rc = upload_idx(idx_e, exact, st);
+  if (!rc) rc = upload_idx(idx_s, samp, st);
+  if (!rc) rc = ensure_rows(h, std::max(ne, nc), st);
+  // the network on every candidate (sampled ones too, for the comparison)
+  if (!rc) rc = network_rows(h, params, Q, idx_c, nc, bf16 != 0, st);
+  // exact Q of the fallback and sampled rows (into fb_Q), the sampled rows'
+  // surrogate errors, then the exact rows into Q
+  if (!rc) rc = exact_rows(h, vhs, f, Q, idx_e, ne, false, st);
+  if (rc) return rc;
+  if (nsamp) {
+    hipLaunchKernelGGL(k_row_err, dim3(nsamp), dim3(256), 0, st, Q, idx_s, h->fb_Q + static_cast<size_t>(nf) * h->n,
+                       nsamp, h->n, h->chk_d);
+    DGFS_LAUNCH_CHECK("row_err");
+  }
+  if (ne) {
+    hipLaunchKernelGGL(k_rows_copy, rows_grid(h, ne), dim3(256), 0, st, h->fb_Q, Q, idx_e, ne, h->n, 0);
     DGFS_LAUNCH_CHECK("rows_scatter");
   }
-  // network on the surrogate rows, in batches of max_batch
-  for (int b0 = 0; b0 < ns; b0 += h->max_batch) {
-    const int nb = std::min(h->max_batch, ns - b0);
-    hipLaunchKernelGGL(k_rows_copy_f, dim3(2, nb), dim3(256), 0, st, h->xin, h->xg, idx_s + b0, nb, kSurModes);
-    DGFS_LAUNCH_CHECK("x_gather");
-    hipLaunchKernelGGL(k_rows_copy_f, dim3(1, nb), dim3(256), 0, st, h->ltin, h->ltg, idx_s + b0, nb, 1);
-    DGFS_LAUNCH_CHECK("lt_gather");
-    rc = sur_forward(h, params, h->xg, h->ltg, nb, h->yout, bf16 != 0, st);
-    if (rc) return rc;
-    DecArgs da{h->yout, h->mom, idx_s + b0, Q, h->L, h->gamma, h->prefac, nb};
-    rc = do_decode(h, da, st);
+  hipLaunchKernelGGL(k_row_norm2, dim3(npts), dim3(256), 0, st, Q, npts, h->n, h->chk_d + 2 * static_cast<size_t>(npts));
+  DGFS_LAUNCH_CHECK("row_norm2");
+  DGFS_HIP_TRY(hipMemcpyAsync(h->chk_h, h->chk_d, static_cast<size_t>(npts) * 3 * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+  DGFS_HIP_TRY(hipStreamSynchronize(st));
+  // per-bin error estimate for the unsampled candidates
+  double qnorm2 = 0.0;
+  for (int i = 0; i < npts; ++i) qnorm2 += h->chk_h[2 * static_cast<size_t>(npts) + i];
+  std::vector<double> err_b(nbins, 0.0), cnt_s(nbins, 0.0);
+  for (int s = 0; s < nsamp; ++s) {
+    err_b[samp_bin[s]] += h->chk_h[2 * s];
+    cnt_s[samp_bin[s]] += 1.0;
+  }
+  std::vector<double> est(nbins, 0.0), per_pt(nbins, 0.0);
+  double kept = 0.0;
+  for (int b = 0; b < nbins; ++b) {
+    const double nb = bin_lo[b + 1] - bin_lo[b];
+    if (cnt_s[b] > 0) {
+      per_pt[b] = err_b[b] / cnt_s[b];
+      est[b] = per_pt[b] * (nb - cnt_s[b]);
+    }
+    kept += est[b];
+  }
+  // drop bins, worst error per point first, until the kept estimate fits
+  const double budget = (chk->safety * chk->tol) * (chk->safety * chk->tol) * qnorm2;
+  std::vector<int> order(nbins);
+  for (int b = 0; b < nbins; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per_pt[a] > per_pt[b]; });
+  std::vector<char> reject(nbins, 0);
+  for (int k = 0; k < nbins && kept > budget; ++k) {
+    reject[order[k]] = 1;
+    kept -= est[order[k]];
+    out.nrejected_bins += 1;
+  }
+  std::vector<int> rej;
+  for (int b = 0; b < nbins; ++b)
+    if (reject[b])
+      for (int r = bin_lo[b]; r < bin_lo[b + 1]; ++r)
+        if (!in_sample[r]) rej.push_back(cand[r]);
+  const int nr = static_cast<int>(rej.size());
+  if (nr) {
+    rc = upload_idx(idx_r, rej, st);
+    if (!rc) rc = ensure_rows(h, nr, st);
+    if (!rc) rc = exact_rows(h, vhs, f, Q, idx_r, nr, true, st);
     if (rc) return rc;
   }
-  // FP64 projection of the surrogate rows (gather, project, scatter)
-  if (ns) {
-    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), ns), dim3(256), 0, st, Q, h->fb_Q,
-                       idx_s, ns, h->n, 1);
-    DGFS_LAUNCH_CHECK("rows_gather");
-    rc = launch_invariant_projection(h->fb_Q, h->Ainv, h->L, h->cw, h->N, ns, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_rows_copy, dim3(std::max(1, std::min(64, h->n / 256)), ns), dim3(256), 0, st, h->fb_Q, Q,
-                       idx_s, ns, h->n, 0);
-    DGFS_LAUNCH_CHECK("rows_scatter");
-  }
+  out.nfallback = nf + nr;
+  out.nsampled = nsamp;
+  out.nsurrogate = nc - nsamp - nr;
+  out.est_rel_err = qnorm2 > 0 ? std::sqrt(std::max(kept, 0.0) / qnorm2) : 0.0;
+  if (stats) *stats = out;
   return DGFS_OK;
 }
 

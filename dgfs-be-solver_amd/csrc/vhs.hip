@@ -1093,13 +1093,6 @@ struct VhsHandle {
   size_t cap_ft = 0;
   size_t cap_wny = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
-  // cap on the register-resident gain grid (0: every CU); set while a paired
-  // call splits the CUs between two grids
-  int rr_grid_cap = 0;
-  // second stream and fork/join events of a paired register-resident call
-  // (created on first use, default priority)
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // optional per-kernel timing: event pairs recorded around every launch on
   // the launch stream, resolved (and synchronised) only when queried
   int timing = 0;
@@ -1225,8 +1218,7 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
                           : reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, false>);
     DGFS_HIP_TRY(hipFuncSetAttribute(kfn,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    const int cap = h->rr_grid_cap > 0 ? std::min(h->rr_grid_cap, cu_count()) : cu_count();
-    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cap)));
+    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cu_count())));
     GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
                      static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
                      static_cast<const cpx<T>*>(h->ephtab), static_cast<int>(items), groups, ppg, h->Nrho, h->M};
@@ -1274,8 +1266,7 @@ static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t item
     int occ = 1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, RR1Shape<N>::NT, lds) != hipSuccess || occ < 1)
       occ = 1;
-    const int cap = h->rr_grid_cap > 0 ? std::min(h->rr_grid_cap, occ * cu_count()) : occ * cu_count();
-    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cap)));
+    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(occ) * cu_count()));
     GainRRArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
                      static_cast<cpx<T>*>(h->Ghat), static_cast<const T*>(h->b1), h->alpha, h->wgt, h->szd,
                      static_cast<const cpx<T>*>(h->ephtab), static_cast<int>(items), groups, ppg, h->Nrho, h->M};
@@ -1496,52 +1487,22 @@ static int run_pair(VhsHandle* ha, const double* fa, double* Qa, int na, VhsHand
   return rc;
 }
 
-// Two register-resident grids side by side: each handle runs its own
-// pipeline (prep, Nyquist planes, gain, finalize, projection), the first on
-// the caller's stream and the second on the second handle's side stream,
-// forked from and joined back into the caller's stream by events (so a
-// stream capture of the caller's stream records both).  The persistent gain
-// grids split the CUs in proportion to the two batches' work (n log2 n K per
-// point, the one-pencil kernel weighted 1.2x, as measured at 24^3 against
-// 32^3), so both gains run at once on disjoint CUs; each handle's items and
-// partial sums are those of a single call, so Q is bitwise the same.
-static double rr_point_cost(const VhsHandle* h) {
-  const double n = static_cast<double>(h->n);
-  return n * std::log2(n) * h->Nrho * h->M * (rr_kind(h) == 4 ? 1.2 : 1.0);
-}
-
+// Two register-resident grids: the two pipelines (prep, Nyquist planes, gain,
+// finalize, projection) run back to back on the caller's stream, each gain
+// grid filling the chip.  The two kernels need different workgroup sizes (512
+// threads at 32^3, 576 at 24^3), so they cannot share one launch; run side by
+// side on a CU split over a second stream they measured 13% slower than back
+// to back on MI355X (90.4 against 79.9 ms for 4096 points at 32^3 plus 1024
+// at 24^3), because both are bound by the CU's VALU and LDS, not by launch
+// gaps.  Q is bitwise that of two separate calls.
 static int eval_pair_rr(VhsHandle* ha, const double* fa, double* Qa, int na, VhsHandle* hb, const double* fb,
                         double* Qb, int nb, hipStream_t st) {
-  if (!hb->side) DGFS_HIP_TRY(hipStreamCreateWithFlags(&hb->side, hipStreamNonBlocking));
-  if (!hb->ev_fork) DGFS_HIP_TRY(hipEventCreateWithFlags(&hb->ev_fork, hipEventDisableTiming));
-  if (!hb->ev_join) DGFS_HIP_TRY(hipEventCreateWithFlags(&hb->ev_join, hipEventDisableTiming));
-  // both grids are whole multiples of the XCD count, so that each XCD holds
-  // its share of both and no workgroup waits for a CU behind a persistent one
-  const int cus = cu_count();
-  const int step = (cus % kXcds == 0 && cus >= 2 * kXcds) ? kXcds : 1;
-  const double wa = na * rr_point_cost(ha), wb = nb * rr_point_cost(hb);
-  int cb = step * static_cast<int>(std::lround(cus * wb / (wa + wb) / step));
-  cb = std::max(step, std::min(cus - step, cb));
-  ha->rr_grid_cap = cus - cb;
-  hb->rr_grid_cap = cb;
-  DGFS_HIP_TRY(hipEventRecord(hb->ev_fork, st));
-  DGFS_HIP_TRY(hipStreamWaitEvent(hb->side, hb->ev_fork, 0));
-  // the side stream's work is queued first so its gain grid claims its CUs
-  // while the caller's stream still runs the first batch's prep
-  int rc = (hb->prec == 64) ? dispatch<double>(hb, fb, Qb, false, 0, 0, 0, 0, 0, nb, hb->side)
-                            : dispatch<float>(hb, fb, Qb, false, 0, 0, 0, 0, 0, nb, hb->side);
-  int rc2 = DGFS_OK;
-  if (!rc)
-    rc2 = (ha->prec == 64) ? dispatch<double>(ha, fa, Qa, false, 0, 0, 0, 0, 0, na, st)
-                           : dispatch<float>(ha, fa, Qa, false, 0, 0, 0, 0, 0, na, st);
-  // join even after an error, so the caller's stream never runs ahead of
-  // work queued on the side stream
-  const hipError_t e1 = hipEventRecord(hb->ev_join, hb->side);
-  const hipError_t e2 = hipStreamWaitEvent(st, hb->ev_join, 0);
-  ha->rr_grid_cap = hb->rr_grid_cap = 0;
+  int rc = (ha->prec == 64) ? dispatch<double>(ha, fa, Qa, false, 0, 0, 0, 0, 0, na, st)
+                            : dispatch<float>(ha, fa, Qa, false, 0, 0, 0, 0, 0, na, st);
   if (rc) return rc;
-  if (rc2) return rc2;
-  if (e1 != hipSuccess || e2 != hipSuccess) return fail(DGFS_ERR_HIP, "pair join: %s", hipGetErrorString(e1 ? e1 : e2));
+  rc = (hb->prec == 64) ? dispatch<double>(hb, fb, Qb, false, 0, 0, 0, 0, 0, nb, st)
+                        : dispatch<float>(hb, fb, Qb, false, 0, 0, 0, 0, 0, nb, st);
+  if (rc) return rc;
   ha->last_algorithm = rr_kind(ha);
   hb->last_algorithm = rr_kind(hb);
   return DGFS_OK;
@@ -1678,9 +1639,6 @@ extern "C" int dgfs_vhs_destroy(void* p) {
   for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
   h->pool.clear();
   free_ws(h);
-  if (h->side) (void)hipStreamDestroy(h->side);
-  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   (void)hipFree(h->b1); (void)hipFree(h->b2); (void)hipFree(h->alpha); (void)hipFree(h->wgt);
   (void)hipFree(h->szd);
   (void)hipFree(h->ephtab);

@@ -16,6 +16,18 @@ import numpy as np
 from dgfs_hip import _lib
 
 _P, _I, _L, _F, _D = C.c_void_p, C.c_int, C.c_long, C.c_float, C.c_double
+
+
+class dgfs_sur_check(C.Structure):
+    _fields_ = [("tol", C.c_double), ("safety", C.c_double), ("sample_frac", C.c_double),
+                ("nbins", C.c_int), ("min_per_bin", C.c_int), ("seed", C.c_ulonglong)]
+
+
+class dgfs_sur_check_stats(C.Structure):
+    _fields_ = [("nfallback", C.c_int), ("nsampled", C.c_int), ("nsurrogate", C.c_int),
+                ("nrejected_bins", C.c_int), ("est_rel_err", C.c_double)]
+
+
 _lib.register({
     "dgfs_sur_create": [C.POINTER(_lib.dgfs_sur_desc), C.POINTER(C.c_void_p)],
     "dgfs_sur_destroy": [_P],
@@ -26,6 +38,8 @@ _lib.register({
     "dgfs_sur_targets": [_P, _P, _P, _I, _P, _P],
     "dgfs_sur_decode": [_P, _P, _P, _P, _I, _P, _I, _P],
     "dgfs_sur_eval": [_P, _P, _P, _P, _P, _I, _D, _D, _I, C.POINTER(_I), _P],
+    "dgfs_sur_eval_checked": [_P, _P, _P, _P, _P, _I, _D, _D, C.POINTER(dgfs_sur_check), _I,
+                              C.POINTER(dgfs_sur_check_stats), _P],
     "dgfs_nn_gemm": [_I, _I, _I, _I, _I, _F, _P, _L, _P, _L, _F, _P, _L, _I, _P],
     "dgfs_sur_param_count": [],
     "dgfs_sur_round_bf16": [_L, _P, _P, _P],
@@ -238,6 +252,26 @@ class SpectralSurrogate:
                   float(threshold), float(threshold_hi), int(bool(bf16)), C.byref(nfb),
                   stream if stream is not None else self._stream())
         return nfb.value
+
+    def eval_checked(self, vhs_op, f, Q, npts, tol=1e-3, threshold=0.05, threshold_hi=1e-3, sample_frac=0.03,
+                     nbins=16, min_per_bin=4, safety=0.7, seed=0, bf16=False, stream=None):
+        """eval_with_fallback plus the sampled spectral check: a stratified
+        sample of the surrogate candidates (ranked by indicator, cut into
+        nbins bins) is evaluated exactly; bins whose sampled error would
+        carry the batch's relative L2 error past safety * tol fall back to
+        the exact operator.  Returns the statistics as a dict."""
+        Nv, _, _, _, L = vhs_op.get_dims()
+        if Nv != self.Nv or abs(L - self.L) > 1e-12 * self.L:
+            raise ValueError(f"surrogate grid (N={self.Nv}, L={self.L}) differs from the exact operator's "
+                             f"(N={Nv}, L={L})")
+        chk = dgfs_sur_check(tol=float(tol), safety=float(safety), sample_frac=float(sample_frac),
+                             nbins=int(nbins), min_per_bin=int(min_per_bin), seed=int(seed))
+        st = dgfs_sur_check_stats()
+        _lib.call("dgfs_sur_eval_checked", self._h, vhs_op._h, _ptr(self.params_bf16), _ptr(f), _ptr(Q), int(npts),
+                  float(threshold), float(threshold_hi), C.byref(chk), int(bool(bf16)), C.byref(st),
+                  stream if stream is not None else self._stream())
+        return {"fallback_points": st.nfallback, "sampled_points": st.nsampled, "surrogate_points": st.nsurrogate,
+                "rejected_bins": st.nrejected_bins, "estimated_rel_l2": st.est_rel_err}
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -79,14 +79,14 @@ int dgfs_vhs_eval(void* handle, const double* f, double* Q, int nupts, int ldim,
 int dgfs_vhs_eval_contig(void* handle, const double* f, double* Q, int npts, void* stream);
 
 /* Two independent contiguous batches on two velocity grids (32+24, 32+16 or
- * 24+16 points per axis, same precision) evaluated together.  When both
- * handles take a register-resident gain (FP32), the two pipelines run on the
- * caller's stream and a second stream of handle_b, forked and joined by
- * events, with the two persistent gain grids splitting the CUs in proportion
- * to their work; Q equals two separate calls bitwise.  Otherwise the gain
- * work items of both grids share one multi-pass persistent launch.  Batches
- * beyond one workspace chunk run as separate launches.  No reference
- * counterpart (the reference has one velocity grid per run). */
+ * 24+16 points per axis, same precision) evaluated together.  With the
+ * multi-pass gain (FP64, or when selected) the gain work items of both grids
+ * share one persistent launch.  When both handles take a register-resident
+ * gain (FP32), whose kernels need different workgroup sizes, the two
+ * pipelines run back to back on the stream, each gain grid filling the chip;
+ * Q then equals two separate calls bitwise.  Batches beyond one workspace
+ * chunk run as separate launches.  No reference counterpart (the reference
+ * has one velocity grid per run). */
 int dgfs_vhs_eval_pair(void* handle_a, const double* f_a, double* Q_a, int npts_a, void* handle_b,
                        const double* f_b, double* Q_b, int npts_b, void* stream);
 
@@ -274,6 +274,36 @@ int dgfs_sur_decode(void* handle, const float* y, const double* mom, const int* 
  * synchronises the stream once to read the indicators. */
 int dgfs_sur_eval(void* handle, void* vhs, const float* params, const double* f, double* Q, int npts,
                   double threshold, double threshold_hi, int bf16, int* nfallback, void* stream);
+
+/* Settings of the sampled spectral check. */
+typedef struct dgfs_sur_check {
+  double tol;          /* target relative L2 error of the batch's Q */
+  double safety;       /* the kept estimate must be within (safety * tol)^2 ||Q||^2; 0 < safety <= 1 */
+  double sample_frac;  /* share of each bin's candidates evaluated exactly as the sample */
+  int nbins;           /* indicator-ranked bins of the surrogate candidates */
+  int min_per_bin;     /* smallest sample of a bin */
+  unsigned long long seed;  /* offset of the stratified sample */
+} dgfs_sur_check;
+
+typedef struct dgfs_sur_check_stats {
+  int nfallback;       /* points given the exact operator for cause (indicator or rejected bin) */
+  int nsampled;        /* points of the sample (exact Q kept) */
+  int nsurrogate;      /* points whose Q is the network's */
+  int nrejected_bins;  /* bins sent to the exact operator by the check */
+  double est_rel_err;  /* estimated relative L2 error of the kept surrogate points */
+} dgfs_sur_check_stats;
+
+/* dgfs_sur_eval with a sampled spectral check.  The surrogate candidates of
+ * the indicator routing are ranked by indicator and cut into nbins bins of
+ * equal count; a stratified sample of every bin is also evaluated by the
+ * exact operator, which estimates each bin's squared surrogate error per
+ * point; whole bins, worst per-point error first, then fall back to the
+ * exact operator until the estimated error of the kept surrogate points is
+ * within safety * tol of ||Q||.  Sampled points keep their exact Q.
+ * Synchronises the stream twice (indicators, sample). */
+int dgfs_sur_eval_checked(void* handle, void* vhs, const float* params, const double* f, double* Q,
+                          int npts, double threshold, double threshold_hi, const dgfs_sur_check* check,
+                          int bf16, dgfs_sur_check_stats* stats, void* stream);
 /* dst[i] = src[i] rounded to BF16 (nearest even), stored as FP32: the BF16
  * weights the network runs with in its FP32-operand mode. */
 int dgfs_sur_round_bf16(long n, const float* src, float* dst, void* stream);

@@ -275,3 +275,64 @@ def test_fallback_rejects_an_operator_of_another_grid(device):
                       C.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     assert bool(torch.all(Q == 7.0))
+
+
+def test_eval_checked_sampled_spectral_check(device):
+    """The sampled spectral check (dgfs_sur_eval_checked): with a target error
+    the untrained network cannot meet, every bin falls back and Q is the exact
+    operator's; with a loose target no bin falls back, the sampled rows carry
+    the exact Q and the rest the network's; the counts add up and the
+    estimate stays within the budget."""
+    import torch
+    N, L, Nrho, M, gamma = 16, 6.0, 8, 6, 0.38
+    P = 96
+    op = _exact_op(N, L, Nrho, M, gamma)
+    sur = _sur(device, Nv=N, L=L, max_batch=32)
+    f = torch.as_tensor(near_equilibrium_states(N, L, P, seed=12, eps_max=0.3), device=device)
+    st = torch.cuda.current_stream().cuda_stream
+    Qx = torch.empty_like(f)
+    op.eval_contig(f.data_ptr(), Qx.data_ptr(), P, st)
+    x, lt, y, mom = sur.make_dataset(f, Qx)
+    Qc = torch.empty_like(f)
+    sur.decode(sur.predict(x, lt), mom, P, Qc, project=True)
+    torch.cuda.synchronize()
+    Qx_h, Qc_h = Qx.cpu().numpy(), Qc.cpu().numpy()
+    # tight target: every bin goes to the exact operator
+    Qa = torch.full_like(f, float("nan"))
+    s = sur.eval_checked(op, f, Qa, P, tol=1e-9, threshold=1e9, threshold_hi=0.0, nbins=8, min_per_bin=2,
+                         sample_frac=0.05, seed=3)
+    torch.cuda.synchronize()
+    say("tight check", s)
+    assert s["surrogate_points"] == 0 and s["rejected_bins"] == 8
+    assert s["fallback_points"] + s["sampled_points"] == P
+    assert rel(Qa.cpu().numpy(), Qx_h) < 1e-12
+    # loose target: nothing falls back; sampled rows exact, the rest the network's
+    Qb = torch.full_like(f, float("nan"))
+    s = sur.eval_checked(op, f, Qb, P, tol=10.0, threshold=1e9, threshold_hi=0.0, nbins=8, min_per_bin=2,
+                         sample_frac=0.05, seed=3)
+    torch.cuda.synchronize()
+    say("loose check", s)
+    Qb_h = Qb.cpu().numpy()
+    assert s["rejected_bins"] == 0 and s["fallback_points"] == 0
+    assert s["sampled_points"] == 16 and s["surrogate_points"] == P - 16
+    assert 0.0 < s["estimated_rel_l2"] <= 0.7 * 10.0
+    ex = np.array([rel(Qb_h[i], Qx_h[i]) < 1e-12 for i in range(P)])
+    nn = np.array([rel(Qb_h[i], Qc_h[i]) < 1e-12 for i in range(P)])
+    assert int(ex.sum()) == s["sampled_points"]
+    assert np.all(ex | nn)
+    # the estimate of the kept error tracks the measured error of the kept rows
+    kept = ~ex
+    meas = float(np.linalg.norm(Qb_h[kept] - Qx_h[kept]) / np.linalg.norm(Qb_h))
+    say("estimated", s["estimated_rel_l2"], "measured", meas)
+    assert 0.3 * meas < s["estimated_rel_l2"] < 3.0 * meas
+    # indicator fallback still applies in front of the check
+    ind = mom[:, 5].cpu().numpy()
+    thr = float(np.median(ind))
+    Qm = torch.full_like(f, float("nan"))
+    s = sur.eval_checked(op, f, Qm, P, tol=10.0, threshold=thr, threshold_hi=0.0, nbins=4, min_per_bin=1,
+                         sample_frac=0.0, seed=1)
+    torch.cuda.synchronize()
+    hi = ind > thr
+    assert s["fallback_points"] == int(hi.sum()) and s["sampled_points"] == 4
+    Qm_h = Qm.cpu().numpy()
+    assert all(rel(Qm_h[i], Qx_h[i]) < 1e-12 for i in np.where(hi)[0])

@@ -238,7 +238,7 @@ def test_pair_launch_matches_separate(precision, device):
 def test_pair_register_resident_c3_grids(device):
     """The C3 shapes (32^3 on [-6, 6]^3 and a 24^3 batch, M = 12, Nrho = 16,
     FP32 with FP64 projection) in one paired call: both register-resident
-    gains run side by side on split CUs.  Q equals the separate calls bitwise
+    gains run back to back.  Q equals the separate calls bitwise
     (same work items and summation order), and each grid meets the FP32 bar
     against the FP64 oracle with the same projection."""
     import torch
