@@ -35,6 +35,7 @@
 #include <type_traits>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -1093,6 +1094,14 @@ struct VhsHandle {
   size_t cap_ft = 0;
   size_t cap_wny = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
+  // set for the duration of an allocation-free call: the workspace pointers
+  // above then point into the device scratch array and growing any of them
+  // is an error instead of a hipMalloc
+  int no_alloc = 0;
+  // this device's scratch array address (resolved at create) or null, with
+  // the failure text when it could not be resolved
+  void* scratch = nullptr;
+  std::string scratch_err;
   // optional per-kernel timing: event pairs recorded around every launch on
   // the launch stream, resolved (and synchronised) only when queried
   int timing = 0;
@@ -1195,6 +1204,7 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
     h->last_algorithm = inl ? 3 : 2;
     const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
     if (!inl && wbytes > h->cap_wny) {
+      if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: Nyquist planes exceed their scratch share");
       (void)hipFree(h->Wny);
       h->Wny = nullptr;
       DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
@@ -1245,6 +1255,7 @@ static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t item
     h->last_algorithm = 4;
     const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
     if (wbytes > h->cap_wny) {
+      if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: Nyquist planes exceed their scratch share");
       (void)hipFree(h->Wny);
       h->Wny = nullptr;
       DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
@@ -1299,6 +1310,7 @@ static int stage_prep(VhsHandle* h, const double* fc, int npts, int groups, int 
   const int ppg = (h->Nrho + groups - 1) / groups;
   const size_t items = static_cast<size_t>(npts) * groups;
   if (items > h->cap_items) {
+    if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: gain partials exceed their scratch share");
     (void)hipFree(h->Ghat);
     h->Ghat = nullptr;
     h->cap_items = 0;
@@ -1311,6 +1323,7 @@ static int stage_prep(VhsHandle* h, const double* fc, int npts, int groups, int 
   if (rr) {
     const size_t fbytes = static_cast<size_t>(npts) * n * sizeof(cpx<T>);
     if (fbytes > h->cap_ft) {
+      if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: transposed spectra exceed their scratch share");
       (void)hipFree(h->Ft);
       h->Ft = nullptr;
       h->cap_ft = 0;
@@ -1382,6 +1395,8 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
 template <typename T, int N>
 static int ensure_ws(VhsHandle* h, int pts) {
   constexpr int n = N * N * N;
+  if (h->no_alloc && (h->cap_slots == 0 || static_cast<size_t>(pts) > h->cap_pts))
+    return fail(DGFS_ERR_ALLOC, "allocation-free call: point buffers exceed their scratch share");
   if (h->cap_slots == 0) {
     const int slots = occupancy_slots<T, N>();
     DGFS_HIP_TRY(hipMalloc(&h->Z, static_cast<size_t>(slots) * n * sizeof(cpx<T>)));
@@ -1462,6 +1477,184 @@ static int dispatch(VhsHandle* h, const double* f, double* Q, bool aosoa, int nu
 
 // two handles (velocity grids NA > NB, same precision) in one persistent gain
 // launch; both take the multi-pass gain
+// ---------------------------------------------------------------------------
+// allocation-free evaluation
+// ---------------------------------------------------------------------------
+// Callers that cannot allocate device memory (code under stream capture,
+// serving loops, harnesses that forbid hipMalloc) evaluate Q through
+// dgfs_vhs_eval_contig_noalloc, whose workspace is carved out of this
+// file-scope array.  Each GPU the library runs on holds its own copy, at its
+// own address, until the process ends; host code never passes the array's
+// name to a kernel, only the device address that hipGetSymbolAddress returns
+// for the current device.
+constexpr size_t kScratchBytes = static_cast<size_t>(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// address of the current device's scratch array, resolved once per device
+static int resolve_scratch(void** out) {
+  *out = nullptr;
+  int dev = 0;
+  DGFS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DGFS_ERR_UNSUPPORTED, "device ordinal %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    DGFS_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(DGFS_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DGFS_OK;
+}
+
+constexpr size_t kScratchAlign = 256;
+static size_t align_up(size_t b) { return (b + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+
+// one chunk's carve-out of the scratch array
+struct ScratchPlan {
+  int pts = 0, groups = 0, slots = 0;
+  size_t items = 0;  // gain partial buffers the Ghat share holds
+  size_t z = 0, s = 0, w = 0, F = 0, Ghat = 0, Ft = 0, Wny = 0;
+  size_t total() const { return z + s + w + F + Ghat + Ft + Wny; }
+};
+
+template <typename T, int N>
+static ScratchPlan plan_scratch(const VhsHandle* h, int pts, int slots) {
+  constexpr size_t n = static_cast<size_t>(N) * N * N;
+  const bool rr = use_rr(h);
+  int gain_slots = rr ? cu_count() : slots;
+  if (!rr && h->slots_override > 0) gain_slots = std::min(gain_slots, h->slots_override);
+  ScratchPlan p;
+  p.pts = pts;
+  p.slots = slots;
+  p.groups = pick_groups(h, pts, gain_slots);
+  p.z = align_up(static_cast<size_t>(slots) * n * sizeof(cpx<T>));
+  if (!rr) {  // the multi-pass gain's per-slot field and Nyquist planes
+    p.s = align_up(static_cast<size_t>(slots) * n * sizeof(T));
+    p.w = align_up(static_cast<size_t>(slots) * 3 * N * N * sizeof(cpx<T>));
+  }
+  p.F = align_up(static_cast<size_t>(pts) * n * sizeof(cpx<T>));
+  p.items = static_cast<size_t>(pts) * p.groups;
+  p.Ghat = align_up(p.items * n * sizeof(cpx<T>));
+  if (rr) {
+    p.Ft = p.F;
+    if (rr_kind(h) != 3)
+      p.Wny = align_up(static_cast<size_t>(pts) * h->Nrho * h->M * 3 * N * N * sizeof(cpx<T>));
+  }
+  return p;
+}
+
+// the handle's own workspace, set aside while a call runs on the scratch
+// array and put back however the call ends
+struct NoAllocScope {
+  VhsHandle* h;
+  void *F, *Ghat, *Z, *S, *W, *Wny, *Ft;
+  double *fc, *Qc;
+  size_t cap_pts, cap_items, cap_ft, cap_wny;
+  int cap_slots, timing;
+  explicit NoAllocScope(VhsHandle* hh)
+      : h(hh), F(hh->F), Ghat(hh->Ghat), Z(hh->Z), S(hh->S), W(hh->W), Wny(hh->Wny), Ft(hh->Ft), fc(hh->fc),
+        Qc(hh->Qc), cap_pts(hh->cap_pts), cap_items(hh->cap_items), cap_ft(hh->cap_ft), cap_wny(hh->cap_wny),
+        cap_slots(hh->cap_slots), timing(hh->timing) {
+    h->no_alloc = 1;
+    h->timing = 0;  // the event pool may grow; no timing in this mode
+  }
+  void use(char* base, const ScratchPlan& p) {
+    char* at = base;
+    auto take = [&](size_t b) -> void* {
+      void* r = b ? static_cast<void*>(at) : nullptr;
+      at += b;
+      return r;
+    };
+    h->Z = take(p.z);
+    h->S = take(p.s);
+    h->W = take(p.w);
+    h->F = take(p.F);
+    h->Ghat = take(p.Ghat);
+    h->Ft = take(p.Ft);
+    h->Wny = take(p.Wny);
+    h->fc = h->Qc = nullptr;  // the contiguous form reads f and writes Q in place
+    h->cap_slots = p.slots;
+    h->cap_pts = static_cast<size_t>(p.pts);
+    h->cap_items = p.items;
+    h->cap_ft = p.Ft;
+    h->cap_wny = p.Wny;
+  }
+  ~NoAllocScope() {
+    h->F = F; h->Ghat = Ghat; h->Z = Z; h->S = S; h->W = W; h->Wny = Wny; h->Ft = Ft;
+    h->fc = fc; h->Qc = Qc;
+    h->cap_pts = cap_pts; h->cap_items = cap_items; h->cap_ft = cap_ft; h->cap_wny = cap_wny;
+    h->cap_slots = cap_slots;
+    h->timing = timing;
+    h->no_alloc = 0;
+  }
+};
+
+template <typename T, int N>
+static int eval_noalloc_impl(VhsHandle* h, const double* f, double* Q, int npts, hipStream_t st) {
+  constexpr size_t n = static_cast<size_t>(N) * N * N;
+  if (!h->scratch)
+    return fail(DGFS_ERR_NOT_INITIALIZED, "no scratch array on this device: %s", h->scratch_err.c_str());
+  // per-slot buffers of the prep/finalize (and multi-pass gain) grids take at
+  // most half of the array; the rest holds the chunk's spectra and partials
+  int slots = std::min(occupancy_slots<T, N>(), cu_count());
+  const size_t per_slot = n * (sizeof(cpx<T>) + sizeof(T)) + 3 * N * N * sizeof(cpx<T>) + 3 * kScratchAlign;
+  slots = static_cast<int>(std::max<size_t>(1, std::min<size_t>(slots, kScratchBytes / 2 / per_slot)));
+  // largest balanced chunk that fits; the last chunk may be smaller and
+  // then split its points into more radial groups, so the gain partials'
+  // share is sized for whichever of the two needs more
+  auto plan_for = [&](int cap, int* chunk_out) {
+    const int nchunks = (npts + cap - 1) / cap;
+    const int chunk = (npts + nchunks - 1) / nchunks;
+    const int last = npts - ((npts + chunk - 1) / chunk - 1) * chunk;
+    ScratchPlan a = plan_scratch<T, N>(h, chunk, slots);
+    const ScratchPlan b = plan_scratch<T, N>(h, last, slots);
+    if (b.Ghat > a.Ghat) {
+      a.Ghat = b.Ghat;
+      a.items = b.items;
+    }
+    *chunk_out = chunk;
+    return a;
+  };
+  int cap = npts, chunk = 0;
+  ScratchPlan plan;
+  for (;;) {
+    if (cap < 1)
+      return fail(DGFS_ERR_UNSUPPORTED,
+                  "one point at Nv=%d (precision %d) needs %zu bytes of workspace, beyond the %zu-byte scratch array",
+                  N, h->prec, plan_scratch<T, N>(h, 1, slots).total(), kScratchBytes);
+    plan = plan_for(cap, &chunk);
+    if (plan.total() <= kScratchBytes) break;
+    const double over = static_cast<double>(plan.total()) / static_cast<double>(kScratchBytes);
+    cap = std::min(cap - 1, static_cast<int>(cap / over));
+  }
+  NoAllocScope scope(h);
+  scope.use(static_cast<char*>(h->scratch), plan);
+  for (int p0 = 0; p0 < npts; p0 += chunk) {
+    const int pts = std::min(chunk, npts - p0);
+    const int rc = run_chunk<T, N>(h, f + static_cast<size_t>(p0) * n, Q + static_cast<size_t>(p0) * n, pts, st);
+    if (rc) return rc;
+  }
+  return DGFS_OK;
+}
+
+template <typename T>
+static int dispatch_noalloc(VhsHandle* h, const double* f, double* Q, int npts, hipStream_t st) {
+  switch (h->N) {
+#define DGFS_CASE(NV) \
+  case NV:            \
+    return eval_noalloc_impl<T, NV>(h, f, Q, npts, st);
+    DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(24) DGFS_CASE(32)
+#undef DGFS_CASE
+    default:
+      return fail(DGFS_ERR_UNSUPPORTED, "Nv=%d is not built", h->N);
+  }
+}
+
 template <typename T, int NA, int NB>
 static int run_pair(VhsHandle* ha, const double* fa, double* Qa, int na, VhsHandle* hb, const double* fb, double* Qb,
                     int nb, hipStream_t st) {
@@ -1628,6 +1821,13 @@ extern "C" int dgfs_vhs_create(const dgfs_vhs_desc* d, void** out) {
       for (int k = 0; k < 5; ++k) A[j * 5 + k] += phi[j] * phi[k] * h->cw * h->cw;
   }
   if (invert5(A, h->Ainv)) return fail(DGFS_ERR_INVALID_VALUE, "singular invariant Gram matrix");
+  // the allocation-free entry point's scratch array on this device; a
+  // failure here only disables that entry point
+  if (resolve_scratch(&h->scratch) != DGFS_OK) {
+    h->scratch = nullptr;
+    h->scratch_err = last_error_ref();
+    last_error_ref().clear();
+  }
   *out = h.release();
   return DGFS_OK;
 }
@@ -1674,6 +1874,18 @@ extern "C" int dgfs_vhs_eval_contig(void* p, const double* f, double* Q, int npt
   auto st = static_cast<hipStream_t>(stream);
   return (h->prec == 64) ? dispatch<double>(h, f, Q, false, 0, 0, 0, 0, 0, npts, st)
                          : dispatch<float>(h, f, Q, false, 0, 0, 0, 0, 0, npts, st);
+}
+
+extern "C" int dgfs_vhs_eval_contig_noalloc(void* p, const double* f, double* Q, int npts, void* stream) {
+  VhsHandle* h = as_handle(p);
+  if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
+  if (!f || !Q) return fail(DGFS_ERR_INVALID_VALUE, "null distribution pointer");
+  if (f == Q) return fail(DGFS_ERR_INVALID_VALUE, "f and Q must not alias");
+  if (npts < 0) return fail(DGFS_ERR_INVALID_VALUE, "npts < 0");
+  if (npts == 0) return DGFS_OK;
+  if (h->no_alloc) return fail(DGFS_ERR_INVALID_VALUE, "handle already inside an allocation-free call");
+  auto st = static_cast<hipStream_t>(stream);
+  return (h->prec == 64) ? dispatch_noalloc<double>(h, f, Q, npts, st) : dispatch_noalloc<float>(h, f, Q, npts, st);
 }
 
 extern "C" int dgfs_vhs_get_dims(void* p, int* Nv, int* Nrho, int* M, int* precision, double* L) {

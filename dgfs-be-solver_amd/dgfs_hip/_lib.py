@@ -88,6 +88,7 @@ _SIGNATURES = {
     "dgfs_vhs_create": [C.POINTER(dgfs_vhs_desc), C.POINTER(C.c_void_p)],
     "dgfs_vhs_eval": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P],
     "dgfs_vhs_eval_contig": [_P, _P, _P, _I, _P],
+    "dgfs_vhs_eval_contig_noalloc": [_P, _P, _P, _I, _P],
     "dgfs_vhs_eval_pair": [_P, _P, _P, _I, _P, _P, _P, _I, _P],
     "dgfs_vhs_destroy": [_P],
     "dgfs_vhs_get_config": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],

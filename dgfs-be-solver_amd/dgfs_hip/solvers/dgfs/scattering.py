@@ -74,6 +74,13 @@ class VHSOperator:
         _lib.call("dgfs_vhs_eval_contig", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)),
                   int(npts), C.c_void_p(stream or 0))
 
+    def eval_contig_noalloc(self, f, Q, npts, stream=None):
+        """eval_contig without any device allocation: the workspace is the
+        library's per-device scratch array (256 MiB), shared by every call on
+        that GPU, so calls that overlap on one GPU race."""
+        _lib.call("dgfs_vhs_eval_contig_noalloc", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)),
+                  int(npts), C.c_void_p(stream or 0))
+
     def eval_pair(self, f, Q, npts, other, f_other, Q_other, npts_other, stream=None):
         """This handle's batch and another grid's batch in one persistent gain launch."""
         _lib.call("dgfs_vhs_eval_pair", self._h, C.c_void_p(int(f)), C.c_void_p(int(Q)), int(npts), other._h,

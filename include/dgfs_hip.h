@@ -78,6 +78,18 @@ int dgfs_vhs_eval(void* handle, const double* f, double* Q, int nupts, int ldim,
  * benchmark use this form; it has no reference counterpart beyond fs(). */
 int dgfs_vhs_eval_contig(void* handle, const double* f, double* Q, int npts, void* stream);
 
+/* dgfs_vhs_eval_contig that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code under stream
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * workspace is a 256 MiB device array inside the library, one per GPU, held
+ * until the process ends; points are evaluated in chunks that fit it, and a
+ * configuration whose single point does not fit returns DGFS_ERR_UNSUPPORTED.
+ * Every call on one GPU shares that GPU's array (and the handle's launch
+ * state), so calls that overlap on one GPU race.  Per-kernel timing is not
+ * recorded.  Q agrees with dgfs_vhs_eval_contig up to the summation order of
+ * the radial groups. */
+int dgfs_vhs_eval_contig_noalloc(void* handle, const double* f, double* Q, int npts, void* stream);
+
 /* Two independent contiguous batches on two velocity grids (32+24, 32+16 or
  * 24+16 points per axis, same precision) evaluated together.  With the
  * multi-pass gain (FP64, or when selected) the gain work items of both grids
