@@ -84,6 +84,81 @@ __global__ __launch_bounds__(256) void k_mul_mfma(const double* __restrict__ A, 
   }
 }
 
+// Paired-column form for 16-byte aligned operands (B, C, ldb, ldc and ncol
+// even): lane (li, lk) loads B[4s + lk][c0 + 2 li .. + 1] as one 16-byte load
+// per 32-column quad q, feeding two 16 x 16 tiles, tile 2q with the even and
+// tile 2q+1 with the odd columns, and stores the two tiles' results for a row
+// as one 16-byte store.  A wave takes 128 columns (8 tiles) per group; KS =
+// ceil(k/4) is a template parameter, so the A operand stays in registers and
+// every k-step's loads can be in flight together.  Rows k..4KS-1 of the last
+// step read row k-1 (A is zero there); only the tail group checks columns.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+constexpr int kMul2Cols = 128;
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_mul_mfma2(const double* __restrict__ A, int m, int k, long lda,
+                                                   const double* __restrict__ B, long ldb,
+                                                   double* __restrict__ C, long ldc, long ncol,
+                                                   double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rtiles = (m + 15) / 16;
+  const long nfull = ncol / kMul2Cols;
+  const long ngroups = (ncol + kMul2Cols - 1) / kMul2Cols;
+  const long wave_id = static_cast<long>(blockIdx.x) * (blockDim.x >> 6) + wave;
+  const long nwaves = static_cast<long>(gridDim.x) * (blockDim.x >> 6);
+  const double* brow[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) brow[s] = B + static_cast<long>(min(4 * s + lk, k - 1)) * ldb + 2 * li;
+
+  for (int rt = 0; rt < rtiles; ++rt) {
+    double a[KS];
+    const int row = rt * 16 + li;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int kk = 4 * s + lk;
+      a[s] = (row < m && kk < k) ? A[row * lda + kk] : 0.0;
+    }
+    for (long g = wave_id; g < ngroups; g += nwaves) {
+      const long c0 = g * kMul2Cols;
+      dbl4 acc[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+      const bool full = g < nfull;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        dbl2 b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long col = c0 + 32 * q + 2 * li;
+          b[q] = (full || col < ncol) ? *reinterpret_cast<const dbl2*>(brow[s] + c0 + 32 * q) : dbl2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc[2 * q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[q].x, acc[2 * q], 0, 0, 0);
+          acc[2 * q + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[q].y, acc[2 * q + 1], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long col = c0 + 32 * q + 2 * li;
+        if (!full && col >= ncol) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int orow = rt * 16 + 4 * r + lk;
+          if (orow < m) {
+            dbl2* cp = reinterpret_cast<dbl2*>(C + orow * ldc + col);
+            dbl2 v = dbl2{alpha * acc[2 * q][r], alpha * acc[2 * q + 1][r]};
+            if (beta != 0.0) v += beta * *cp;
+            *cp = v;
+          }
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // pointwise kernels
 // ---------------------------------------------------------------------------
@@ -110,6 +185,72 @@ __global__ __launch_bounds__(256) void k_tflux(int ndims, int nupts, int nvars, 
     for (int k2 = 0; k2 < ndims; ++k2)
       acc += smats[(static_cast<long>(i) * nupts + upt) * ldsm + soa_ix(e, k2, ndims)] * cvv[k2];
     f[(static_cast<long>(i) * nupts + upt) * ldf + c] = acc * s;
+  }
+}
+
+// Paired forms of tflux and negdivconf for 16-byte aligned registers: a
+// thread takes two neighbouring elements of one 32-element AoSoA block (one
+// 16-byte load and store per row), a workgroup 16 consecutive blocks; the
+// (element block, velocity) pair of the workgroup's first block comes from
+// one division, the threads step from it.
+constexpr int kPairBlocks = 16;
+
+__device__ __forceinline__ void block_coords(long nvars, int j, long* eb, int* v) {
+  const unsigned b0 = blockIdx.x * kPairBlocks;
+  const unsigned e0 = b0 / static_cast<unsigned>(nvars);
+  long e = e0;
+  int vv = static_cast<int>(b0 - e0 * static_cast<unsigned>(nvars)) + j;
+  while (vv >= nvars) {
+    vv -= static_cast<int>(nvars);
+    ++e;
+  }
+  *eb = e;
+  *v = vv;
+}
+
+__global__ __launch_bounds__(256) void k_tflux2(int ndims, int nupts, int nvars, int vidx, long nblk,
+                                                const double* __restrict__ u, long ldu,
+                                                const double* __restrict__ smats, long ldsm,
+                                                double* __restrict__ f, long ldf,
+                                                const double* __restrict__ cv, int vsize) {
+  const int j = threadIdx.x >> 4, lane = 2 * (threadIdx.x & 15);
+  const long blk = static_cast<long>(blockIdx.x) * kPairBlocks + j;
+  const int upt = blockIdx.y;
+  if (blk >= nblk) return;
+  long eb;
+  int v;
+  block_coords(nvars, j, &eb, &v);
+  const long c = blk * 32 + lane;
+  const dbl2 s = *reinterpret_cast<const dbl2*>(u + upt * ldu + c);
+  double cvv[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < ndims; ++d) cvv[d] = cv[d * static_cast<long>(vsize) + vidx + v];
+  for (int i = 0; i < ndims; ++i) {
+    const double* sm = smats + (static_cast<long>(i) * nupts + upt) * ldsm + eb * ndims * 32 + lane;
+    dbl2 acc = dbl2{0.0, 0.0};
+    for (int k2 = 0; k2 < ndims; ++k2) acc += *reinterpret_cast<const dbl2*>(sm + 32 * k2) * cvv[k2];
+    *reinterpret_cast<dbl2*>(f + (static_cast<long>(i) * nupts + upt) * ldf + c) = acc * s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_negdivconf2(int nvars, int neles, long nblk, double* __restrict__ out,
+                                                     long ld, const double* __restrict__ rcpdjac, long ldr) {
+  const int j = threadIdx.x >> 4, lane = 2 * (threadIdx.x & 15);
+  const long blk = static_cast<long>(blockIdx.x) * kPairBlocks + j;
+  const int upt = blockIdx.y;
+  if (blk >= nblk) return;
+  long eb;
+  int v;
+  block_coords(nvars, j, &eb, &v);
+  const long e = eb * 32 + lane;
+  if (e >= neles) return;
+  dbl2* p = reinterpret_cast<dbl2*>(out + upt * ld + blk * 32 + lane);
+  const double r0 = rcpdjac[upt * ldr + e];
+  const double r1 = (e + 1 < neles) ? rcpdjac[upt * ldr + e + 1] : 0.0;
+  dbl2 x = *p;
+  if (e + 1 < neles) {
+    *p = dbl2{-r0 * x.x, -r1 * x.y};
+  } else {
+    out[upt * ld + blk * 32 + lane] = -r0 * x.x;
   }
 }
 
@@ -381,6 +522,8 @@ __global__ __launch_bounds__(256) void k_moments(int nupts, int neles, int vsize
   for (int m = 0; m < 15; ++m) out[(static_cast<long>(upt) * 15 + m) * neles + e] = vals[m];
 }
 
+static bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
+
 static dim3 grid2(long ncol, int rows) {
   return dim3(static_cast<unsigned>((ncol + 255) / 256), static_cast<unsigned>(rows));
 }
@@ -397,11 +540,28 @@ extern "C" int frfs_mul(const double* A, int m, int k, long lda, const double* B
   if (ldb < ncol || ldc < ncol || lda < k)
     return fail(DGFS_ERR_INVALID_VALUE, "frfs_mul: leading dimension too small");
   if (ncol == 0) return DGFS_OK;
+  auto st = static_cast<hipStream_t>(stream);
+  const bool paired = (reinterpret_cast<uintptr_t>(B) % 16 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0) &&
+                      ldb % 2 == 0 && ldc % 2 == 0 && ncol % 2 == 0;
+  if (paired) {
+    const long groups = (ncol + kMul2Cols - 1) / kMul2Cols;
+    const unsigned blocks = static_cast<unsigned>(std::max<long>(1, std::min<long>((groups + 3) / 4, 8L * cu_count())));
+    switch ((k + 3) / 4) {
+#define DGFS_MUL2(KS) \
+  case KS: hipLaunchKernelGGL(k_mul_mfma2<KS>, dim3(blocks), dim3(256), 0, st, A, m, k, lda, B, ldb, C, ldc, ncol, alpha, beta); break;
+      DGFS_MUL2(1) DGFS_MUL2(2) DGFS_MUL2(3) DGFS_MUL2(4) DGFS_MUL2(5) DGFS_MUL2(6) DGFS_MUL2(7) DGFS_MUL2(8)
+      DGFS_MUL2(9) DGFS_MUL2(10) DGFS_MUL2(11) DGFS_MUL2(12) DGFS_MUL2(13) DGFS_MUL2(14) DGFS_MUL2(15) DGFS_MUL2(16)
+#undef DGFS_MUL2
+      default: return fail(DGFS_ERR_UNSUPPORTED, "frfs_mul: k=%d", k);
+    }
+    DGFS_LAUNCH_CHECK("frfs_mul");
+    return DGFS_OK;
+  }
   const long ctiles = (ncol + 15) / 16;
   const long groups = (ctiles + kMulTiles - 1) / kMulTiles;
   const long blocks = std::min<long>((groups + 3) / 4, 8L * cu_count());
-  hipLaunchKernelGGL(k_mul_mfma, dim3(static_cast<unsigned>(std::max<long>(blocks, 1))), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), A, m, k, lda, B, ldb, C, ldc, ncol, alpha, beta);
+  hipLaunchKernelGGL(k_mul_mfma, dim3(static_cast<unsigned>(std::max<long>(blocks, 1))), dim3(256), 0, st, A, m, k,
+                     lda, B, ldb, C, ldc, ncol, alpha, beta);
   DGFS_LAUNCH_CHECK("frfs_mul");
   return DGFS_OK;
 }
@@ -415,8 +575,16 @@ extern "C" int dgfs_tflux(int ndims, int nupts, int neles, int nvars, int vidx, 
   const long ncol = ((neles + 31L) / 32) * 32 * nvars;
   if (ldu < ncol || ldf < ncol) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_tflux: leading dimension < ncol");
   if (ncol == 0 || nupts == 0) return DGFS_OK;
-  hipLaunchKernelGGL(k_tflux, grid2(ncol, nupts), dim3(256), 0, static_cast<hipStream_t>(stream), ndims, nupts,
-                     nvars, vidx, ncol, u, ldu, smats, ldsm, f, ldf, cv, vsize);
+  const long nblk = ncol / 32;
+  if (aligned16(u) && aligned16(smats) && aligned16(f) && ldu % 2 == 0 && ldsm % 2 == 0 && ldf % 2 == 0 &&
+      nblk < (1L << 31) / kPairBlocks) {
+    hipLaunchKernelGGL(k_tflux2, dim3(static_cast<unsigned>((nblk + kPairBlocks - 1) / kPairBlocks), nupts),
+                       dim3(256), 0, static_cast<hipStream_t>(stream), ndims, nupts, nvars, vidx, nblk, u, ldu,
+                       smats, ldsm, f, ldf, cv, vsize);
+  } else {
+    hipLaunchKernelGGL(k_tflux, grid2(ncol, nupts), dim3(256), 0, static_cast<hipStream_t>(stream), ndims, nupts,
+                       nvars, vidx, ncol, u, ldu, smats, ldsm, f, ldf, cv, vsize);
+  }
   DGFS_LAUNCH_CHECK("dgfs_tflux");
   return DGFS_OK;
 }
@@ -427,8 +595,14 @@ extern "C" int dgfs_negdivconf(int nupts, int neles, int nvars, double* out, lon
   const long ncol = ((neles + 31L) / 32) * 32 * nvars;
   if (ld < ncol || ldr < neles) return fail(DGFS_ERR_INVALID_VALUE, "dgfs_negdivconf: leading dimension");
   if (ncol == 0 || nupts == 0) return DGFS_OK;
-  hipLaunchKernelGGL(k_negdivconf, grid2(ncol, nupts), dim3(256), 0, static_cast<hipStream_t>(stream), nvars,
-                     neles, ncol, out, ld, rcpdjac, ldr);
+  const long nblk = ncol / 32;
+  if (aligned16(out) && ld % 2 == 0 && nblk < (1L << 31) / kPairBlocks) {
+    hipLaunchKernelGGL(k_negdivconf2, dim3(static_cast<unsigned>((nblk + kPairBlocks - 1) / kPairBlocks), nupts),
+                       dim3(256), 0, static_cast<hipStream_t>(stream), nvars, neles, nblk, out, ld, rcpdjac, ldr);
+  } else {
+    hipLaunchKernelGGL(k_negdivconf, grid2(ncol, nupts), dim3(256), 0, static_cast<hipStream_t>(stream), nvars,
+                       neles, ncol, out, ld, rcpdjac, ldr);
+  }
   DGFS_LAUNCH_CHECK("dgfs_negdivconf");
   return DGFS_OK;
 }

@@ -1425,13 +1425,18 @@ extern "C" int dgfs_sur_eval(void* p, void* vhs, const float* params, const doub
 }
 
 // Surrogate with fallback and a sampled spectral check.  After the indicator
-// routing, the surrogate candidates are ranked by indicator and split into
-// bins of equal count; a stratified sample of each bin is also evaluated by
-// the exact operator, which gives each bin's squared surrogate error per
-// point.  Whole bins, highest estimated error per point first, then fall back
-// until the estimated error of what the surrogate keeps is within
-// `safety * tol` of ||Q|| (relative L2 over the batch).  Sampled points keep
-// their exact Q.  Two host synchronisations: the indicators, the sample.
+// routing, the surrogate candidates are ranked by indicator and cut into bins
+// of equal count, and a stratified sample of every bin is evaluated both by
+// the network and by the exact operator (with the indicator fallback points
+// in the same exact call).  In each bin the sample's squared errors, and the
+// exact |Q|^2, are taken proportional to the squared indicator (a ratio
+// estimator with the indicator as the auxiliary variable), which predicts the
+// unsampled points' error and |Q|^2 from their own indicators.  Whole bins,
+// the largest predicted error per point first, then fall back until the
+// predicted error of the points the network keeps is within safety * tol of
+// the batch's |Q|.  The rejected points take one more exact call, the kept
+// ones the network; sampled points keep their exact Q.  Two host
+// synchronisations: the indicators, the sample.
 extern "C" int dgfs_sur_eval_checked(void* p, void* vhs, const float* params, const double* f, double* Q,
                                      int npts, double threshold, double threshold_hi, const dgfs_sur_check* chk,
                                      int bf16, dgfs_sur_check_stats* stats, void* stream) {
@@ -1452,12 +1457,11 @@ extern "C" int dgfs_sur_eval_checked(void* p, void* vhs, const float* params, co
   auto st = static_cast<hipStream_t>(stream);
   rc = encode_and_read(h, f, npts, st);
   if (rc) return rc;
+  auto ind = [&](int i) { return h->mom_h[static_cast<size_t>(i) * 8 + 5]; };
   std::vector<int> cand, fb;
   route(h, npts, threshold, threshold_hi, cand, fb);
   // rank the candidates by indicator, largest first, and cut equal bins
-  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
-    return h->mom_h[static_cast<size_t>(a) * 8 + 5] > h->mom_h[static_cast<size_t>(b) * 8 + 5];
-  });
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return ind(a) > ind(b); });
   const int nc = static_cast<int>(cand.size());
   const int nbins = std::max(1, std::min(chk->nbins, nc));
   std::vector<int> bin_lo(nbins + 1);
@@ -1489,22 +1493,20 @@ extern "C" int dgfs_sur_eval_checked(void* p, void* vhs, const float* params, co
     }
   }
   const int nsamp = static_cast<int>(samp.size()), nf = static_cast<int>(fb.size());
-  // device index lists: [candidates | fallback + sample | sample | rejected]
-  int* idx_c = h->idx_d;
+  // device index lists: [sample | fallback + sample | rejected | kept]
+  int* idx_s = h->idx_d;
   int* idx_e = h->idx_d + npts;
-  int* idx_s = h->idx_d + 2 * npts;
-  int* idx_r = h->idx_d + 3 * npts;
+  int* idx_r = h->idx_d + 2 * npts;
+  int* idx_k = h->idx_d + 3 * npts;
   std::vector<int> exact(fb);
   exact.insert(exact.end(), samp.begin(), samp.end());
   const int ne = static_cast<int>(exact.size());
-  rc = upload_idx(idx_c, cand, st);
+  rc = upload_idx(idx_s, samp, st);
   if (!rc) rc = upload_idx(idx_e, exact, st);
-  if (!rc) rc = upload_idx(idx_s, samp, st);
   if (!rc) rc = ensure_rows(h, std::max(ne, nc), st);
-  // the network on every candidate (sampled ones too, for the comparison)
-  if (!rc) rc = network_rows(h, params, Q, idx_c, nc, bf16 != 0, st);
-  // exact Q of the fallback and sampled rows (into fb_Q), the sampled rows'
-  // surrogate errors, then the exact rows into Q
+  // the network on the sample, then the exact operator on the fallback and
+  // sampled rows; the sample's errors and every exact row's |Q|^2
+  if (!rc) rc = network_rows(h, params, Q, idx_s, nsamp, bf16 != 0, st);
   if (!rc) rc = exact_rows(h, vhs, f, Q, idx_e, ne, false, st);
   if (rc) return rc;
   if (nsamp) {
@@ -1513,58 +1515,72 @@ extern "C" int dgfs_sur_eval_checked(void* p, void* vhs, const float* params, co
     DGFS_LAUNCH_CHECK("row_err");
   }
   if (ne) {
+    hipLaunchKernelGGL(k_row_norm2, dim3(ne), dim3(256), 0, st, h->fb_Q, ne, h->n,
+                       h->chk_d + 2 * static_cast<size_t>(npts));
+    DGFS_LAUNCH_CHECK("row_norm2");
     hipLaunchKernelGGL(k_rows_copy, rows_grid(h, ne), dim3(256), 0, st, h->fb_Q, Q, idx_e, ne, h->n, 0);
     DGFS_LAUNCH_CHECK("rows_scatter");
   }
-  hipLaunchKernelGGL(k_row_norm2, dim3(npts), dim3(256), 0, st, Q, npts, h->n, h->chk_d + 2 * static_cast<size_t>(npts));
-  DGFS_LAUNCH_CHECK("row_norm2");
   DGFS_HIP_TRY(hipMemcpyAsync(h->chk_h, h->chk_d, static_cast<size_t>(npts) * 3 * sizeof(double),
                               hipMemcpyDeviceToHost, st));
   DGFS_HIP_TRY(hipStreamSynchronize(st));
-  // per-bin error estimate for the unsampled candidates
-  double qnorm2 = 0.0;
-  for (int i = 0; i < npts; ++i) qnorm2 += h->chk_h[2 * static_cast<size_t>(npts) + i];
-  std::vector<double> err_b(nbins, 0.0), cnt_s(nbins, 0.0);
+  // ratio estimates per bin: squared error and |Q|^2 per unit squared
+  // indicator, from the sample
+  constexpr double kTiny = 1e-300;
+  std::vector<double> e_b(nbins, 0.0), q_b(nbins, 0.0), i_b(nbins, 0.0), cnt_b(nbins, 0.0);
   for (int s = 0; s < nsamp; ++s) {
-    err_b[samp_bin[s]] += h->chk_h[2 * s];
-    cnt_s[samp_bin[s]] += 1.0;
+    const int b = samp_bin[s];
+    e_b[b] += h->chk_h[2 * s];
+    q_b[b] += h->chk_h[2 * s + 1];
+    i_b[b] += ind(samp[s]) * ind(samp[s]);
+    cnt_b[b] += 1.0;
   }
+  std::vector<double> rest_b(nbins, 0.0), nrest_b(nbins, 0.0);  // unsampled: sum of ind^2, count
+  for (int b = 0; b < nbins; ++b)
+    for (int r = bin_lo[b]; r < bin_lo[b + 1]; ++r)
+      if (!in_sample[r]) {
+        rest_b[b] += ind(cand[r]) * ind(cand[r]);
+        nrest_b[b] += 1.0;
+      }
+  double qnorm2 = 0.0;
+  for (int i = 0; i < ne; ++i) qnorm2 += h->chk_h[2 * static_cast<size_t>(npts) + i];
   std::vector<double> est(nbins, 0.0), per_pt(nbins, 0.0);
   double kept = 0.0;
   for (int b = 0; b < nbins; ++b) {
-    const double nb = bin_lo[b + 1] - bin_lo[b];
-    if (cnt_s[b] > 0) {
-      per_pt[b] = err_b[b] / cnt_s[b];
-      est[b] = per_pt[b] * (nb - cnt_s[b]);
-    }
+    if (cnt_b[b] == 0 || nrest_b[b] == 0) continue;
+    // with no indicator spread to scale by (all zero), the sample mean
+    const bool ratio = i_b[b] > kTiny;
+    est[b] = ratio ? e_b[b] / i_b[b] * rest_b[b] : e_b[b] / cnt_b[b] * nrest_b[b];
+    qnorm2 += ratio ? q_b[b] / i_b[b] * rest_b[b] : q_b[b] / cnt_b[b] * nrest_b[b];
+    per_pt[b] = est[b] / nrest_b[b];
     kept += est[b];
   }
-  // drop bins, worst error per point first, until the kept estimate fits
+  // drop bins, largest predicted error per point first, until the kept
+  // estimate fits the budget
   const double budget = (chk->safety * chk->tol) * (chk->safety * chk->tol) * qnorm2;
   std::vector<int> order(nbins);
   for (int b = 0; b < nbins; ++b) order[b] = b;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return per_pt[a] > per_pt[b]; });
   std::vector<char> reject(nbins, 0);
   for (int k = 0; k < nbins && kept > budget; ++k) {
+    if (nrest_b[order[k]] == 0) continue;
     reject[order[k]] = 1;
     kept -= est[order[k]];
     out.nrejected_bins += 1;
   }
-  std::vector<int> rej;
+  std::vector<int> rej, keep;
   for (int b = 0; b < nbins; ++b)
-    if (reject[b])
-      for (int r = bin_lo[b]; r < bin_lo[b + 1]; ++r)
-        if (!in_sample[r]) rej.push_back(cand[r]);
-  const int nr = static_cast<int>(rej.size());
-  if (nr) {
-    rc = upload_idx(idx_r, rej, st);
-    if (!rc) rc = ensure_rows(h, nr, st);
-    if (!rc) rc = exact_rows(h, vhs, f, Q, idx_r, nr, true, st);
-    if (rc) return rc;
-  }
+    for (int r = bin_lo[b]; r < bin_lo[b + 1]; ++r)
+      if (!in_sample[r]) (reject[b] ? rej : keep).push_back(cand[r]);
+  const int nr = static_cast<int>(rej.size()), nk = static_cast<int>(keep.size());
+  rc = upload_idx(idx_r, rej, st);
+  if (!rc) rc = upload_idx(idx_k, keep, st);
+  if (!rc) rc = exact_rows(h, vhs, f, Q, idx_r, nr, true, st);
+  if (!rc) rc = network_rows(h, params, Q, idx_k, nk, bf16 != 0, st);
+  if (rc) return rc;
   out.nfallback = nf + nr;
   out.nsampled = nsamp;
-  out.nsurrogate = nc - nsamp - nr;
+  out.nsurrogate = nk;
   out.est_rel_err = qnorm2 > 0 ? std::sqrt(std::max(kept, 0.0) / qnorm2) : 0.0;
   if (stats) *stats = out;
   return DGFS_OK;

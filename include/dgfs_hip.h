@@ -307,12 +307,13 @@ typedef struct dgfs_sur_check_stats {
 
 /* dgfs_sur_eval with a sampled spectral check.  The surrogate candidates of
  * the indicator routing are ranked by indicator and cut into nbins bins of
- * equal count; a stratified sample of every bin is also evaluated by the
- * exact operator, which estimates each bin's squared surrogate error per
- * point; whole bins, worst per-point error first, then fall back to the
- * exact operator until the estimated error of the kept surrogate points is
- * within safety * tol of ||Q||.  Sampled points keep their exact Q.
- * Synchronises the stream twice (indicators, sample). */
+ * equal count; a stratified sample of every bin is evaluated by the network
+ * and by the exact operator.  Per bin, the sample's squared errors and |Q|^2
+ * per unit squared indicator (a ratio estimator) predict the unsampled
+ * points' from their indicators; whole bins, worst predicted error per point
+ * first, fall back to the exact operator until the predicted error of the
+ * points the network keeps is within safety * tol of ||Q||.  Sampled points
+ * keep their exact Q.  Synchronises the stream twice (indicators, sample). */
 int dgfs_sur_eval_checked(void* handle, void* vhs, const float* params, const double* f, double* Q,
                           int npts, double threshold, double threshold_hi, const dgfs_sur_check* check,
                           int bf16, dgfs_sur_check_stats* stats, void* stream);

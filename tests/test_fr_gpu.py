@@ -22,15 +22,21 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("m,k,ncol,beta", [(12, 9, 1000, 0.0), (9, 18, 4096, 0.0), (16, 32, 777, 1.0),
-                                           (9, 12, 333, 1.0), (6, 12, 64, 0.5), (40, 20, 500, 0.0)])
-def test_mul_mfma(m, k, ncol, beta, device):
+@pytest.mark.parametrize("m,k,ncol,beta,pad", [(12, 9, 1000, 0.0, 16), (9, 18, 4096, 0.0, 16), (16, 32, 777, 1.0, 16),
+                                               (9, 12, 333, 1.0, 16), (6, 12, 64, 0.5, 16), (40, 20, 500, 0.0, 16),
+                                               (16, 16, 8256, 0.0, 16), (16, 32, 4128, 1.0, 32),
+                                               (12, 9, 1000, 0.0, 15), (16, 64, 640, 0.5, 0),
+                                               (9, 6, 130, 1.0, 2)])
+def test_mul_mfma(m, k, ncol, beta, pad, device):
+    """frfs_mul against A @ B: the paired-column kernel (even ncol and
+    leading dimensions, 16-byte aligned operands; full 128-column groups and
+    a tail) and the single-column kernel (odd ncol or leading dimension)."""
     import torch
     from dgfs_hip import fr
-    say("mul", m, k, ncol, beta)
+    say("mul", m, k, ncol, beta, pad)
     rng = np.random.default_rng(m * k)
     A = rng.normal(size=(m, k))
-    ldb = ncol + 16
+    ldb = ncol + pad
     B = rng.normal(size=(k, ldb))
     C0 = rng.normal(size=(m, ldb))
     tA = torch.as_tensor(A, device=device)
@@ -42,6 +48,55 @@ def test_mul_mfma(m, k, ncol, beta, device):
     want = 1.5 * A @ B[:, :ncol] + beta * C0[:, :ncol]
     assert rel(C[:, :ncol], want) < 1e-14
     assert np.array_equal(C[:, ncol:], C0[:, ncol:])
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_tflux_negdivconf_kernels(offset, device):
+    """dgfs_tflux and dgfs_negdivconf on a ragged AoSoA register (37
+    elements, 5 velocities, 3 points) against numpy: with 16-byte aligned
+    registers (the paired kernels) and with every register one double off
+    alignment (the single-element kernels).  Padding elements of the last
+    block are left alone by negdivconf."""
+    import torch
+    from dgfs_hip import fr
+    ndims, nupts, neles, nv, vsize, vidx = 2, 3, 37, 5, 9, 3
+    nb = (neles + 31) // 32
+    ncol = nb * 32 * nv
+    ld = ncol + 16
+    rng = np.random.default_rng(offset + 1)
+    u = rng.normal(size=(nupts, ld))
+    sm = rng.normal(size=(ndims * nupts, ld))
+    cv = rng.normal(size=(3, vsize))
+    rcp = rng.normal(size=(nupts, neles + 1))
+    dev = lambda a: torch.as_tensor(np.concatenate([np.zeros(offset), a.ravel()]), device=device)
+    tu, tsm, tcv, trcp = dev(u), dev(sm), torch.as_tensor(cv, device=device), dev(rcp)
+    tf = dev(np.full((ndims * nupts, ld), 7.0))
+    st = torch.cuda.current_stream().cuda_stream
+    p = lambda t: t.data_ptr() + 8 * offset
+    fr.tflux(ndims, nupts, neles, nv, vidx, p(tu), ld, p(tsm), ld, p(tf), ld, tcv, vsize, st)
+    torch.cuda.synchronize()
+    f = tf.cpu().numpy()[offset:].reshape(ndims * nupts, ld)
+    c = np.arange(ncol)
+    blk, lane = c // 32, c % 32
+    v, eb = blk % nv, blk // nv
+    e = eb * 32 + lane
+    want = np.full_like(f, 7.0)
+    for i in range(ndims):
+        for upt in range(nupts):
+            acc = sum(sm[i * nupts + upt][(eb * ndims + k2) * 32 + lane] * cv[k2][vidx + v] for k2 in range(ndims))
+            want[i * nupts + upt, :ncol] = acc * u[upt, :ncol]
+    say("tflux offset", offset, rel(f, want))
+    assert rel(f, want) < 1e-15
+    tout = dev(u)
+    fr.negdivconf(nupts, neles, nv, p(tout), ld, p(trcp), neles + 1, st)
+    torch.cuda.synchronize()
+    out = tout.cpu().numpy()[offset:].reshape(nupts, ld)
+    want = u.copy()
+    live = e < neles
+    for upt in range(nupts):
+        want[upt, :ncol][live] = -rcp[upt][e[live]] * u[upt, :ncol][live]
+    say("negdivconf offset", offset, rel(out, want))
+    assert np.array_equal(out, want)
 
 
 def build_system(order=2, nx=3, ny=2, periodic_x=True, periodic_y=False, Nv=16, L=5.0, NvBS=None,
