@@ -977,7 +977,7 @@ def main():
     bq32 = algorithmic_bytes(nv, K, 8)
     traffic, traffic_src = None, None
     tname = {2: f"vhs_gain_rr_fp32_n{nv}_traffic.json", 3: f"vhs_gain_rr_inl_fp32_n{nv}_traffic.json",
-             4: f"vhs_gain_rr1_fp32_n{nv}_traffic.json"}.get(algo, "")
+             4: f"vhs_gain_rr1_fp32_n{nv}_traffic.json", 5: f"vhs_gain_rr_wl_fp32_n{nv}_traffic.json"}.get(algo, "")
     tr = read_traffic(tname)
     if tr and tr.get("points_per_dispatch"):
         traffic = tr["bytes_per_dispatch"] * pts_per_launch / tr["points_per_dispatch"]
@@ -985,6 +985,7 @@ def main():
                        f"({tr.get('points_per_dispatch')} points per dispatch), scaled by the copy-kernel "
                        f"calibration and to this launch's points; not measured in this run")
     kname = (f"k_vhs_gain_rr<float,{nv},{'true' if algo == 3 else 'false'}>" if algo in (2, 3)
+             else f"k_vhs_gain_rr<float,{nv},false,true>" if algo == 5
              else f"k_vhs_gain_rr1<float,{nv}>" if algo == 4 else f"k_vhs_gain<float,{nv}>")
 
     line = {

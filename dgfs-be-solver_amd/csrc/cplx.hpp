@@ -121,7 +121,19 @@ struct FFT {
       } else {
         constexpr T c = static_cast<T>(cos_rational(static_cast<long>(r) * k, N));
         constexpr T s = static_cast<T>(SIGN * sin_rational(static_cast<long>(r) * k, N));
-        v[r] = twmul<T, SIGN>(y[r * M + k], c, s);
+        // quarter turns (c or s exactly 0, the other +-1) are a swap and sign
+        // changes; the general product would multiply by an exact 0, which
+        // IEEE arithmetic does not let the compiler drop
+        const cpx<T> a = y[r * M + k];
+        if constexpr (c == T(0) && s == T(1)) {
+          v[r] = cpx<T>{-a.y, a.x};
+        } else if constexpr (c == T(0) && s == T(-1)) {
+          v[r] = cpx<T>{a.y, -a.x};
+        } else if constexpr (s == T(0) && c == T(-1)) {
+          v[r] = cpx<T>{-a.x, -a.y};
+        } else {
+          v[r] = twmul<T, SIGN>(a, c, s);
+        }
       }
       twiddle_in<k, r + 1>(y, v);
     }

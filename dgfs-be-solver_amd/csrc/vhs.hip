@@ -380,13 +380,81 @@ struct RRLds {
   static constexpr int s0 = N * N * N / 2;            // real (one per thread and x)
 };
 
-// inverse z then y transform of plane kx for the thread (group g, lane l);
-// the in-plane transpose runs for plane groups [half*G/2, (half+1)*G/2)
-template <typename T, int N, int G>
+// LDS writes and reads of one wave complete before it goes on: the in-plane
+// transposes of the wave-local form exchange values between the lanes of
+// one wave only, so no workgroup barrier is needed, and the wave waits for
+// its own LDS operations only
+__device__ __forceinline__ void lds_wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// In-plane transpose of the pencils of plane group g (lane l of the group)
+// through LDS.  Workgroup form (WL false): plane groups [half*G/2,
+// (half+1)*G/2) take turns on G/2 padded planes between workgroup barriers.
+// Wave-local form (WL true): the 64/N plane groups of a wave take turns on
+// the wave's own padded plane (NT/64 planes in all), with wave-level waits
+// only.  ROWS: the thread writes its pencil as row l and reads column l
+// (z -> y); else it writes column l and reads row l (y -> z).
+template <typename T, int N, int G, bool WL, bool ROWS>
+__device__ __forceinline__ void rr_plane_transpose(cpx<T> (&v)[N], cpx<T>* Xb, int g, int l) {
+  constexpr int NP = N + 1;
+  if constexpr (WL) {
+    constexpr int GPW = 64 / N;
+    static_assert(64 % N == 0, "plane groups must not straddle waves");
+    const int w = g / GPW, sub = g - (g / GPW) * GPW;
+    cpx<T>* Pc = Xb + w * (N * NP) + l;
+    cpx<T>* Pr = Xb + w * (N * NP) + l * NP;
+#pragma unroll
+    for (int r = 0; r < GPW; ++r) {
+      if (sub == r) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if constexpr (ROWS) Pr[k] = v[k]; else Pc[k * NP] = v[k];
+        }
+      }
+      lds_wave_sync();
+      if (sub == r) {
+#pragma unroll
+        for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+          for (int k = k0; k < k0 + 8; ++k) v[k] = ROWS ? Pc[k * NP] : Pr[k];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      lds_wave_sync();
+    }
+  } else {
+    constexpr int GH = G / 2;
+    const int slot = g % GH;
+    cpx<T>* Pc = Xb + slot * (N * NP) + l;
+    cpx<T>* Pr = Xb + slot * (N * NP) + l * NP;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const bool mine = (g / GH) == half;
+      __syncthreads();
+      if (mine) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if constexpr (ROWS) Pr[k] = v[k]; else Pc[k * NP] = v[k];
+        }
+      }
+      __syncthreads();
+      if (mine) {
+#pragma unroll
+        for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+          for (int k = k0; k < k0 + 8; ++k) v[k] = ROWS ? Pc[k * NP] : Pr[k];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+  }
+}
+
+// inverse z then y transform of plane kx for the thread (group g, lane l)
+template <typename T, int N, int G, bool WL>
 __device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __restrict__ Ft, cpx<T>* Xb,
                                               const cpx<T> (*eph)[N], int kx, int g, int l) {
   // eph[0..2]: phase factors of the x, y, z axes; eph[3][k] = (Re + Im, Re - Im) of eph[2][k]
-  constexpr int NP = N + 1, NN = N * N, GH = G / 2;
+  constexpr int NN = N * N;
   // z-pencil (kx, ky = l) from the transposed spectrum Ft[kx][kz][ky].  The
   // loads of a pencil use at most two base addresses and immediate offsets
   // (4 KB reach): the second base is made opaque so the compiler does not
@@ -414,27 +482,7 @@ __device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __re
   }
   FFT<T, N, +1>::run(v);
   __builtin_amdgcn_sched_barrier(0);
-  const int slot = g % GH;
-  cpx<T>* Pc = Xb + slot * (N * NP) + l;
-  cpx<T>* Pr = Xb + slot * (N * NP) + l * NP;
-#pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const bool mine = (g / GH) == half;
-    __syncthreads();
-    if (mine) {
-#pragma unroll
-      for (int z = 0; z < N; ++z) Pr[z] = v[z];
-    }
-    __syncthreads();
-    if (mine) {
-#pragma unroll
-      for (int k0 = 0; k0 < N; k0 += 8) {
-#pragma unroll
-        for (int ky = k0; ky < k0 + 8; ++ky) v[ky] = Pc[ky * NP];
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-  }
+  rr_plane_transpose<T, N, G, WL, true>(v, Xb, g, l);
   FFT<T, N, +1>::run(v);
   __builtin_amdgcn_sched_barrier(0);
 }
@@ -515,34 +563,17 @@ __device__ __forceinline__ void rr_x_forward(const T (&sacc)[N], cpx<T> (&v0)[N]
   }
 }
 
-// forward y then z transform of plane kx, weighted accumulation into G^
-template <typename T, int N, int G>
+// forward y then z transform of plane kx, weighted accumulation into G^;
+// sync: a workgroup barrier first (the wave-local transpose must not
+// overwrite the exchange buffer while other waves still read it)
+template <typename T, int N, int G, bool WL>
 __device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx, int g, int l, T wp,
-                                              const T* __restrict__ b1p, cpx<T>* Gh, bool first) {
-  constexpr int NN = N * N, NP = N + 1, GH = G / 2;
+                                              const T* __restrict__ b1p, cpx<T>* Gh, bool first, bool sync) {
+  constexpr int NN = N * N;
   FFT<T, N, -1>::run(v);
   __builtin_amdgcn_sched_barrier(0);
-  const int slot = g % GH;
-  cpx<T>* Pc = Xb + slot * (N * NP) + l;
-  cpx<T>* Pr = Xb + slot * (N * NP) + l * NP;
-#pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const bool mine = (g / GH) == half;
-    __syncthreads();
-    if (mine) {
-#pragma unroll
-      for (int ky = 0; ky < N; ++ky) Pc[ky * NP] = v[ky];
-    }
-    __syncthreads();
-    if (mine) {
-#pragma unroll
-      for (int k0 = 0; k0 < N; k0 += 8) {
-#pragma unroll
-        for (int z = k0; z < k0 + 8; ++z) v[z] = Pr[z];
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-  }
+  if (WL && sync) __syncthreads();
+  rr_plane_transpose<T, N, G, WL, false>(v, Xb, g, l);
   FFT<T, N, -1>::run(v);
   __builtin_amdgcn_sched_barrier(0);
   const size_t o = static_cast<size_t>(kx) * NN + l * N;
@@ -564,7 +595,8 @@ __device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx
 // INL: the three Nyquist-plane transforms of every (p, q) are computed in the
 // kernel (from the transposed spectrum, into LDS) instead of being read from
 // the buffer k_vhs_nyq writes
-template <typename T, int N, bool INL>
+// WL: in-plane transposes wave-local (rr_plane_transpose)
+template <typename T, int N, bool INL, bool WL = false>
 __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a) {
   constexpr int NT = RRShape<N>::NT, G = RRShape<N>::G, NN = N * N, n = N * N * N, h = N / 2;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -659,8 +691,8 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
           for (int k = 0; k < 3 * NN / NT; ++k) Wl[tid + k * NT] = wv[k];
           __syncthreads();
         }
-        rr_zy_inverse<T, N, G>(v0, Ft, Xb, eph, g, g, l);
-        rr_zy_inverse<T, N, G>(v1, Ft, Xb, eph, g + G, g, l);
+        rr_zy_inverse<T, N, G, WL>(v0, Ft, Xb, eph, g, g, l);
+        rr_zy_inverse<T, N, G, WL>(v1, Ft, Xb, eph, g + G, g, l);
         T val[N];
         cpx<T> u[N];
         // target slot 0 (y = g): both sub-rounds
@@ -697,8 +729,8 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
       rr_x_forward<T, N, G>(s1, v0, v1, Xb, 1, g, l);
       const T wp = static_cast<T>(a.wgt[p]);
       const T* b1p = a.b1 + static_cast<size_t>(p) * n;
-      rr_yz_forward<T, N, G>(v0, Xb, g, g, l, wp, b1p, Gh, p == p0);
-      rr_yz_forward<T, N, G>(v1, Xb, g + G, g, l, wp, b1p, Gh, p == p0);
+      rr_yz_forward<T, N, G, WL>(v0, Xb, g, g, l, wp, b1p, Gh, p == p0, true);
+      rr_yz_forward<T, N, G, WL>(v1, Xb, g + G, g, l, wp, b1p, Gh, p == p0, false);
     }
   }
 }
@@ -1087,7 +1119,7 @@ struct VhsHandle {
   int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident (Nyquist planes
                          // from k_vhs_nyq), 3 register-resident with in-kernel Nyquist planes,
                          // 4 register-resident with one pencil per thread (Nyquist planes
-                         // from k_vhs_nyq)
+                         // from k_vhs_nyq), 5 as 2 with wave-local in-plane transposes
   int last_algorithm = 0;
   void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
   void* Ft = nullptr;    // spectrum transposed to [kx][kz][ky] [cap_pts][n]
@@ -1179,17 +1211,20 @@ constexpr bool rr1_available() {
 static bool rr2_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 32); }
 static bool rr1_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 24); }
 
-// gain algorithm a launch uses: 1 multi-pass, 2/3 two pencils per thread,
+// gain algorithm a launch uses: 1 multi-pass, 2/3/5 two pencils per thread,
 // 4 one pencil per thread.  The automatic choice takes the one-pencil kernel
 // where the field fits in LDS (Nv 16, 24: 1.8x the two-pencil kernel at 16^3
-// on MI355X), the two-pencil kernel at Nv 32, the multi-pass one otherwise.
+// on MI355X), the two-pencil kernel with wave-local in-plane transposes at
+// Nv 32 (61.0 against 62.0 ms per 4096-point launch with the workgroup-wide
+// ones, bitwise the same Q), the multi-pass one otherwise.
 static int rr_kind(const VhsHandle* h) {
   switch (h->algorithm) {
     case 1: return 1;
     case 2: return rr2_ok(h) ? 2 : 1;
     case 3: return rr2_ok(h) ? 3 : 1;
     case 4: return rr1_ok(h) ? 4 : 1;
-    default: return rr1_ok(h) ? 4 : (rr2_ok(h) ? 2 : 1);
+    case 5: return rr2_ok(h) ? 5 : 1;
+    default: return rr1_ok(h) ? 4 : (rr2_ok(h) ? 5 : 1);
   }
 }
 
@@ -1200,8 +1235,9 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
   if constexpr (rr_available<T, N>()) {
     constexpr int NN = N * N;
     const int K = h->Nrho * h->M;
-    const bool inl = (rr_kind(h) == 3);
-    h->last_algorithm = inl ? 3 : 2;
+    const int kind = rr_kind(h);
+    const bool inl = (kind == 3);
+    h->last_algorithm = kind;
     const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
     if (!inl && wbytes > h->cap_wny) {
       if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: Nyquist planes exceed their scratch share");
@@ -1225,7 +1261,8 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
     const size_t lds = (static_cast<size_t>(RRLds<N>::xsize) + RRLds<N>::wl) * sizeof(cpx<T>) +
                        static_cast<size_t>(RRLds<N>::s0) * sizeof(T);
     const void* kfn = inl ? reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, true>)
-                          : reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, false>);
+                      : (kind == 5) ? reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, false, true>)
+                                    : reinterpret_cast<const void*>(k_vhs_gain_rr<T, N, false>);
     DGFS_HIP_TRY(hipFuncSetAttribute(kfn,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(cu_count())));
@@ -1234,6 +1271,8 @@ static int run_gain_rr(VhsHandle* h, int npts, int groups, int ppg, size_t items
                      static_cast<const cpx<T>*>(h->ephtab), static_cast<int>(items), groups, ppg, h->Nrho, h->M};
     if (inl)
       hipLaunchKernelGGL((k_vhs_gain_rr<T, N, true>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
+    else if (kind == 5)
+      hipLaunchKernelGGL((k_vhs_gain_rr<T, N, false, true>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
     else
       hipLaunchKernelGGL((k_vhs_gain_rr<T, N, false>), dim3(grid), dim3(RRShape<N>::NT), lds, st, ga);
     DGFS_LAUNCH_CHECK("vhs_gain_rr");
@@ -1294,9 +1333,29 @@ static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t item
 }
 
 // radial groups per point: enough (point, group) items for two per slot
+// radial groups per point.  Automatic: the split whose (point, group) items
+// fill the persistent grid in the fewest rounds of radial work: rounds of
+// items ceil(npts g / slots), each ppg = ceil(Nrho / g) radial nodes long, with
+// every group beyond the first costing 1 % (the finalize kernel reads its
+// partial G^).  A batch that is not a multiple of the grid then splits finer
+// rather than leaving most of its last round idle (3600 points on 256 CUs:
+// 2 groups, 14.5 rounds of full items instead of 15).
 static int pick_groups(const VhsHandle* h, int npts, int slots) {
   int groups = h->groups_override;
-  if (groups <= 0) groups = std::max(1, std::min(h->Nrho, (2 * slots + npts - 1) / std::max(npts, 1)));
+  if (groups <= 0) {
+    const long P = std::max(npts, 1), S = std::max(slots, 1);
+    double best = 0.0;
+    groups = 1;
+    for (int g = 1; g <= h->Nrho; ++g) {
+      const int ppg = (h->Nrho + g - 1) / g, ge = (h->Nrho + ppg - 1) / ppg;
+      if (ge != g) continue;  // the same split as a smaller g
+      const double cost = static_cast<double>((P * ge + S - 1) / S) * ppg * (1.0 + 0.01 * (ge - 1));
+      if (g == 1 || cost < best) {
+        best = cost;
+        groups = ge;
+      }
+    }
+  }
   const int ppg = (h->Nrho + groups - 1) / groups;
   return (h->Nrho + ppg - 1) / ppg;
 }
@@ -1951,8 +2010,8 @@ extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
 extern "C" int dgfs_vhs_set_algorithm(void* p, int algorithm) {
   VhsHandle* h = as_handle(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
-  if (algorithm < 0 || algorithm > 4) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0, 1, 2, 3 or 4");
-  if ((algorithm == 2 || algorithm == 3) && !rr2_ok(h))
+  if (algorithm < 0 || algorithm > 5) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0 to 5");
+  if ((algorithm == 2 || algorithm == 3 || algorithm == 5) && !rr2_ok(h))
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (two pencils per thread): FP32 and Nv in {16, 32} only");
   if (algorithm == 4 && !rr1_ok(h))
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (one pencil per thread): FP32 and Nv in {16, 24} only");

@@ -71,7 +71,7 @@ class Autotuner:
         if op.precision == 32 and op.Nv in (16, 24):
             algos[:0] = [4]
         if op.precision == 32 and op.Nv in (16, 32):
-            algos[:0] = [2, 3]
+            algos[:0] = [2, 3, 5]
         groups = [0]
         g = 1
         while g <= op.Nrho:

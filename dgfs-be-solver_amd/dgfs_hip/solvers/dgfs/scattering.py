@@ -113,7 +113,8 @@ class VHSOperator:
     def set_algorithm(self, algorithm):
         """0 automatic, 1 multi-pass, 2 register-resident (FP32, Nv 16/32),
         3 register-resident with in-kernel Nyquist planes, 4 register-resident
-        with one pencil per thread (FP32, Nv 16/24)."""
+        with one pencil per thread (FP32, Nv 16/24), 5 as 2 with wave-local
+        in-plane transposes."""
         _lib.call("dgfs_vhs_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):

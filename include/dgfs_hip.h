@@ -126,9 +126,10 @@ int dgfs_vhs_set_slots(void* handle, int nslots);
  * corrections precomputed by a separate kernel), 3 register-resident with the
  * Nyquist-plane corrections computed inside the gain kernel, 4 register-resident
  * with one pencil per thread and the transposes through LDS (FP32 with Nv 16
- * or 24; Nyquist planes from the separate kernel).  The automatic choice is 4
- * at Nv 16 and 24, 2 at Nv 32, 1 otherwise.  get_algorithm reports the one the
- * last eval ran.
+ * or 24; Nyquist planes from the separate kernel), 5 as 2 with the in-plane
+ * transposes wave-local (no workgroup barrier between the z, y passes).  The
+ * automatic choice is 4 at Nv 16 and 24, 5 at Nv 32, 1 otherwise.
+ * get_algorithm reports the one the last eval ran.
  * No reference counterpart (the reference has one cuFFT dataflow). */
 int dgfs_vhs_set_algorithm(void* handle, int algorithm);
 int dgfs_vhs_get_algorithm(void* handle, int* algorithm);

@@ -178,17 +178,20 @@ def test_bad_arguments_raise(device):
         make_op(10, 5.0, 4, 6, 0.38, 1.0)
 
 
-RR_CASES = {16: [(1, 0), (2, 0), (2, 1), (2, 3), (3, 0), (3, 3), (4, 0), (4, 3)],
+RR_CASES = {16: [(1, 0), (2, 0), (2, 1), (2, 3), (3, 0), (3, 3), (4, 0), (4, 3), (5, 0), (5, 3)],
             24: [(1, 0), (4, 0), (4, 1), (4, 3)],
-            32: [(1, 0), (2, 0), (2, 1), (2, 3), (3, 0), (3, 3)]}
+            32: [(1, 0), (2, 0), (2, 1), (2, 3), (3, 0), (3, 3), (5, 0), (5, 1), (5, 3)]}
 
 
 @pytest.mark.parametrize("N,L,Nrho,M", [(16, 6.0, 8, 6), (24, 7.0, 12, 12), (32, 6.0, 16, 12)])
 def test_register_resident_matches(N, L, Nrho, M, device):
     """The register-resident FP32 gain kernels (two pencils per thread with
     the Nyquist planes precomputed, algorithm 2, or computed in the kernel,
-    algorithm 3; one pencil per thread, algorithm 4) against the multi-pass
-    FP32 kernel (same arithmetic up to FP32 rounding) and the FP64 oracle."""
+    algorithm 3; one pencil per thread, algorithm 4; algorithm 2 with
+    wave-local in-plane transposes, algorithm 5) against the multi-pass FP32
+    kernel (same arithmetic up to FP32 rounding) and the FP64 oracle.
+    Algorithms 2 and 5 do the same arithmetic in the same order: bitwise
+    equal."""
     ov = O.VelocityMesh(N, L, Nrho, M)
     f = smooth_states(ov, 5, seed=N + 1)
     Qref = O.VHSOracle(ov, 0.38, 1.0).q(f)
@@ -206,6 +209,9 @@ def test_register_resident_matches(N, L, Nrho, M, device):
         say("  ", key, "vs oracle", e, "vs multi-pass", rel(Q, outs[(1, 0)]))
         assert e < max(3 * e_mp, 1e-4)
         assert rel(Q, outs[(1, 0)]) < 1e-4
+    for key in outs:
+        if key[0] == 5:
+            assert np.array_equal(outs[key], outs[(2, key[1])])
 
 
 @pytest.mark.parametrize("precision", [64, 32])
@@ -256,7 +262,7 @@ def test_pair_register_resident_c3_grids(device):
         opa.eval_pair(fa_t.data_ptr(), Qa_t.data_ptr(), 6, opb, fb_t.data_ptr(), Qb_t.data_ptr(), 5,
                       torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
-        assert opa.get_algorithm() in (2, 3) and opb.get_algorithm() == 4
+        assert opa.get_algorithm() in (2, 3, 5) and opb.get_algorithm() == 4
         assert np.array_equal(Qa_t.cpu().numpy(), Qa) and np.array_equal(Qb_t.cpu().numpy(), Qb)
     for ov, f, Q in ((ova, fa, Qa), (ovb, fb, Qb)):
         ref = O.project_invariants(ov, O.VHSOracle(ov, 0.38, 1.0).q(f))

@@ -104,6 +104,37 @@ def test_oracle_parity_fp64_fp32_16(device):
     assert max(e32) < 1e-3
 
 
+# the cylinder workload's grid and rule at a reduced radial rule: 24^3 on
+# [-6, 6]^3, M = 12, Nrho = 6, Mpre = 24, VHS (alpha = 1, the isotropic gain)
+# and VSS (alpha = 1.3, the general gain)
+CFG24 = (24, 6.0, 6, 12, 24, (1.0, 2.1), 1.0, (0.81, 0.805, 0.805, 0.80), (1.0, 0.9, 0.9, 0.8))
+
+
+@pytest.mark.parametrize("alpha", [1.0, 1.3])
+def test_oracle_parity_fp64_fp32_24(alpha, device):
+    """24^3 against the FP64 CPU oracle on 2 bimodal mixture points: the FP64
+    operator (multi-pass) to 1e-10 and the FP32 one (register-resident, the
+    default at 24^3) to 1e-3."""
+    cfg = CFG24[:6] + (alpha,) + CFG24[7:]
+    N, L, Nrho, M, Mpre, masses = cfg[:6]
+    vm = V.BiVelocityMesh(N, L, Nrho, M, masses)
+    fs = bimodal_states(vm, 2, seed=24)
+    Qref, _ = oracle_q(*cfg, fs)
+    say("vss oracle parity 24^3, alpha", alpha)
+    op64, _ = make_op(*cfg, precision=64)
+    Q64 = run_contig(op64, fs, device)
+    e64 = [rel(Q64[p], Qref[p]) for p in range(2)]
+    say("  fp64", e64)
+    assert max(e64) < 1e-10
+    op32, _ = make_op(*cfg, precision=32)
+    Q32 = run_contig(op32, fs, device)
+    say("  fp32 algorithm", op32.get_algorithm())
+    e32 = [rel(Q32[p], Qref[p]) for p in range(2)]
+    say("  fp32", e32)
+    assert op32.get_algorithm() >= 4
+    assert max(e32) < 1e-3
+
+
 def test_group_split_and_aosoa_path(device):
     import torch
     N, L, Nrho, M, Mpre, masses, alpha, omegas, prefac = 8, 5.0, 4, 12, 24, (1.0, 2.1), 1.2, \
