@@ -105,8 +105,12 @@ struct first_radix {
 // in-register DFT of length N (stride-1 in and out); SIGN = -1 forward,
 // +1 inverse, both unnormalised.  Decimation in time: x[r + R*m] gives R
 // sub-DFTs of length M = N/R, recombined by R-point butterflies with
-// twiddles W_N^{r k} that are compile-time constants.
-template <typename T, int N, int SIGN>
+// twiddles W_N^{r k} that are compile-time constants.  TAN: radix-4
+// butterflies in the tangent form (twpm), 6.4 % fewer operations at N = 32;
+// measured on MI355X it speeds the register-resident gain kernels (1.6 %)
+// and slows the LDS-staged prep / finalize / Nyquist kernels (their FFTs
+// then hold more literal constants), so only the gain kernels take it.
+template <typename T, int N, int SIGN, bool TAN = false>
 struct FFT {
   static_assert(N >= 1, "bad length");
   static constexpr int R = first_radix<N>::value;
@@ -139,14 +143,82 @@ struct FFT {
     }
   }
 
+  __device__ __forceinline__ static T fmadd(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c); else return __builtin_fmaf(a, b, c);
+  }
+
+  // a * W^e, W = exp(SIGN 2 pi i / N)
+  template <int e>
+  __device__ __forceinline__ static cpx<T> tw(cpx<T> a) {
+    constexpr T c = static_cast<T>(cos_rational(e, N));
+    constexpr T s = static_cast<T>(SIGN * sin_rational(e, N));
+    if constexpr (e % N == 0) {
+      return a;
+    } else if constexpr (c == T(0) && s == T(1)) {
+      return cpx<T>{-a.y, a.x};
+    } else if constexpr (c == T(0) && s == T(-1)) {
+      return cpx<T>{a.y, -a.x};
+    } else if constexpr (s == T(0) && c == T(-1)) {
+      return cpx<T>{-a.x, -a.y};
+    } else {
+      return twmul<T, SIGN>(a, c, s);
+    }
+  }
+
+  // sum = p + W^e a, dif = p - W^e a.  With W^e = c + i s and |c| >= |s|,
+  // W^e a = c (a.x - t a.y, a.y + t a.x), t = s / c; otherwise
+  // W^e a = s (t a.x - a.y, t a.y + a.x), t = c / s.  The scaled vector u
+  // costs two fused multiply-adds and each output two more: six operations
+  // where the product and the two sums take eight (the tangent form of a
+  // twiddled butterfly; t is a compile-time constant).
+  template <int e>
+  __device__ __forceinline__ static void twpm(cpx<T> p, cpx<T> a, cpx<T>& sum, cpx<T>& dif) {
+    constexpr double cd = cos_rational(e, N), sd = SIGN * sin_rational(e, N);
+    constexpr T c = static_cast<T>(cd), s = static_cast<T>(sd);
+    if constexpr (e % N == 0 || (c == T(0) && (s == T(1) || s == T(-1))) || (s == T(0) && c == T(-1))) {
+      const cpx<T> b = tw<e>(a);
+      sum = p + b;
+      dif = p - b;
+    } else if constexpr ((cd < 0 ? -cd : cd) >= (sd < 0 ? -sd : sd)) {
+      constexpr T t = static_cast<T>(sd / cd);
+      const cpx<T> u{fmadd(-t, a.y, a.x), fmadd(t, a.x, a.y)};
+      sum = cpx<T>{fmadd(c, u.x, p.x), fmadd(c, u.y, p.y)};
+      dif = cpx<T>{fmadd(-c, u.x, p.x), fmadd(-c, u.y, p.y)};
+    } else {
+      constexpr T t = static_cast<T>(cd / sd);
+      const cpx<T> u{fmadd(t, a.x, -a.y), fmadd(t, a.y, a.x)};
+      sum = cpx<T>{fmadd(s, u.x, p.x), fmadd(s, u.y, p.y)};
+      dif = cpx<T>{fmadd(-s, u.x, p.x), fmadd(-s, u.y, p.y)};
+    }
+  }
+
+  // radix-4 butterfly k with the twiddles folded into the sums:
+  // t0,1 = a0 +- w2 a2; b1 = w1 a1; t2,3 = b1 +- w3 a3; X = t0 +- t2,
+  // t1 +- (SIGN i) t3 (w_r = W^{r k})
+  template <int k>
+  __device__ __forceinline__ static void radix4(cpx<T>* x, const cpx<T>* y) {
+    cpx<T> t0, t1, t2, t3;
+    twpm<2 * k>(y[k], y[2 * M + k], t0, t1);
+    twpm<3 * k>(tw<k>(y[M + k]), y[3 * M + k], t2, t3);
+    const cpx<T> t3r = (SIGN > 0) ? mk<T>(-t3.y, t3.x) : mk<T>(t3.y, -t3.x);
+    x[k] = t0 + t2;
+    x[k + 2 * M] = t0 - t2;
+    x[k + M] = t1 + t3r;
+    x[k + 3 * M] = t1 - t3r;
+  }
+
   template <int k>
   __device__ __forceinline__ static void butterflies(cpx<T>* x, const cpx<T>* y) {
     if constexpr (k < M) {
-      cpx<T> v[R];
-      twiddle_in<k, 0>(y, v);
-      FFT<T, R, SIGN>::run(v);
+      if constexpr (R == 4 && TAN) {
+        radix4<k>(x, y);
+      } else {
+        cpx<T> v[R];
+        twiddle_in<k, 0>(y, v);
+        FFT<T, R, SIGN, TAN>::run(v);
 #pragma unroll
-      for (int s = 0; s < R; ++s) x[k + M * s] = v[s];
+        for (int s = 0; s < R; ++s) x[k + M * s] = v[s];
+      }
       butterflies<k + 1>(x, y);
     }
   }
@@ -183,7 +255,7 @@ struct FFT {
 #pragma unroll
         for (int m = 0; m < M; ++m) y[r * M + m] = x[r + R * m];
 #pragma unroll
-      for (int r = 0; r < R; ++r) FFT<T, M, SIGN>::run(y + r * M);
+      for (int r = 0; r < R; ++r) FFT<T, M, SIGN, TAN>::run(y + r * M);
       butterflies<0>(x, y);
     }
   }

@@ -480,10 +480,10 @@ __device__ __forceinline__ void rr_zy_inverse(cpx<T> (&v)[N], const cpx<T>* __re
     }
     __builtin_amdgcn_sched_barrier(0);
   }
-  FFT<T, N, +1>::run(v);
+  FFT<T, N, +1, true>::run(v);
   __builtin_amdgcn_sched_barrier(0);
   rr_plane_transpose<T, N, G, WL, true>(v, Xb, g, l);
-  FFT<T, N, +1>::run(v);
+  FFT<T, N, +1, true>::run(v);
   __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -514,7 +514,7 @@ __device__ __forceinline__ void rr_x_read(cpx<T> (&u)[N], const cpx<T>* Xb, int 
 template <typename T, int N, int G>
 __device__ __forceinline__ void rr_x_compute(T (&val)[N], cpx<T> (&u)[N], const cpx<T>* Wl, int r, int g, int l) {
   constexpr int NN = N * N;
-  FFT<T, N, +1>::run(u);
+  FFT<T, N, +1, true>::run(u);
   __builtin_amdgcn_sched_barrier(0);
   const int y = g + G * r, z = l;
   const cpx<T> w1 = Wl[y * N + z];
@@ -543,7 +543,7 @@ __device__ __forceinline__ void rr_x_forward(const T (&sacc)[N], cpx<T> (&v0)[N]
   cpx<T> u[N];
 #pragma unroll
   for (int x = 0; x < N; ++x) u[x] = cpx<T>{sacc[x], T(0)};
-  FFT<T, N, -1>::run(u);
+  FFT<T, N, -1, true>::run(u);
   __builtin_amdgcn_sched_barrier(0);
   cpx<T>* Xw = Xb + g * N + l;         // write (kxl, yy = g, z = l)
   const cpx<T>* Xr = Xb + (g * G) * N + l;
@@ -570,11 +570,11 @@ template <typename T, int N, int G, bool WL>
 __device__ __forceinline__ void rr_yz_forward(cpx<T> (&v)[N], cpx<T>* Xb, int kx, int g, int l, T wp,
                                               const T* __restrict__ b1p, cpx<T>* Gh, bool first, bool sync) {
   constexpr int NN = N * N;
-  FFT<T, N, -1>::run(v);
+  FFT<T, N, -1, true>::run(v);
   __builtin_amdgcn_sched_barrier(0);
   if (WL && sync) __syncthreads();
   rr_plane_transpose<T, N, G, WL, false>(v, Xb, g, l);
-  FFT<T, N, -1>::run(v);
+  FFT<T, N, -1, true>::run(v);
   __builtin_amdgcn_sched_barrier(0);
   const size_t o = static_cast<size_t>(kx) * NN + l * N;
   const T* b = b1p + o;
@@ -818,20 +818,20 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
             v[kz] = (exy.x * pm.x + exy.y * pm.y) * v[kz];
           }
         }
-        FFT<T, N, +1>::run(v);  // z
+        FFT<T, N, +1, true>::run(v);  // z
 #pragma unroll
         for (int z = 0; z < N; ++z) Xp[l * PP + z] = v[z];
         __syncthreads();
 #pragma unroll
         for (int ky = 0; ky < N; ++ky) v[ky] = Xp[ky * PP + l];
-        FFT<T, N, +1>::run(v);  // y: (kx = g, y, z = l)
+        FFT<T, N, +1, true>::run(v);  // y: (kx = g, y, z = l)
         __syncthreads();
 #pragma unroll
         for (int y = 0; y < N; ++y) X[g * KS + y * N + l] = v[y];
         __syncthreads();
 #pragma unroll
         for (int kx = 0; kx < N; ++kx) v[kx] = X[kx * KS + g * N + l];
-        FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
+        FFT<T, N, +1, true>::run(v);  // x: (x, y = g, z = l)
         {
           const cpx<T> w1 = Wl[g * N + l];
           const cpx<T>* W2 = Wl + NN + l;
@@ -850,21 +850,21 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
       // forward transform of S_p and accumulation into G^
 #pragma unroll
       for (int x = 0; x < N; ++x) v[x] = cpx<T>{sacc[x], T(0)};
-      FFT<T, N, -1>::run(v);  // x: (kx, y = g, z = l)
+      FFT<T, N, -1, true>::run(v);  // x: (kx, y = g, z = l)
       __syncthreads();
 #pragma unroll
       for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = v[kx];
       __syncthreads();
 #pragma unroll
       for (int y = 0; y < N; ++y) v[y] = X[g * KS + y * N + l];
-      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
+      FFT<T, N, -1, true>::run(v);  // y: (kx = g, ky, z = l)
       __syncthreads();
 #pragma unroll
       for (int ky = 0; ky < N; ++ky) Xp[ky * PP + l] = v[ky];
       __syncthreads();
 #pragma unroll
       for (int z = 0; z < N; ++z) v[z] = Xp[l * PP + z];
-      FFT<T, N, -1>::run(v);  // z: (kx = g, ky = l, kz)
+      FFT<T, N, -1, true>::run(v);  // z: (kx = g, ky = l, kz)
       const T wp = static_cast<T>(a.wgt[p]);
       const size_t o = static_cast<size_t>(g) * NN + l * N;
       const T* b = a.b1 + static_cast<size_t>(p) * n + o;

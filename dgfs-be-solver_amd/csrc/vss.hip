@@ -363,20 +363,20 @@ __device__ __forceinline__ void vss_rr1_inverse(cpx<T> (&v)[N], const cpx<T>* __
 template <typename T, int N>
 __device__ __forceinline__ void vss_rr1_inverse_tail(cpx<T> (&v)[N], cpx<T>* X, cpx<T>* Xp, int g, int l) {
   constexpr int PP = VssRR1Shape<N>::PP, KS = VssRR1Shape<N>::KS;
-  FFT<T, N, +1>::run(v);  // y: (kx = g, y, kz = l)
+  FFT<T, N, +1, true>::run(v);  // y: (kx = g, y, kz = l)
 #pragma unroll
   for (int y = 0; y < N; ++y) Xp[y * PP + l] = v[y];
   __syncthreads();
 #pragma unroll
   for (int kz = 0; kz < N; ++kz) v[kz] = Xp[l * PP + kz];
-  FFT<T, N, +1>::run(v);  // z: (kx = g, y = l, z)
+  FFT<T, N, +1, true>::run(v);  // z: (kx = g, y = l, z)
   __syncthreads();
 #pragma unroll
   for (int z = 0; z < N; ++z) X[g * KS + z * N + l] = v[z];
   __syncthreads();
 #pragma unroll
   for (int kx = 0; kx < N; ++kx) v[kx] = X[kx * KS + g * N + l];
-  FFT<T, N, +1>::run(v);  // x: (x, y = l, z = g)
+  FFT<T, N, +1, true>::run(v);  // x: (x, y = l, z = g)
 }
 
 template <typename T, int N>
@@ -437,21 +437,21 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
         for (int k = 0; k < KP; ++k) u[N - KP + k] = upark[k][tid];
 #pragma unroll
         for (int xx = 0; xx < N; ++xx) u[xx] = u[xx] * v[xx];
-        FFT<T, N, -1>::run(u);  // x: (kx, y = l, z = g)
+        FFT<T, N, -1, true>::run(u);  // x: (kx, y = l, z = g)
         __syncthreads();
 #pragma unroll
         for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = u[kx];
         __syncthreads();
 #pragma unroll
         for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
-        FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
+        FFT<T, N, -1, true>::run(v);  // z: (kx = g, y = l, kz)
         __syncthreads();
 #pragma unroll
         for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
         __syncthreads();
 #pragma unroll
         for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
-        FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+        FFT<T, N, -1, true>::run(v);  // y: (kx = g, ky, kz = l)
         {
           const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
                                        (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n + o);
@@ -546,21 +546,21 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGain
 #pragma unroll
       for (int k = 0; k < KP; ++k) acc[N - KP + k] = apark[k][tid];
       // G^ += w b(x) FFT(sum_y t3 t4)
-      FFT<T, N, -1>::run(acc);  // x: (kx, y = l, z = g)
+      FFT<T, N, -1, true>::run(acc);  // x: (kx, y = l, z = g)
       __syncthreads();
 #pragma unroll
       for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = acc[kx];
       __syncthreads();
 #pragma unroll
       for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
-      FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
+      FFT<T, N, -1, true>::run(v);  // z: (kx = g, y = l, kz)
       __syncthreads();
 #pragma unroll
       for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
       __syncthreads();
 #pragma unroll
       for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
-      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+      FFT<T, N, -1, true>::run(v);  // y: (kx = g, ky, kz = l)
       const T w = static_cast<T>(a.wgt[x]);
       {
         const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
           const int q = tid / N, i = tid - q * N;
 #pragma unroll
           for (int jj = 0; jj < N; ++jj) v[jj] = Wp[q][i * PW + jj];
-          FFT<T, N, +1>::run(v);
+          FFT<T, N, +1, true>::run(v);
 #pragma unroll
           for (int jj = 0; jj < N; ++jj) Wp[q][i * PW + jj] = v[jj];
         }
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
           const int q = tid / N, jj = tid - q * N;
 #pragma unroll
           for (int i = 0; i < N; ++i) v[i] = Wp[q][i * PW + jj];
-          FFT<T, N, +1>::run(v);
+          FFT<T, N, +1, true>::run(v);
 #pragma unroll
           for (int i = 0; i < N; ++i) Wp[q][i * PW + jj] = v[i];
         }
@@ -733,21 +733,21 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
 #pragma unroll
       for (int k = 0; k < KP; ++k) acc[N - KP + k] = apark[k][tid];
       // G^ += w b(x) FFT(sum_y t3 t4)
-      FFT<T, N, -1>::run(acc);  // x: (kx, y = l, z = g)
+      FFT<T, N, -1, true>::run(acc);  // x: (kx, y = l, z = g)
       __syncthreads();
 #pragma unroll
       for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = acc[kx];
       __syncthreads();
 #pragma unroll
       for (int z = 0; z < N; ++z) v[z] = X[g * KS + z * N + l];
-      FFT<T, N, -1>::run(v);  // z: (kx = g, y = l, kz)
+      FFT<T, N, -1, true>::run(v);  // z: (kx = g, y = l, kz)
       __syncthreads();
 #pragma unroll
       for (int kz = 0; kz < N; ++kz) Xp[kz * PP + l] = v[kz];
       __syncthreads();
 #pragma unroll
       for (int yy = 0; yy < N; ++yy) v[yy] = Xp[l * PP + yy];
-      FFT<T, N, -1>::run(v);  // y: (kx = g, ky, kz = l)
+      FFT<T, N, -1, true>::run(v);  // y: (kx = g, ky, kz = l)
       const T w = static_cast<T>(a.wgt[x]);
       {
         const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
