@@ -1062,7 +1062,17 @@ def main():
         torch.cuda.synchronize()
         w64 = time.perf_counter() - t0
         v64 = P * args.fp64_steps / w64
+        alg64 = op64.get_algorithm()
         Q64 = from_aosoa(reg_q64, nupts, neles, vm.vsize()).contiguous()
+        # the multi-pass FP64 kernel the slab kernel replaced, one timed eval
+        # (Q bitwise unused: the default's Q64 is the reference below)
+        op64.set_algorithm(1)
+        step64()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step64()
+        torch.cuda.synchronize()
+        w64mp = time.perf_counter() - t0
         del reg_q64
         # the precision claim compares like with like: FP64 with the same FP64
         # projection (the projection itself moves Q by the spectral method's
@@ -1085,7 +1095,10 @@ def main():
             d64 = (Q64p @ phi[j]) * vm.cw()
             mom[name] = float((d32 - d64).abs().max() / absq.max())
         line["fp64_plain"] = {"value": v64, "unit": "Q(f) evals/s", "ms_per_step": 1e3 * w64 / args.fp64_steps,
-                              "steps": args.fp64_steps}
+                              "steps": args.fp64_steps, "algorithm": alg64,
+                              "kernel": {6: "slab", 1: "multi-pass"}.get(alg64, str(alg64))}
+        line["fp64_multipass"] = {"value": P / w64mp, "unit": "Q(f) evals/s", "ms_per_step": 1e3 * w64mp,
+                                  "steps": 1, "note": "the FP64 multi-pass gain kernel (algorithm 1)"}
         line["fp32_vs_fp64_rel_l2"] = err
         line["fp32_vs_fp64_unprojected_rel_l2"] = err_unproj
         line["projection_size_rel_l2"] = proj_size

@@ -880,6 +880,280 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Slab gain kernel: the FP64 form for fields that do not fit on one CU (a
+// 32^3 FP64 complex field is 512 KB, the whole register file).  A workgroup
+// owns a (point, radial group, slab) item; slab s holds the XS = N/NS planes
+// x = s + NS j (j < XS) of the inverse transforms, so a slab of XS planes is
+// all that sits in LDS.  Per (p, q):
+//   x-pass  pencils (kz, ky) of the (1 + i)-folded spectrum times the real
+//           phase factor Re E + Im E.  Only the outputs x = s + NS j of the
+//           N-point inverse DFT are needed: with kx = r + XS m,
+//             Y(s + NS j) = sum_r w_XS^{r j} w_N^{r s} sum_m w_NS^{m s} u(r + XS m),
+//           a fold over m, a twiddle and an XS-point inverse FFT
+//           (w_K = exp(2 pi i / K)); plane j, row ky, column kz of LDS.
+//   z-pass  one row (plane j, ky) per thread, inverse FFT along kz.
+//   y-pass  one column (plane j, z) per thread, inverse FFT along ky, then
+//           Re(P+ P-) with the Nyquist-plane corrections into S_p[y], which
+//           stays in the thread's registers over q.
+// At the end of p the slab of S_p is forward transformed along y and z and
+// stored as T[point][p][x][ky][kz]; k_vhs_xfwd completes the transform
+// along x and accumulates w_p b1_p FFT(S_p) over p into G^.  Same algebra
+// and tables as k_vhs_gain_rr (spectrum Ft, phase table, Nyquist planes from
+// k_vhs_nyq); only the order of the transforms' stages differs.
+// LDS planes are swizzled (column b of row a at nyq_swizzle(b, a)) so the
+// row, column and pencil accesses are all conflict free.
+// ---------------------------------------------------------------------------
+template <int N>
+struct SlabShape {
+  static constexpr int NS = (N == 32) ? 4 : (N == 24) ? 3 : (N == 16) ? 2 : 1;  // slabs per point
+  static constexpr int XS = N / NS;                                              // planes per slab
+  static constexpr int NT = XS * N;                                              // one row per thread
+  static constexpr int planes = XS * N * N;                                      // complex
+  static constexpr int wl = N * N + 2 * XS * N;  // complex: W0 plane, the slab's W1 and W2 rows
+  static constexpr int wpt = wl / NT;            // Nyquist-plane values per thread
+  static_assert(NS * XS == N, "slabs must tile the x axis");
+  static_assert(wl % NT == 0, "Nyquist planes split evenly over the workgroup");
+  static_assert((N * N) % NT == 0, "x-pass pencils split evenly over the workgroup");
+};
+
+// f(integral_constant<int, I>) for I in [B, E), unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <typename T>
+struct GainSlabArgs {
+  const cpx<T>* Ft;     // [npts][n] (1 + i) F transposed to [kx][kz][ky]
+  const cpx<T>* W;      // [npts][K][3][N][N]
+  cpx<T>* Tb;           // [npts][Nrho][N][N][N]: y, z forward transformed slabs of S_p
+  const cpx<T>* ephtab; // [K][3][N] phase factors
+  int nitems, ngroups, ppg, Nrho, M;
+};
+
+// x-pass of slab S (compile-time, so the fold's roots of unity and the
+// twiddles are constants: quarter turns cost nothing at NS = 4)
+template <typename T, int N, int S>
+__device__ __forceinline__ void slab_x_pass(const cpx<T>* __restrict__ Ft, cpx<T>* X, const cpx<T> (*eph)[N]) {
+  using SS = SlabShape<N>;
+  constexpr int NS = SS::NS, XS = SS::XS, NT = SS::NT, NN = N * N;
+  // two pencils per round: their loads are in flight together (4.6 % faster
+  // than one at a time at 32^3 FP64 on MI355X)
+#pragma unroll 2
+  for (int i = 0; i < NN / NT; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + NT * i;  // kz * N + ky
+    const int kz = idx / N, ky = idx - (idx / N) * N;
+    const cpx<T> eyz = eph[1][ky] * eph[2][kz];
+    const T A = eyz.x + eyz.y, B = eyz.x - eyz.y;  // Re E + Im E = ex.x A + ex.y B
+    const cpx<T>* P = Ft + idx;
+    cpx<T> U[XS];
+#pragma unroll
+    for (int r = 0; r < XS; ++r) U[r] = cpx<T>{T(0), T(0)};
+    static_for<0, NS>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      static_for<0, XS>([&](auto rc) {
+        constexpr int r = decltype(rc)::value, kx = r + XS * m;
+        const cpx<T> e = eph[0][kx];
+        const T fac = e.x * A + e.y * B;
+        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(fac * P[kx * NN]);
+      });
+    });
+    static_for<1, XS>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      U[r] = FFT<T, N, +1>::template tw<(r * S) % N>(U[r]);
+    });
+    FFT<T, XS, +1, true>::run(U);
+#pragma unroll
+    for (int j = 0; j < XS; ++j) X[j * NN + ky * N + nyq_swizzle<N>(kz, ky)] = U[j];
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void slab_x_dispatch(int s, const cpx<T>* __restrict__ Ft, cpx<T>* X,
+                                                const cpx<T> (*eph)[N]) {
+  constexpr int NS = SlabShape<N>::NS;
+  if constexpr (NS == 1) {
+    slab_x_pass<T, N, 0>(Ft, X, eph);
+  } else if constexpr (NS == 2) {
+    if (s == 0) slab_x_pass<T, N, 0>(Ft, X, eph); else slab_x_pass<T, N, 1>(Ft, X, eph);
+  } else if constexpr (NS == 3) {
+    if (s == 0) slab_x_pass<T, N, 0>(Ft, X, eph);
+    else if (s == 1) slab_x_pass<T, N, 1>(Ft, X, eph);
+    else slab_x_pass<T, N, 2>(Ft, X, eph);
+  } else {
+    static_assert(NS == 4, "slab counts 1..4");
+    if (s == 0) slab_x_pass<T, N, 0>(Ft, X, eph);
+    else if (s == 1) slab_x_pass<T, N, 1>(Ft, X, eph);
+    else if (s == 2) slab_x_pass<T, N, 2>(Ft, X, eph);
+    else slab_x_pass<T, N, 3>(Ft, X, eph);
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs<T> a) {
+  using SS = SlabShape<N>;
+  constexpr int NS = SS::NS, XS = SS::XS, NN = N * N, n = NN * N;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* W0 = X + SS::planes;  // [y][z]
+  cpx<T>* W1 = W0 + NN;         // [j][z]: plane ky = N/2 at x = s + NS j
+  cpx<T>* W2 = W1 + XS * N;     // [j][y]: plane kz = N/2 at x = s + NS j
+  __shared__ cpx<T> eph[2][3][N];  // phase factors of this (p, q) and the next
+  const int tid = threadIdx.x;
+  const int j = tid / N, c = tid - (tid / N) * N;  // plane, row (z-pass) / column (y-pass)
+  const int K = a.Nrho * a.M;
+  T sacc[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int s = item % NS, pg = item / NS;
+    const int pt = pg / a.ngroups, grp = pg - pt * a.ngroups;
+    const int p0 = grp * a.ppg, p1 = min(p0 + a.ppg, a.Nrho);
+    const int x = s + NS * j;
+    const T sx = (x & 1) ? T(-1) : T(1), sz = (c & 1) ? T(-1) : T(1);
+    const cpx<T>* Ft = a.Ft + static_cast<size_t>(pt) * n;
+    __syncthreads();  // the previous item's readers of eph are done
+    if (tid < 3 * N) (&eph[0][0][0])[tid] = a.ephtab[static_cast<size_t>(p0) * a.M * 3 * N + tid];
+    int it = 0;  // (p, q) iterations of this item: eph[it & 1] holds the current phases
+    for (int p = p0; p < p1; ++p) {
+#pragma unroll
+      for (int y = 0; y < N; ++y) sacc[y] = T(0);
+      for (int q = 0; q < a.M; ++q, ++it) {
+        __syncthreads();  // the previous (p, q)'s readers of X and W are done; eph[it & 1] is in place
+        // this (p, q)'s Nyquist planes: loads issued before the x-pass, stored after it
+        const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
+        cpx<T> wv[SS::wpt];
+#pragma unroll
+        for (int k = 0; k < SS::wpt; ++k) {
+          const int e = tid + k * SS::NT;
+          if (e < NN) {
+            wv[k] = Wq[e];
+          } else {
+            const int e2 = e - NN, pl = e2 / (XS * N), rem = e2 - pl * (XS * N);
+            const int jj = rem / N, cc = rem - jj * N;
+            wv[k] = Wq[(1 + pl) * NN + (s + NS * jj) * N + cc];
+          }
+        }
+        slab_x_dispatch<T, N>(s, Ft, X, eph[it & 1]);
+#pragma unroll
+        for (int k = 0; k < SS::wpt; ++k) W0[tid + k * SS::NT] = wv[k];
+        __syncthreads();
+        {  // z-pass: row (j, ky = c)
+          cpx<T>* row = X + j * NN + c * N;
+          cpx<T> v[N];
+#pragma unroll
+          for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+          FFT<T, N, +1, true>::run(v);
+#pragma unroll
+          for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+        }
+        __syncthreads();
+        // the next (p, q)'s phase factors into the other buffer (read after
+        // the next iteration's first barrier)
+        if (tid < 3 * N) {
+          const int qn = (q + 1 < a.M) ? q + 1 : 0, pn = (q + 1 < a.M) ? p : p + 1;
+          if (pn < p1) (&eph[(it + 1) & 1][0][0])[tid] = a.ephtab[(static_cast<size_t>(pn) * a.M + qn) * 3 * N + tid];
+        }
+        {  // y-pass: column (j, z = c), then the products
+          const cpx<T>* pl = X + j * NN;
+          cpx<T> v[N];
+#pragma unroll
+          for (int ky = 0; ky < N; ++ky) v[ky] = pl[ky * N + nyq_swizzle<N>(c, ky)];
+          FFT<T, N, +1, true>::run(v);
+          const cpx<T> w1 = W1[j * N + c];
+#pragma unroll
+          for (int y = 0; y < N; ++y) {
+            const T sy = (y & 1) ? T(-1) : T(1);
+            const cpx<T> w0 = W0[y * N + c], w2 = W2[j * N + y];
+            const T wr = sx * w0.x + sy * w1.x + sz * w2.x;
+            const T wi = sx * w0.y + sy * w1.y + sz * w2.y;
+            sacc[y] += (v[y].x + wi) * (v[y].y - wr) - wr * wi;
+          }
+        }
+      }
+      // forward transform of the slab of S_p along y (own column) and z
+      {
+        cpx<T> v[N];
+#pragma unroll
+        for (int y = 0; y < N; ++y) v[y] = cpx<T>{sacc[y], T(0)};
+        FFT<T, N, -1, true>::run(v);
+        cpx<T>* pl = X + j * NN;
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) pl[ky * N + nyq_swizzle<N>(c, ky)] = v[ky];
+      }
+      __syncthreads();
+      {
+        cpx<T>* row = X + j * NN + c * N;
+        cpx<T> v[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+        FFT<T, N, -1, true>::run(v);
+#pragma unroll
+        for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+      }
+      __syncthreads();
+      {
+        cpx<T>* Tp = a.Tb + (static_cast<size_t>(pt) * a.Nrho + p) * n;
+        for (int e = tid; e < SS::planes; e += SS::NT) {
+          const int jj = e / NN, rem = e - jj * NN, ky = rem / N, kz = rem - ky * N;
+          Tp[static_cast<size_t>(s + NS * jj) * NN + rem] = X[jj * NN + ky * N + nyq_swizzle<N>(kz, ky)];
+        }
+      }
+    }
+  }
+}
+
+// x-forward transform of the slab kernel's T and the weighted sum over p:
+// G^[pt](k) = sum_p w_p b1_p(k) FFT_x(T_p)(k).  A block of 256 threads takes
+// 128 (point, pencil (ky, kz)) columns; waves 2i and 2i + 1 share 64 of them
+// and keep the lower and the upper half of the kx sums, so the accumulators
+// stay in registers and the half is uniform over each wave.
+template <typename T>
+struct XfwdArgs {
+  const cpx<T>* Tb;     // [npts][Nrho][N][N][N]
+  const T* b1;          // [Nrho][n]
+  const double* wgt;    // [Nrho]
+  cpx<T>* Ghat;         // [npts][n]
+  int npts, Nrho;
+};
+
+template <typename T, int N, int K0>
+__device__ __forceinline__ void xfwd_half(const XfwdArgs<T>& a, int pt, int idx) {
+  constexpr int NN = N * N, n = NN * N, H = N / 2;
+  cpx<T> acc[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) acc[k] = cpx<T>{T(0), T(0)};
+  for (int p = 0; p < a.Nrho; ++p) {
+    const cpx<T>* Tp = a.Tb + (static_cast<size_t>(pt) * a.Nrho + p) * n + idx;
+    cpx<T> v[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) v[x] = Tp[static_cast<size_t>(x) * NN];
+    FFT<T, N, -1>::run(v);
+    const T wp = static_cast<T>(a.wgt[p]);
+    const T* b = a.b1 + static_cast<size_t>(p) * n + idx;
+#pragma unroll
+    for (int k = 0; k < H; ++k) acc[k] = acc[k] + (wp * b[static_cast<size_t>(k + K0) * NN]) * v[k + K0];
+  }
+  cpx<T>* G = a.Ghat + static_cast<size_t>(pt) * n + idx;
+#pragma unroll
+  for (int k = 0; k < H; ++k) G[static_cast<size_t>(k + K0) * NN] = acc[k];
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_vhs_xfwd(XfwdArgs<T> a) {
+  constexpr int NN = N * N;
+  const int w = threadIdx.x / 64, lane = threadIdx.x - (threadIdx.x / 64) * 64;
+  const long col = static_cast<long>(blockIdx.x) * 128 + (w >> 1) * 64 + lane;
+  if (col >= static_cast<long>(a.npts) * NN) return;
+  const int pt = static_cast<int>(col / NN), idx = static_cast<int>(col - static_cast<long>(pt) * NN);
+  if (w & 1)
+    xfwd_half<T, N, N / 2>(a, pt, idx);
+  else
+    xfwd_half<T, N, 0>(a, pt, idx);
+}
+
 template <typename T>
 struct FinArgs {
   const cpx<T>* Ghat;  // [npts*ngroups][n]
@@ -1119,12 +1393,15 @@ struct VhsHandle {
   int algorithm = 0;     // 0 automatic, 1 multi-pass, 2 register-resident (Nyquist planes
                          // from k_vhs_nyq), 3 register-resident with in-kernel Nyquist planes,
                          // 4 register-resident with one pencil per thread (Nyquist planes
-                         // from k_vhs_nyq), 5 as 2 with wave-local in-plane transposes
+                         // from k_vhs_nyq), 5 as 2 with wave-local in-plane transposes,
+                         // 6 slab kernel (x-slabs in LDS, any precision, Nv 16/24/32)
   int last_algorithm = 0;
   void* Wny = nullptr;   // Nyquist planes [cap_pts][K][3][N][N]
   void* Ft = nullptr;    // spectrum transposed to [kx][kz][ky] [cap_pts][n]
   size_t cap_ft = 0;
   size_t cap_wny = 0;
+  void* Tb = nullptr;    // slab kernel's forward-transformed slabs [cap_pts][Nrho][n]
+  size_t cap_tb = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
   // set for the duration of an allocation-free call: the workspace pointers
   // above then point into the device scratch array and growing any of them
@@ -1148,6 +1425,9 @@ static void free_ws(VhsHandle* h) {
   (void)hipFree(h->Wny);
   h->Wny = nullptr;
   h->cap_wny = 0;
+  (void)hipFree(h->Tb);
+  h->Tb = nullptr;
+  h->cap_tb = 0;
   (void)hipFree(h->Ft);
   h->Ft = nullptr;
   h->cap_ft = 0;
@@ -1210,13 +1490,20 @@ constexpr bool rr1_available() {
 }
 static bool rr2_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 32); }
 static bool rr1_ok(const VhsHandle* h) { return h->prec == 32 && (h->N == 16 || h->N == 24); }
+static bool slab_ok(const VhsHandle* h) { return h->N == 16 || h->N == 24 || h->N == 32; }
+template <int N>
+constexpr bool slab_available() {
+  return N == 16 || N == 24 || N == 32;
+}
 
 // gain algorithm a launch uses: 1 multi-pass, 2/3/5 two pencils per thread,
-// 4 one pencil per thread.  The automatic choice takes the one-pencil kernel
-// where the field fits in LDS (Nv 16, 24: 1.8x the two-pencil kernel at 16^3
-// on MI355X), the two-pencil kernel with wave-local in-plane transposes at
-// Nv 32 (61.0 against 62.0 ms per 4096-point launch with the workgroup-wide
-// ones, bitwise the same Q), the multi-pass one otherwise.
+// 4 one pencil per thread, 6 slab kernel.  The automatic choice takes, in
+// FP32, the one-pencil kernel where the field fits in LDS (Nv 16, 24: 1.8x
+// the two-pencil kernel at 16^3 on MI355X) and the two-pencil kernel with
+// wave-local in-plane transposes at Nv 32 (61.0 against 62.0 ms per
+// 4096-point launch with the workgroup-wide ones, bitwise the same Q); in
+// FP64 the slab kernel at Nv 16, 24, 32 (2.7x, 1.8x, 1.6x the multi-pass
+// kernel at 32^3, 24^3, 16^3); the multi-pass one otherwise.
 static int rr_kind(const VhsHandle* h) {
   switch (h->algorithm) {
     case 1: return 1;
@@ -1224,7 +1511,8 @@ static int rr_kind(const VhsHandle* h) {
     case 3: return rr2_ok(h) ? 3 : 1;
     case 4: return rr1_ok(h) ? 4 : 1;
     case 5: return rr2_ok(h) ? 5 : 1;
-    default: return rr1_ok(h) ? 4 : (rr2_ok(h) ? 5 : 1);
+    case 6: return slab_ok(h) ? 6 : 1;
+    default: return rr1_ok(h) ? 4 : (rr2_ok(h) ? 5 : (slab_ok(h) ? 6 : 1));
   }
 }
 
@@ -1332,6 +1620,80 @@ static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t item
   }
 }
 
+// The slab kernel's items of one point read the same spectrum once per
+// (p, q); at least 4 radial groups keep 16 of them (4 slabs x 4 groups) per
+// point, so the workgroups of an XCD share the spectra of two points in its L2
+// (32^3 FP64 on MI355X: 229.7 ms per 2048 points with 1 group, 157.4 with 2,
+// 156.1 with 4; 16^3: 2.3 against 1.8 ms per 1024 points)
+constexpr int kSlabMinGroups = 4;
+
+// slab gain: Nyquist planes, the slab kernel over (point, group, slab) items,
+// then the x-forward sum into G^ (one partial per point)
+template <typename T, int N>
+static int run_gain_slab(VhsHandle* h, int npts, int groups, int ppg, hipStream_t st) {
+  if constexpr (slab_available<N>()) {
+    using SS = SlabShape<N>;
+    constexpr int NN = N * N;
+    constexpr size_t n = static_cast<size_t>(NN) * N;
+    const int K = h->Nrho * h->M;
+    h->last_algorithm = 6;
+    const size_t wbytes = static_cast<size_t>(npts) * K * 3 * NN * sizeof(cpx<T>);
+    if (wbytes > h->cap_wny) {
+      if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: Nyquist planes exceed their scratch share");
+      (void)hipFree(h->Wny);
+      h->Wny = nullptr;
+      h->cap_wny = 0;
+      DGFS_HIP_TRY(hipMalloc(&h->Wny, wbytes));
+      h->cap_wny = wbytes;
+    }
+    const size_t tbytes = static_cast<size_t>(npts) * h->Nrho * n * sizeof(cpx<T>);
+    if (tbytes > h->cap_tb) {
+      if (h->no_alloc) return fail(DGFS_ERR_ALLOC, "allocation-free call: slab transforms exceed their scratch share");
+      (void)hipFree(h->Tb);
+      h->Tb = nullptr;
+      h->cap_tb = 0;
+      DGFS_HIP_TRY(hipMalloc(&h->Tb, tbytes));
+      h->cap_tb = tbytes;
+    }
+    const long nplanes = static_cast<long>(npts) * K * 3;
+    NyqArgs<T> na{static_cast<const cpx<T>*>(h->F), static_cast<const cpx<T>*>(h->Ft),
+                  static_cast<cpx<T>*>(h->Wny), static_cast<const cpx<T>*>(h->ephtab), nplanes, h->Nrho, h->M};
+    const long waves = (nplanes + NyqShape<N>::PW - 1) / NyqShape<N>::PW;
+    {
+      KTimerT<VhsHandle> t4(h, 4, st);
+      hipLaunchKernelGGL((k_vhs_nyq<T, N>), dim3(static_cast<unsigned>(std::min<long>(waves, 16L * cu_count()))),
+                         dim3(64), 0, st, na);
+      DGFS_LAUNCH_CHECK("vhs_nyq");
+      t4.stop();
+    }
+    const size_t items = static_cast<size_t>(npts) * groups * SS::NS;
+    const size_t lds = (static_cast<size_t>(SS::planes) + SS::wl) * sizeof(cpx<T>);
+    const void* kfn = reinterpret_cast<const void*>(k_vhs_gain_slab<T, N>);
+    DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, SS::NT, lds) != hipSuccess || occ < 1) occ = 1;
+    const int grid = static_cast<int>(std::min<size_t>(items, static_cast<size_t>(occ) * cu_count()));
+    GainSlabArgs<T> ga{static_cast<const cpx<T>*>(h->Ft), static_cast<const cpx<T>*>(h->Wny),
+                       static_cast<cpx<T>*>(h->Tb), static_cast<const cpx<T>*>(h->ephtab),
+                       static_cast<int>(items), groups, ppg, h->Nrho, h->M};
+    KTimerT<VhsHandle> t1(h, 1, st);
+    hipLaunchKernelGGL((k_vhs_gain_slab<T, N>), dim3(grid), dim3(SS::NT), lds, st, ga);
+    DGFS_LAUNCH_CHECK("vhs_gain_slab");
+    XfwdArgs<T> xa{static_cast<const cpx<T>*>(h->Tb), static_cast<const T*>(h->b1), h->wgt,
+                   static_cast<cpx<T>*>(h->Ghat), npts, h->Nrho};
+    const long cols = static_cast<long>(npts) * NN;
+    hipLaunchKernelGGL((k_vhs_xfwd<T, N>), dim3(static_cast<unsigned>((cols + 127) / 128)), dim3(256), 0, st, xa);
+    DGFS_LAUNCH_CHECK("vhs_xfwd");
+    t1.stop();
+    h->threads = SS::NT;
+    h->last_slots = grid;
+    return DGFS_OK;
+  } else {
+    (void)h; (void)npts; (void)groups; (void)ppg; (void)st;
+    return fail(DGFS_ERR_UNSUPPORTED, "slab gain kernel not built for Nv=%d", N);
+  }
+}
+
 // radial groups per point: enough (point, group) items for two per slot
 // radial groups per point.  Automatic: the split whose (point, group) items
 // fill the persistent grid in the fewest rounds of radial work: rounds of
@@ -1340,17 +1702,18 @@ static int run_gain_rr1(VhsHandle* h, int npts, int groups, int ppg, size_t item
 // partial G^).  A batch that is not a multiple of the grid then splits finer
 // rather than leaving most of its last round idle (3600 points on 256 CUs:
 // 2 groups, 14.5 rounds of full items instead of 15).
-static int pick_groups(const VhsHandle* h, int npts, int slots) {
+static int pick_groups(const VhsHandle* h, int npts, int slots, int min_groups = 1) {
   int groups = h->groups_override;
   if (groups <= 0) {
     const long P = std::max(npts, 1), S = std::max(slots, 1);
+    const int g0 = std::max(1, std::min(min_groups, h->Nrho));
     double best = 0.0;
-    groups = 1;
-    for (int g = 1; g <= h->Nrho; ++g) {
+    groups = g0;
+    for (int g = g0; g <= h->Nrho; ++g) {
       const int ppg = (h->Nrho + g - 1) / g, ge = (h->Nrho + ppg - 1) / ppg;
       if (ge != g) continue;  // the same split as a smaller g
       const double cost = static_cast<double>((P * ge + S - 1) / S) * ppg * (1.0 + 0.01 * (ge - 1));
-      if (g == 1 || cost < best) {
+      if (g == g0 || cost < best) {
         best = cost;
         groups = ge;
       }
@@ -1431,6 +1794,16 @@ static int run_chunk(VhsHandle* h, const double* fc, double* Qc, int npts, hipSt
   const bool rr = use_rr(h);
   int slots = rr ? cu_count() : h->cap_slots;
   if (!rr && h->slots_override > 0) slots = std::min(slots, h->slots_override);
+  if constexpr (slab_available<N>()) {
+    if (rr_kind(h) == 6) {  // items are (point, group, slab): each slab counts as a point
+      const int groups = pick_groups(h, npts * SlabShape<N>::NS, slots, kSlabMinGroups);
+      GainArgs<T> ga;
+      int rc = stage_prep<T, N>(h, fc, npts, groups, slots, true, &ga, st);
+      if (!rc) rc = run_gain_slab<T, N>(h, npts, groups, ga.ppg, st);
+      if (!rc) rc = stage_finish<T, N>(h, fc, Qc, npts, 1, st);
+      return rc;
+    }
+  }
   const int groups = pick_groups(h, npts, slots);
   GainArgs<T> ga;
   int rc = stage_prep<T, N>(h, fc, npts, groups, slots, rr, &ga, st);
@@ -1474,11 +1847,13 @@ static int ensure_ws(VhsHandle* h, int pts) {
   return DGFS_OK;
 }
 
-// points per launch: F + G^ (worst case one partial per radial node) + f + Q
+// points per launch: F + G^ (worst case one partial per radial node) + f + Q,
+// plus the register-resident and slab kernels' spectra, Nyquist planes and T
 // kept under 48 GiB of the 288 GB of HBM; launches are balanced in size
 static int chunk_points(const VhsHandle* h, int total) {
   double per = static_cast<double>(h->n) * (2.0 * h->tsize * (1 + h->Nrho) + 16.0);
   if (use_rr(h)) per += 3.0 * h->N * h->N * h->Nrho * h->M * 2.0 * h->tsize + 2.0 * h->tsize * h->n;
+  if (rr_kind(h) == 6) per += 2.0 * h->tsize * h->n * h->Nrho;  // the slab kernel's T
   const double budget = 48.0 * 1024.0 * 1024.0 * 1024.0;
   const int cap = std::max(1, static_cast<int>(budget / per));
   if (total <= cap) return std::max(total, 1);
@@ -1577,8 +1952,8 @@ static size_t align_up(size_t b) { return (b + kScratchAlign - 1) / kScratchAlig
 struct ScratchPlan {
   int pts = 0, groups = 0, slots = 0;
   size_t items = 0;  // gain partial buffers the Ghat share holds
-  size_t z = 0, s = 0, w = 0, F = 0, Ghat = 0, Ft = 0, Wny = 0;
-  size_t total() const { return z + s + w + F + Ghat + Ft + Wny; }
+  size_t z = 0, s = 0, w = 0, F = 0, Ghat = 0, Ft = 0, Wny = 0, Tb = 0;
+  size_t total() const { return z + s + w + F + Ghat + Ft + Wny + Tb; }
 };
 
 template <typename T, int N>
@@ -1591,6 +1966,12 @@ static ScratchPlan plan_scratch(const VhsHandle* h, int pts, int slots) {
   p.pts = pts;
   p.slots = slots;
   p.groups = pick_groups(h, pts, gain_slots);
+  if constexpr (slab_available<N>()) {
+    if (rr_kind(h) == 6) {
+      p.groups = pick_groups(h, pts * SlabShape<N>::NS, gain_slots, kSlabMinGroups);
+      p.Tb = align_up(static_cast<size_t>(pts) * h->Nrho * n * sizeof(cpx<T>));
+    }
+  }
   p.z = align_up(static_cast<size_t>(slots) * n * sizeof(cpx<T>));
   if (!rr) {  // the multi-pass gain's per-slot field and Nyquist planes
     p.s = align_up(static_cast<size_t>(slots) * n * sizeof(T));
@@ -1611,14 +1992,14 @@ static ScratchPlan plan_scratch(const VhsHandle* h, int pts, int slots) {
 // array and put back however the call ends
 struct NoAllocScope {
   VhsHandle* h;
-  void *F, *Ghat, *Z, *S, *W, *Wny, *Ft;
+  void *F, *Ghat, *Z, *S, *W, *Wny, *Ft, *Tb;
   double *fc, *Qc;
-  size_t cap_pts, cap_items, cap_ft, cap_wny;
+  size_t cap_pts, cap_items, cap_ft, cap_wny, cap_tb;
   int cap_slots, timing;
   explicit NoAllocScope(VhsHandle* hh)
-      : h(hh), F(hh->F), Ghat(hh->Ghat), Z(hh->Z), S(hh->S), W(hh->W), Wny(hh->Wny), Ft(hh->Ft), fc(hh->fc),
-        Qc(hh->Qc), cap_pts(hh->cap_pts), cap_items(hh->cap_items), cap_ft(hh->cap_ft), cap_wny(hh->cap_wny),
-        cap_slots(hh->cap_slots), timing(hh->timing) {
+      : h(hh), F(hh->F), Ghat(hh->Ghat), Z(hh->Z), S(hh->S), W(hh->W), Wny(hh->Wny), Ft(hh->Ft), Tb(hh->Tb),
+        fc(hh->fc), Qc(hh->Qc), cap_pts(hh->cap_pts), cap_items(hh->cap_items), cap_ft(hh->cap_ft),
+        cap_wny(hh->cap_wny), cap_tb(hh->cap_tb), cap_slots(hh->cap_slots), timing(hh->timing) {
     h->no_alloc = 1;
     h->timing = 0;  // the event pool may grow; no timing in this mode
   }
@@ -1636,17 +2017,19 @@ struct NoAllocScope {
     h->Ghat = take(p.Ghat);
     h->Ft = take(p.Ft);
     h->Wny = take(p.Wny);
+    h->Tb = take(p.Tb);
     h->fc = h->Qc = nullptr;  // the contiguous form reads f and writes Q in place
     h->cap_slots = p.slots;
     h->cap_pts = static_cast<size_t>(p.pts);
     h->cap_items = p.items;
     h->cap_ft = p.Ft;
     h->cap_wny = p.Wny;
+    h->cap_tb = p.Tb;
   }
   ~NoAllocScope() {
-    h->F = F; h->Ghat = Ghat; h->Z = Z; h->S = S; h->W = W; h->Wny = Wny; h->Ft = Ft;
+    h->F = F; h->Ghat = Ghat; h->Z = Z; h->S = S; h->W = W; h->Wny = Wny; h->Ft = Ft; h->Tb = Tb;
     h->fc = fc; h->Qc = Qc;
-    h->cap_pts = cap_pts; h->cap_items = cap_items; h->cap_ft = cap_ft; h->cap_wny = cap_wny;
+    h->cap_pts = cap_pts; h->cap_items = cap_items; h->cap_ft = cap_ft; h->cap_wny = cap_wny; h->cap_tb = cap_tb;
     h->cap_slots = cap_slots;
     h->timing = timing;
     h->no_alloc = 0;
@@ -2010,11 +2393,13 @@ extern "C" int dgfs_vhs_get_timing(void* p, double* ms, int* counts) {
 extern "C" int dgfs_vhs_set_algorithm(void* p, int algorithm) {
   VhsHandle* h = as_handle(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VHS handle");
-  if (algorithm < 0 || algorithm > 5) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0 to 5");
+  if (algorithm < 0 || algorithm > 6) return fail(DGFS_ERR_INVALID_VALUE, "algorithm must be 0 to 6");
   if ((algorithm == 2 || algorithm == 3 || algorithm == 5) && !rr2_ok(h))
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (two pencils per thread): FP32 and Nv in {16, 32} only");
   if (algorithm == 4 && !rr1_ok(h))
     return fail(DGFS_ERR_UNSUPPORTED, "register-resident kernel (one pencil per thread): FP32 and Nv in {16, 24} only");
+  if (algorithm == 6 && !slab_ok(h))
+    return fail(DGFS_ERR_UNSUPPORTED, "slab kernel: Nv in {16, 24, 32} only");
   h->algorithm = algorithm;
   return DGFS_OK;
 }

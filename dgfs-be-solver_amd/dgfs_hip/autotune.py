@@ -3,9 +3,10 @@ two-species VSS).
 
 For a given operator (velocity grid, precision, K = M Nrho) and batch size it
 times every candidate of
-  * the gain algorithm: multi-pass (any grid) or register-resident (FP32:
+  * the gain algorithm: multi-pass (any grid), register-resident (FP32:
     two pencils per thread at Nv 16 / 32 with the Nyquist planes from a
-    separate kernel or inside the kernel; one pencil per thread at Nv 16 / 24);
+    separate kernel or inside the kernel; one pencil per thread at Nv 16 / 24)
+    or the slab kernel (FP64 at Nv 16 / 24 / 32);
   * the radial-group split (work items per point: 1, 2, 4, ..., Nrho);
   * the persistent grid of the multi-pass kernels (1, 2, 4 workgroups per CU,
     or the occupancy limit),
@@ -72,6 +73,8 @@ class Autotuner:
             algos[:0] = [4]
         if op.precision == 32 and op.Nv in (16, 32):
             algos[:0] = [2, 3, 5]
+        if op.precision == 64 and op.Nv in (16, 24, 32):
+            algos[:0] = [6]
         groups = [0]
         g = 1
         while g <= op.Nrho:

@@ -114,7 +114,8 @@ class VHSOperator:
         """0 automatic, 1 multi-pass, 2 register-resident (FP32, Nv 16/32),
         3 register-resident with in-kernel Nyquist planes, 4 register-resident
         with one pencil per thread (FP32, Nv 16/24), 5 as 2 with wave-local
-        in-plane transposes."""
+        in-plane transposes, 6 slab kernel (x-slabs of the field in LDS, FP64
+        or FP32, Nv 16/24/32)."""
         _lib.call("dgfs_vhs_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):

@@ -165,6 +165,10 @@ def test_autotune_candidates_for_both_operators():
     vhs = types.SimpleNamespace(Nv=32, precision=32, K=192, project=True, Nrho=16)
     assert sorted({x["algorithm"] for x in Autotuner.candidates(vhs, 256)}) == [1, 2, 3, 5]
     assert _key(vhs, 4096) == "vhs-N32-p32-K192-proj1-pts4096"
+    vhs.precision = 64
+    c64 = Autotuner.candidates(vhs, 256)
+    assert sorted({x["algorithm"] for x in c64}) == [1, 6]
+    assert all(x["slots"] == 0 for x in c64 if x["algorithm"] == 6)
 
 
 def test_autotune_cache_is_keyed_by_platform_and_drops_stale_entries(tmp_path, monkeypatch):
@@ -203,5 +207,5 @@ def test_autotune_cache_is_keyed_by_platform_and_drops_stale_entries(tmp_path, m
     monkeypatch.setattr(torch.cuda, "synchronize", lambda *a: None)
     op = Op(**vars(vhs))
     best, _ = tuner.tune(op, 0, 0, 64, stream=0)
-    assert best["algorithm"] in (1, 2, 3, 4, 5)
+    assert best["algorithm"] in (1, 2, 3, 4, 5, 6)
     assert tuner._cache[key]["best"] == best
