@@ -906,7 +906,11 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
 // ---------------------------------------------------------------------------
 template <int N>
 struct SlabShape {
-  static constexpr int NS = (N == 32) ? 4 : (N == 24) ? 3 : (N == 16) ? 2 : 1;  // slabs per point
+  // slabs per point: as few as the FP64 planes fit in LDS, and a slab size
+  // whose FFT is cheap (MI355X, FP64: at 24^3 3 slabs of 8 planes 36.9k Q/s
+  // against 22.8k with 2 of 12 and 23.1k with 4 of 6; at 16^3 one slab of 16
+  // planes 812k against 537k with 2 of 8)
+  static constexpr int NS = (N == 32) ? 4 : (N == 24) ? 3 : 1;
   static constexpr int XS = N / NS;                                              // planes per slab
   static constexpr int NT = XS * N;                                              // one row per thread
   static constexpr int planes = XS * N * N;                                      // complex
@@ -1502,8 +1506,8 @@ constexpr bool slab_available() {
 // the two-pencil kernel at 16^3 on MI355X) and the two-pencil kernel with
 // wave-local in-plane transposes at Nv 32 (61.0 against 62.0 ms per
 // 4096-point launch with the workgroup-wide ones, bitwise the same Q); in
-// FP64 the slab kernel at Nv 16, 24, 32 (2.7x, 1.8x, 1.6x the multi-pass
-// kernel at 32^3, 24^3, 16^3); the multi-pass one otherwise.
+// FP64 the slab kernel at Nv 16, 24, 32 (2.3x, 1.8x, 2.7x the multi-pass
+// kernel at 16^3, 24^3, 32^3); the multi-pass one otherwise.
 static int rr_kind(const VhsHandle* h) {
   switch (h->algorithm) {
     case 1: return 1;

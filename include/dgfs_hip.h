@@ -129,7 +129,7 @@ int dgfs_vhs_set_slots(void* handle, int nslots);
  * or 24; Nyquist planes from the separate kernel), 5 as 2 with the in-plane
  * transposes wave-local (no workgroup barrier between the z, y passes), 6 slab
  * kernel (FP64 or FP32, Nv 16, 24 or 32: a workgroup holds N/NS x-planes of
- * the inverse transforms in LDS, NS = 2, 3, 4 slabs per point, with the
+ * the inverse transforms in LDS, NS = 1, 3, 4 slabs per point, with the
  * x-transform decimated to the slab's planes; a second kernel completes the
  * forward transform along x and sums over the radial nodes).  The automatic
  * choice is, in FP32, 4 at Nv 16 and 24 and 5 at Nv 32; in FP64, 6 at Nv 16,
