@@ -1026,23 +1026,35 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
       for (int y = 0; y < N; ++y) sacc[y] = T(0);
       for (int q = 0; q < a.M; ++q, ++it) {
         __syncthreads();  // the previous (p, q)'s readers of X and W are done; eph[it & 1] is in place
-        // this (p, q)'s Nyquist planes: loads issued before the x-pass, stored after it
+        // this (p, q)'s Nyquist planes, in flight during the x-pass
         const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
-        cpx<T> wv[SS::wpt];
+        auto wsrc = [&](int e) -> const cpx<T>* {
+          if (e < NN) return Wq + e;
+          const int e2 = e - NN, pl = e2 / (XS * N), rem = e2 - pl * (XS * N);
+          const int jj = rem / N, cc = rem - jj * N;
+          return Wq + (1 + pl) * NN + (s + NS * jj) * N + cc;
+        };
+        if constexpr (sizeof(cpx<T>) == 16) {
+          // FP64: the load unit writes them into LDS itself (global_load_lds,
+          // lane i of a wave to the wave's base + 16 i), so they hold no
+          // registers across the x-pass
+          static_assert(SS::NT % 64 == 0, "whole waves");
+          const int wbase = tid & ~63;
 #pragma unroll
-        for (int k = 0; k < SS::wpt; ++k) {
-          const int e = tid + k * SS::NT;
-          if (e < NN) {
-            wv[k] = Wq[e];
-          } else {
-            const int e2 = e - NN, pl = e2 / (XS * N), rem = e2 - pl * (XS * N);
-            const int jj = rem / N, cc = rem - jj * N;
-            wv[k] = Wq[(1 + pl) * NN + (s + NS * jj) * N + cc];
-          }
+          for (int k = 0; k < SS::wpt; ++k)
+            __builtin_amdgcn_global_load_lds(
+                static_cast<const void*>(wsrc(tid + k * SS::NT)),
+                (void __attribute__((address_space(3)))*)(W0 + wbase + k * SS::NT), 16, 0, 0);
+          slab_x_dispatch<T, N>(s, Ft, X, eph[it & 1]);
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        } else {
+          cpx<T> wv[SS::wpt];
+#pragma unroll
+          for (int k = 0; k < SS::wpt; ++k) wv[k] = *wsrc(tid + k * SS::NT);
+          slab_x_dispatch<T, N>(s, Ft, X, eph[it & 1]);
+#pragma unroll
+          for (int k = 0; k < SS::wpt; ++k) W0[tid + k * SS::NT] = wv[k];
         }
-        slab_x_dispatch<T, N>(s, Ft, X, eph[it & 1]);
-#pragma unroll
-        for (int k = 0; k < SS::wpt; ++k) W0[tid + k * SS::NT] = wv[k];
         __syncthreads();
         {  // z-pass: row (j, ky = c)
           cpx<T>* row = X + j * NN + c * N;
