@@ -386,6 +386,28 @@ struct RRLds {
 // its own LDS operations only
 __device__ __forceinline__ void lds_wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// Copy NCH 16-byte chunks from global memory to LDS with the load unit
+// (global_load_lds_dwordx4: lane i of a wave writes the wave's LDS base +
+// 16 i, so chunk c goes to dst + 16 c), without passing through registers.
+// The copy is in flight until the issuing wave waits for its vector memory
+// counter: call lds_copy_wait() and then a workgroup barrier before any wave
+// reads dst.  src and dst 16-byte aligned; NT a multiple of 64.
+template <int NT, int NCH>
+__device__ __forceinline__ void lds_copy16(const void* src, void* dst) {
+  static_assert(NT % 64 == 0, "whole waves");
+  const int tid = threadIdx.x, wbase = tid & ~63;
+#pragma unroll
+  for (int k = 0; k < (NCH + NT - 1) / NT; ++k) {
+    const int c = tid + k * NT;
+    if (NCH % NT == 0 || c < NCH)
+      __builtin_amdgcn_global_load_lds(static_cast<const char*>(src) + 16 * static_cast<size_t>(c),
+                                       (void __attribute__((address_space(3)))*)(static_cast<char*>(dst) +
+                                                                                 16 * (wbase + k * NT)),
+                                       16, 0, 0);
+  }
+}
+__device__ __forceinline__ void lds_copy_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // In-plane transpose of the pencils of plane group g (lane l of the group)
 // through LDS.  Workgroup form (WL false): plane groups [half*G/2,
 // (half+1)*G/2) take turns on G/2 padded planes between workgroup barriers.
@@ -681,20 +703,19 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
           }
           __syncthreads();
         } else {
-          // this (p, q)'s Nyquist planes into LDS (read after later barriers)
+          // this (p, q)'s Nyquist planes into LDS by the load unit, in
+          // flight during the z, y transforms (read after later barriers;
+          // 67.9 -> 66.7 ms per 4096-point launch at 32^3 on MI355X against
+          // loading them through registers before the transforms)
           const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
-          static_assert((3 * NN) % NT == 0, "Nyquist planes split evenly over the workgroup");
-          cpx<T> wv[3 * NN / NT];
-#pragma unroll
-          for (int k = 0; k < 3 * NN / NT; ++k) wv[k] = Wq[tid + k * NT];
-#pragma unroll
-          for (int k = 0; k < 3 * NN / NT; ++k) Wl[tid + k * NT] = wv[k];
-          __syncthreads();
+          lds_copy16<NT, 3 * NN * static_cast<int>(sizeof(cpx<T>)) / 16>(Wq, Wl);
+          __syncthreads();  // eph is in place
         }
         rr_zy_inverse<T, N, G, WL>(v0, Ft, Xb, eph, g, g, l);
         rr_zy_inverse<T, N, G, WL>(v1, Ft, Xb, eph, g + G, g, l);
         T val[N];
         cpx<T> u[N];
+        if constexpr (!INL) lds_copy_wait();  // the Nyquist planes have landed (read after the barrier)
         // target slot 0 (y = g): both sub-rounds
         __syncthreads();
         rr_x_write<T, N, G>(v0, Xb, 0, g, l);
@@ -790,6 +811,9 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
           if (tid >= 2 * N) eph[3][tid - 2 * N] = cpx<T>{e.x + e.y, e.x - e.y};
         }
         {
+          // this (p, q)'s Nyquist planes through registers: their loads overlap
+          // the z-pencil loads below (a copy by the load unit, waited for before
+          // the products, measured 2.5 % slower here on MI355X)
           const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
           static_assert((3 * NN) % NT == 0, "Nyquist planes split evenly over the workgroup");
           cpx<T> wv[3 * NN / NT];
@@ -1035,9 +1059,9 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
           return Wq + (1 + pl) * NN + (s + NS * jj) * N + cc;
         };
         if constexpr (sizeof(cpx<T>) == 16) {
-          // FP64: the load unit writes them into LDS itself (global_load_lds,
-          // lane i of a wave to the wave's base + 16 i), so they hold no
-          // registers across the x-pass
+          // FP64: the load unit writes them into LDS itself (one gathered
+          // 16-byte value per lane, lane i of a wave to the wave's base + 16 i),
+          // so they hold no registers across the x-pass
           static_assert(SS::NT % 64 == 0, "whole waves");
           const int wbase = tid & ~63;
 #pragma unroll
@@ -1046,7 +1070,7 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
                 static_cast<const void*>(wsrc(tid + k * SS::NT)),
                 (void __attribute__((address_space(3)))*)(W0 + wbase + k * SS::NT), 16, 0, 0);
           slab_x_dispatch<T, N>(s, Ft, X, eph[it & 1]);
-          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          lds_copy_wait();
         } else {
           cpx<T> wv[SS::wpt];
 #pragma unroll
