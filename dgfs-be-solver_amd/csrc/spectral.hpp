@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -69,6 +70,28 @@ __device__ __forceinline__ void pencil_pass(Load load, Epi epi) {
     epi(idx, v);
   }
   __syncthreads();
+}
+
+// column c of row r in an N x N tile: XOR swizzle for power-of-two N (both
+// row and column accesses conflict free), (c + r) mod N otherwise (a
+// permutation of the row as well)
+template <int N>
+__device__ __forceinline__ int nyq_swizzle(int c, int r) {
+  if constexpr ((N & (N - 1)) == 0) {
+    return c ^ r;
+  } else {
+    const int s = c + r;
+    return s >= N ? s - N : s;
+  }
+}
+
+// f(integral_constant<int, I>) for I in [B, E), unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
 }
 
 // Consecutive workgroups of a launch go round-robin to the 8 XCDs of an

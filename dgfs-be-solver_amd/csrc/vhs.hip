@@ -231,18 +231,6 @@ struct NyqShape {
   static constexpr int PSTR = N * N + 16;  // LDS plane stride (planes on distinct banks)
 };
 
-// column c of row r in an N x N tile: XOR swizzle for power-of-two N (both
-// row and column accesses conflict free), (c + r) mod N otherwise (a
-// permutation of the row as well)
-template <int N>
-__device__ __forceinline__ int nyq_swizzle(int c, int r) {
-  if constexpr ((N & (N - 1)) == 0) {
-    return c ^ r;
-  } else {
-    const int s = c + r;
-    return s >= N ? s - N : s;
-  }
-}
 
 template <typename T, int N>
 __global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
@@ -945,14 +933,6 @@ struct SlabShape {
   static_assert((N * N) % NT == 0, "x-pass pencils split evenly over the workgroup");
 };
 
-// f(integral_constant<int, I>) for I in [B, E), unrolled at compile time
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
 
 template <typename T>
 struct GainSlabArgs {

@@ -778,6 +778,268 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
 // ---------------------------------------------------------------------------
 // finalize: one item = (species p, point)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Slab gain for isotropic scattering (eta = 0 for every pair), the FP64 form
+// at Nv 16 / 24.  A workgroup owns a (point, job, radial group, slab) item;
+// slab s holds the XS = N/NS planes x = s + NS j of the two inverse
+// transforms t3 = IFFT(E1 F_p) and t4 = IFFT(E2 F_q) in LDS.  Per direction
+// product:
+//   x-pass  pencils (ky, kz) of F_p and F_q times their x phase factors,
+//           folded over kx = r + XS m with exp(2 pi i m s / NS), the yz phase
+//           and the twiddle exp(2 pi i r s / N) applied after the fold, an
+//           XS-point inverse FFT: plane j, row ky, column kz of the t3 and
+//           the t4 slab;
+//   z-pass  one row (plane j, ky) of each slab per thread;
+//   y-pass  one column (plane j, z) of each: t3 goes back to its own column,
+//           t4 stays in registers, and S_x[y] += t3 t4 (complex, in
+//           registers over the job's directions).
+// At the end of radial node x the slab of S_x is forward transformed along
+// y and z into T[job][point][x]; k_vss_xfwd completes the transform along x
+// and forms G^ = sum_x w_x b(x) FFT(S_x) for both outputs of the job.  Same
+// jobs, tables and sums as k_vss_gain<T, N, true>; only the order of the
+// transforms' stages differs.
+// ---------------------------------------------------------------------------
+template <int N>
+struct VssSlabShape {
+  // slabs per point: the two FP64 slabs fill the LDS at NS = 3 for 24^3;
+  // 16^3 fits whole
+  static constexpr int NS = (N == 24) ? 3 : 1;
+  static constexpr int XS = N / NS;
+  static constexpr int NT = XS * N;  // one row per thread
+  static constexpr int planes = XS * N * N;  // complex, per slab
+  static_assert(NS * XS == N, "slabs must tile the x axis");
+  static_assert((N * N) % NT == 0, "x-pass pencils split evenly over the workgroup");
+};
+
+template <typename T>
+struct VssSlabArgs {
+  VssGainArgs<T> g;  // spectra, jobs, tables, phase inputs, item split
+  cpx<T>* Tb;        // [njobs][npts][Nrho][n]
+  int njobs;
+};
+
+template <typename T, int N, int S>
+__device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cpx<T>* X, const cpx<T> (*e)[N]) {
+  using SS = VssSlabShape<N>;
+  constexpr int NS = SS::NS, XS = SS::XS, NT = SS::NT, NN = N * N;
+#pragma unroll 1
+  for (int i = 0; i < NN / NT; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + NT * i;  // ky * N + kz
+    const int ky = idx / N, kz = idx - (idx / N) * N;
+    const cpx<T> eyz = e[1][ky] * e[2][kz];
+    cpx<T> U[XS];
+#pragma unroll
+    for (int r = 0; r < XS; ++r) U[r] = cpx<T>{T(0), T(0)};
+    static_for<0, NS>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      static_for<0, XS>([&](auto rc) {
+        constexpr int r = decltype(rc)::value, kx = r + XS * m;
+        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(F[kx * NN + idx] * e[0][kx]);
+      });
+    });
+    static_for<0, XS>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      U[r] = FFT<T, N, +1>::template tw<(r * S) % N>(U[r] * eyz);
+    });
+    FFT<T, XS, +1, true>::run(U);
+#pragma unroll
+    for (int j = 0; j < XS; ++j) X[j * NN + ky * N + nyq_swizzle<N>(kz, ky)] = U[j];
+  }
+}
+
+// x-pass of both spectra (one after the other, so that one spectrum's loads
+// are in flight at a time)
+template <typename T, int N>
+__device__ __forceinline__ void vss_slab_x_dispatch(int s, const cpx<T>* F1, const cpx<T>* F2, cpx<T>* X1,
+                                                    cpx<T>* X2, const cpx<T> (*e1)[N], const cpx<T> (*e2)[N]) {
+  constexpr int NS = VssSlabShape<N>::NS;
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    const cpx<T>* F = h ? F2 : F1;
+    cpx<T>* X = h ? X2 : X1;
+    const cpx<T> (*e)[N] = h ? e2 : e1;
+    if constexpr (NS == 1) {
+      vss_slab_x_pass<T, N, 0>(F, X, e);
+    } else {
+      static_assert(NS == 3, "slab counts 1 and 3");
+      if (s == 0) vss_slab_x_pass<T, N, 0>(F, X, e);
+      else if (s == 1) vss_slab_x_pass<T, N, 1>(F, X, e);
+      else vss_slab_x_pass<T, N, 2>(F, X, e);
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabArgs<T> sa) {
+  using SS = VssSlabShape<N>;
+  constexpr int NS = SS::NS, NN = N * N, n = NN * N;
+  const VssGainArgs<T>& a = sa.g;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X1 = reinterpret_cast<cpx<T>*>(smem_raw);  // t3 slab
+  cpx<T>* X2 = X1 + SS::planes;                       // t4 slab
+  __shared__ cpx<T> eph[2][2][3][N];  // [buffer][t3, t4][axis][k]: this product's phases and the next's
+  const int tid = threadIdx.x;
+  const int j = tid / N, c = tid - (tid / N) * N;  // plane, row (z-pass) / column (y-pass)
+  // items run point-major, so one point's spectra serve its consecutive
+  // items (jobs, groups, slabs) from one XCD's L2
+  const int per_pt = sa.njobs * a.ngroups * NS;
+  cpx<T> sacc[N];
+  for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
+    const int pt = item / per_pt, rem = item - pt * per_pt;
+    const int jb_i = rem / (a.ngroups * NS), rem2 = rem - jb_i * (a.ngroups * NS);
+    const int grp = rem2 / NS, s = rem2 - grp * NS;
+    const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
+    const VssJob<T>& jb = a.job[jb_i];
+    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
+    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const double mu1 = jb.mu1, mu2 = jb.mu2;
+    // phase factors of direction product (x, iy) into buffer b
+    auto phases = [&](int b, int x, int iy) {
+      if (tid < 3 * N) {
+        const int d = tid / N, k = tid - d * N;
+        const int y = a.dirs[jb.dir_off + iy];
+        const double th = a.alpha[x] * a.sz[y * 3 + d] * lfreq(k, N);
+        double sn, cs;
+        sincos(mu1 * th, &sn, &cs);
+        eph[b][0][d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
+        sincos(mu2 * th, &sn, &cs);
+        eph[b][1][d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(-sn)};
+      }
+    };
+    __syncthreads();  // the previous item's readers of eph are done
+    if (x0 < x1 && jb.ndir > 0) phases(0, x0, 0);
+    int it = 0;  // products of this item: eph[it & 1] holds the current phases
+    for (int x = x0; x < x1; ++x) {
+#pragma unroll
+      for (int y = 0; y < N; ++y) sacc[y] = cpx<T>{T(0), T(0)};
+      for (int iy = 0; iy < jb.ndir; ++iy, ++it) {
+        __syncthreads();  // the previous product's readers of X1, X2 are done; eph[it & 1] is in place
+        vss_slab_x_dispatch<T, N>(s, F1, F2, X1, X2, eph[it & 1][0], eph[it & 1][1]);
+        __syncthreads();
+        {  // z-pass: row (j, ky = c) of both slabs
+#pragma unroll 1
+          for (int h = 0; h < 2; ++h) {
+            cpx<T>* row = (h ? X2 : X1) + j * NN + c * N;
+            cpx<T> v[N];
+#pragma unroll
+            for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+            FFT<T, N, +1, true>::run(v);
+#pragma unroll
+            for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+          }
+        }
+        __syncthreads();
+        // the next product's phase factors into the other buffer (the x-pass
+        // above was this product's last reader of eph)
+        {
+          const int iyn = (iy + 1 < jb.ndir) ? iy + 1 : 0, xn = (iy + 1 < jb.ndir) ? x : x + 1;
+          if (xn < x1) phases((it + 1) & 1, xn, iyn);
+        }
+        {  // y-pass: column (j, z = c); t3 back to its own column, S += t3 t4
+          cpx<T>* p1 = X1 + j * NN;
+          const cpx<T>* p2 = X2 + j * NN;
+          cpx<T> v[N];
+#pragma unroll
+          for (int ky = 0; ky < N; ++ky) v[ky] = p1[ky * N + nyq_swizzle<N>(c, ky)];
+          FFT<T, N, +1, true>::run(v);
+#pragma unroll
+          for (int y = 0; y < N; ++y) p1[y * N + nyq_swizzle<N>(c, y)] = v[y];
+#pragma unroll
+          for (int ky = 0; ky < N; ++ky) v[ky] = p2[ky * N + nyq_swizzle<N>(c, ky)];
+          FFT<T, N, +1, true>::run(v);
+#pragma unroll
+          for (int y = 0; y < N; ++y) sacc[y] = sacc[y] + p1[y * N + nyq_swizzle<N>(c, y)] * v[y];
+        }
+      }
+      // forward transform of the slab of S_x along y (own column) and z
+      {
+        cpx<T> v[N];
+#pragma unroll
+        for (int y = 0; y < N; ++y) v[y] = sacc[y];
+        FFT<T, N, -1, true>::run(v);
+        cpx<T>* pl = X1 + j * NN;
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) pl[ky * N + nyq_swizzle<N>(c, ky)] = v[ky];
+      }
+      __syncthreads();
+      {
+        cpx<T>* row = X1 + j * NN + c * N;
+        cpx<T> v[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+        FFT<T, N, -1, true>::run(v);
+#pragma unroll
+        for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+      }
+      __syncthreads();
+      {
+        cpx<T>* Tp = sa.Tb + ((static_cast<size_t>(jb_i) * a.npts + pt) * a.Nrho + x) * n;
+        for (int e = tid; e < SS::planes; e += SS::NT) {
+          const int jj = e / NN, r = e - jj * NN, ky = r / N, kz = r - ky * N;
+          Tp[static_cast<size_t>(s + NS * jj) * NN + r] = X1[jj * NN + ky * N + nyq_swizzle<N>(kz, ky)];
+        }
+      }
+    }
+  }
+}
+
+// x-forward transform of the slab kernel's T and G^_out = sum_x w_x b(x)
+// FFT(S_x) for both outputs of a job (tables read at the job's first
+// direction row, as the isotropic multi-pass kernel).  A block of 256 threads
+// takes 128 (job, point, pencil (ky, kz)) columns; waves 2i and 2i + 1 share
+// 64 of them and keep the lower and the upper half of the kx sums.
+template <typename T, int N, int K0>
+__device__ __forceinline__ void vss_xfwd_half(const VssSlabArgs<T>& sa, int jb_i, int pt, int idx) {
+  constexpr int NN = N * N, n = NN * N, H = N / 2;
+  const VssGainArgs<T>& a = sa.g;
+  const VssJob<T>& jb = a.job[jb_i];
+  const bool two = jb.out2 >= 0;
+  const int row1 = a.rows1[jb.dir_off], row2 = two ? a.rows2[jb.dir_off] : 0;
+  cpx<T> acc1[H], acc2[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) acc1[k] = acc2[k] = cpx<T>{T(0), T(0)};
+  for (int x = 0; x < a.Nrho; ++x) {
+    const cpx<T>* Tp = sa.Tb + ((static_cast<size_t>(jb_i) * a.npts + pt) * a.Nrho + x) * n + idx;
+    cpx<T> v[N];
+#pragma unroll
+    for (int xx = 0; xx < N; ++xx) v[xx] = Tp[static_cast<size_t>(xx) * NN];
+    FFT<T, N, -1>::run(v);
+    const T w = static_cast<T>(a.wgt[x]);
+    const cpx<T>* b1 = jb.tab1 + (static_cast<size_t>(x) * jb.tabM1 + row1) * n + idx;
+#pragma unroll
+    for (int k = 0; k < H; ++k) acc1[k] = acc1[k] + (w * b1[static_cast<size_t>(k + K0) * NN]) * v[k + K0];
+    if (two) {
+      const cpx<T>* b2 = jb.tab2 + (static_cast<size_t>(x) * jb.tabM2 + row2) * n + idx;
+#pragma unroll
+      for (int k = 0; k < H; ++k) acc2[k] = acc2[k] + (w * b2[static_cast<size_t>(k + K0) * NN]) * v[k + K0];
+    }
+  }
+  cpx<T>* G1 = a.Ghat + (static_cast<size_t>(jb.out1) * a.npts + pt) * n + idx;
+#pragma unroll
+  for (int k = 0; k < H; ++k) G1[static_cast<size_t>(k + K0) * NN] = acc1[k];
+  if (two) {
+    cpx<T>* G2 = a.Ghat + (static_cast<size_t>(jb.out2) * a.npts + pt) * n + idx;
+#pragma unroll
+    for (int k = 0; k < H; ++k) G2[static_cast<size_t>(k + K0) * NN] = acc2[k];
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_vss_xfwd(VssSlabArgs<T> sa) {
+  constexpr int NN = N * N;
+  const int w = threadIdx.x / 64, lane = threadIdx.x - (threadIdx.x / 64) * 64;
+  const long col = static_cast<long>(blockIdx.x) * 128 + (w >> 1) * 64 + lane;
+  const long ncol = static_cast<long>(sa.njobs) * sa.g.npts * NN;
+  if (col >= ncol) return;
+  const long jp = col / NN;
+  const int idx = static_cast<int>(col - jp * NN);
+  const int jb_i = static_cast<int>(jp / sa.g.npts), pt = static_cast<int>(jp - static_cast<long>(jb_i) * sa.g.npts);
+  if (w & 1)
+    vss_xfwd_half<T, N, N / 2>(sa, jb_i, pt, idx);
+  else
+    vss_xfwd_half<T, N, 0>(sa, jb_i, pt, idx);
+}
+
 template <typename T>
 struct VssFinArgs {
   const cpx<T>* Ghat;  // [npairs][npts][ngroups][n]
@@ -956,7 +1218,10 @@ struct VssHandle {
   void* Z3 = nullptr;
   int groups_override = 0;
   int slots_override = 0;  // cap on the persistent gain grid (0: occupancy)
-  int algorithm = 0;       // 0: automatic, 1: multi-pass, 4: register-resident (FP32, Nv 16 / 24)
+  int algorithm = 0;       // 0: automatic, 1: multi-pass, 4: register-resident (FP32, Nv 16 / 24),
+                           // 6: packed isotropic register-resident, 7: slab (isotropic, Nv 16 / 24)
+  void* Tb = nullptr;      // slab kernel's forward-transformed slabs [njobs][cap_pts][Nrho][n]
+  size_t cap_tb = 0;
   int iso = 0;             // eta = 0 for every pair: b_pq(x, y) independent of y
   int last_algorithm = 0;
   int last_groups = 0, last_slots = 0, threads = 0;
@@ -975,7 +1240,7 @@ struct VssHandle {
     (void)hipFree(dirs_d); (void)hipFree(rows1_d); (void)hipFree(rows2_d);
     (void)hipFree(alpha); (void)hipFree(wgt); (void)hipFree(szd);
     (void)hipFree(fc); (void)hipFree(Qc); (void)hipFree(F); (void)hipFree(Ghat);
-    (void)hipFree(Z1); (void)hipFree(Z2); (void)hipFree(Z3);
+    (void)hipFree(Z1); (void)hipFree(Z2); (void)hipFree(Z3); (void)hipFree(Tb);
     for (auto& pr : pending) {
       (void)hipEventDestroy(pr.second.first);
       (void)hipEventDestroy(pr.second.second);
@@ -1050,8 +1315,10 @@ static int vss_ensure_ws(VssHandle* h, int pts) {
 // points per launch: per point the species' f, Q (FP64) and spectra, plus the
 // gain partials of every pair (worst case one per radial node); under 48 GiB
 static int vss_chunk_points(const VssHandle* h, int total) {
+  // (the slab kernel's T, one transform per job and radial node, is bounded by
+  // the same npairs * Nrho transforms)
   const double per = static_cast<double>(h->n) *
-                     (h->nspcs * (16.0 + 2.0 * h->tsize) + h->npairs * 2.0 * h->tsize * h->Nrho);
+                     (h->nspcs * (16.0 + 2.0 * h->tsize) + h->npairs * 2.0 * h->tsize * h->Nrho * (h->iso ? 2 : 1));
   const double budget = 48.0 * 1024.0 * 1024.0 * 1024.0;
   const int cap = std::max(1, static_cast<int>(budget / per));
   if (total <= cap) return std::max(total, 1);
@@ -1059,13 +1326,22 @@ static int vss_chunk_points(const VssHandle* h, int total) {
   return (total + nchunks - 1) / nchunks;
 }
 
-// gain algorithm of this call: 4 (register-resident) where it is built and
-// not switched off, else 1 (multi-pass)
+// gain algorithm of this call: 4 (register-resident, FP32 at Nv 16 / 24)
+// where it is built and not switched off; in FP64 with isotropic scattering
+// at Nv 16 the slab kernel (7: 3.1x the multi-pass kernel on MI355X, 21.8
+// against 67.0 ms per 2048 mixture points; at 24^3, three slabs of 8 planes
+// and three waves per CU, it is 291 against 301 ms, not worth its T buffer,
+// so it runs there only when set); else 1 (multi-pass)
 template <typename T, int N>
 static int vss_rr_kind(const VssHandle* h) {
   constexpr bool rr_ok = std::is_same<T, float>::value && (N == 16 || N == 24);
-  if (!rr_ok || h->algorithm == 1) return 1;  // algorithm 6 is validated to FP32 at 16 / 24 when set
-  return 4;
+  constexpr bool slab_ok = (N == 16 || N == 24);
+  if (h->algorithm == 1) return 1;
+  if (h->algorithm == 7) return (slab_ok && h->iso) ? 7 : 1;
+  if (rr_ok) return 4;  // algorithm 6 is validated to FP32 at 16 / 24 when set
+  // FP64 isotropic at 16^3: the slab kernel
+  if (N == 16 && h->iso && h->algorithm == 0) return 7;
+  return 1;
 }
 
 // fs[s], Qs[s]: [npts][n] contiguous per species
@@ -1075,15 +1351,25 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   const int slots = h->slots_override > 0 ? std::min(h->cap_slots, h->slots_override) : h->cap_slots;
   const std::vector<VssHandle::Job>& jobs = h->sym ? h->jobs_sym : h->jobs_plain;
   const int njobs = static_cast<int>(jobs.size());
+  const int kind = vss_rr_kind<T, N>(h);
+  int slab_ns = 1;
+  if constexpr (N == 16 || N == 24) slab_ns = VssSlabShape<N>::NS;
   int groups = h->groups_override;
   if (groups <= 0) {
-    const int base = njobs * std::max(npts, 1);
-    groups = std::max(1, std::min(h->Nrho, (2 * slots + base - 1) / base));
+    if (kind == 7) {  // (point, job, group, slab) items, one workgroup per CU
+      const int base = njobs * std::max(npts, 1) * slab_ns;
+      groups = std::max(1, std::min(h->Nrho, (cu_count() + base - 1) / base));
+    } else {
+      const int base = njobs * std::max(npts, 1);
+      groups = std::max(1, std::min(h->Nrho, (2 * slots + base - 1) / base));
+    }
   }
   const int ppg = (h->Nrho + groups - 1) / groups;
   groups = (h->Nrho + ppg - 1) / ppg;
   const size_t items = static_cast<size_t>(njobs) * npts * groups;
-  const size_t outs = static_cast<size_t>(h->npairs) * npts * groups;
+  // the slab kernel's x-forward sum leaves one G^ per (pair, point)
+  const int fin_groups = (kind == 7) ? 1 : groups;
+  const size_t outs = static_cast<size_t>(h->npairs) * npts * fin_groups;
   if (outs > h->cap_items) {
     (void)hipFree(h->Ghat);
     h->Ghat = nullptr;
@@ -1141,7 +1427,38 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   ga.ngroups = groups;
   ga.ppg = ppg;
   ga.Nrho = h->Nrho;
-  if (vss_rr_kind<T, N>(h) == 4) {
+  if (kind == 7) {
+    if constexpr (N == 16 || N == 24) {
+      using SS = VssSlabShape<N>;
+      const size_t tbytes = static_cast<size_t>(njobs) * npts * h->Nrho * n * sizeof(cpx<T>);
+      if (tbytes > h->cap_tb) {
+        (void)hipFree(h->Tb);
+        h->Tb = nullptr;
+        h->cap_tb = 0;
+        DGFS_HIP_TRY(hipMalloc(&h->Tb, tbytes));
+        h->cap_tb = tbytes;
+      }
+      VssSlabArgs<T> sa{ga, static_cast<cpx<T>*>(h->Tb), njobs};
+      sa.g.nitems = static_cast<int>(items * SS::NS);
+      const size_t lds = 2 * static_cast<size_t>(SS::planes) * sizeof(cpx<T>);
+      const void* kfn = reinterpret_cast<const void*>(k_vss_gain_slab<T, N>);
+      DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      int occ = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, SS::NT, lds) != hipSuccess || occ < 1) occ = 1;
+      int grid = static_cast<int>(std::min<size_t>(items * SS::NS, static_cast<size_t>(occ) * cu_count()));
+      if (h->slots_override > 0) grid = std::min(grid, h->slots_override);
+      KTimerT<VssHandle> t1(h, 1, st);
+      hipLaunchKernelGGL((k_vss_gain_slab<T, N>), dim3(grid), dim3(SS::NT), lds, st, sa);
+      DGFS_LAUNCH_CHECK("vss_gain_slab");
+      const long cols = static_cast<long>(njobs) * npts * N * N;
+      hipLaunchKernelGGL((k_vss_xfwd<T, N>), dim3(static_cast<unsigned>((cols + 127) / 128)), dim3(256), 0, st, sa);
+      DGFS_LAUNCH_CHECK("vss_xfwd");
+      t1.stop();
+      h->last_slots = grid;
+      h->threads = SS::NT;
+      h->last_algorithm = 7;
+    }
+  } else if (kind == 4) {
     if constexpr (std::is_same<T, float>::value && (N == 16 || N == 24)) {
       const size_t lds = static_cast<size_t>(VssRR1Shape<N>::field) * sizeof(cpx<T>);
       const bool pack = h->iso && (h->algorithm == 6 || (h->algorithm == 0 && kVssPackDefault));
@@ -1195,7 +1512,7 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   }
   fa.nspcs = h->nspcs;
   fa.npts = npts;
-  fa.ngroups = groups;
+  fa.ngroups = fin_groups;
   {
     KTimerT<VssHandle> t2(h, 2, st);
     hipLaunchKernelGGL((k_vss_finalize<T, N>), dim3(std::min(h->nspcs * npts, slots)), dim3(Shape<N>::NT), 0,
@@ -1594,13 +1911,16 @@ extern "C" int dgfs_vss_set_groups(void* p, int ngroups) {
 extern "C" int dgfs_vss_set_algorithm(void* p, int algorithm) {
   VssHandle* h = as_vss(p);
   if (!h) return fail(DGFS_ERR_NOT_INITIALIZED, "invalid VSS handle");
-  if (algorithm != 0 && algorithm != 1 && algorithm != 4 && algorithm != 6)
+  if (algorithm != 0 && algorithm != 1 && algorithm != 4 && algorithm != 6 && algorithm != 7)
     return fail(DGFS_ERR_INVALID_VALUE,
-                "algorithm must be 0 (automatic), 1 (multi-pass), 4 (register-resident) or 6 (packed isotropic)");
+                "algorithm must be 0 (automatic), 1 (multi-pass), 4 (register-resident), 6 (packed isotropic) or "
+                "7 (slab)");
   if ((algorithm == 4 || algorithm == 6) && !(h->prec == 32 && (h->N == 16 || h->N == 24)))
     return fail(DGFS_ERR_UNSUPPORTED, "the register-resident VSS gain is built for FP32 at Nv 16 and 24");
   if (algorithm == 6 && !h->iso)
     return fail(DGFS_ERR_UNSUPPORTED, "the packed VSS gain needs isotropic scattering (eta = 0 for every pair)");
+  if (algorithm == 7 && !(h->iso && (h->N == 16 || h->N == 24)))
+    return fail(DGFS_ERR_UNSUPPORTED, "the slab VSS gain is built for isotropic scattering at Nv 16 and 24");
   h->algorithm = algorithm;
   return DGFS_OK;
 }

@@ -88,12 +88,13 @@ class VSSOperator:
 
     def set_algorithm(self, algorithm):
         """Gain kernel: 0 automatic, 1 multi-pass, 4 register-resident (FP32,
-        Nv 16 / 24), 6 packed isotropic register-resident (eta = 0 only)."""
+        Nv 16 / 24), 6 packed isotropic register-resident (eta = 0 only), 7 slab
+        kernel (eta = 0, Nv 16 / 24; the FP64 default at Nv 16)."""
         _lib.call("dgfs_vss_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):
         """Gain kernel of the last call: 1 multi-pass, 4 register-resident,
-        5 register-resident for isotropic scattering (eta = 0), 6 packed."""
+        5 register-resident for isotropic scattering (eta = 0), 6 packed, 7 slab."""
         v = C.c_int()
         _lib.call("dgfs_vss_get_algorithm", self._h, C.byref(v))
         return v.value

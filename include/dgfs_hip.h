@@ -202,14 +202,18 @@ int dgfs_vss_set_groups(void* handle, int ngroups);
 /* Cap the persistent grid (and so the scratch cubes in use) at nslots
  * workgroups; 0 = the occupancy limit. */
 int dgfs_vss_set_slots(void* handle, int nslots);
-/* Gain kernel: 0 automatic (register-resident where built, else multi-pass),
- * 1 multi-pass (any precision and grid), 4 register-resident with one pencil
- * per thread (FP32 at Nv 16 and 24), 6 its isotropic form with t3 and t4 from
- * one packed complex transform plus Nyquist-plane corrections (not the
- * automatic choice: slower than 4 as measured).  With eta = 0 for every pair
- * (isotropic scattering) 1 and 4 sum the direction products in space before
- * one forward transform per radial node.  get_algorithm reports the kernel of
- * the last call: 1, 4, 5 for the isotropic register-resident kernel, or 6. */
+/* Gain kernel: 0 automatic (register-resident in FP32 at Nv 16 and 24; the
+ * slab kernel in FP64 with isotropic scattering at Nv 16; multi-pass
+ * otherwise), 1 multi-pass (any precision and grid), 4 register-resident with
+ * one pencil per thread (FP32 at Nv 16 and 24), 6 its isotropic form with t3
+ * and t4 from one packed complex transform plus Nyquist-plane corrections (not
+ * the automatic choice: slower than 4 as measured), 7 slab kernel (eta = 0 for
+ * every pair, Nv 16 or 24, either precision: x-slabs of t3 and t4 in LDS, a
+ * second kernel for the x-forward transform and the sum over radial nodes).
+ * With eta = 0 for every pair (isotropic scattering) 1, 4 and 7 sum the
+ * direction products in space before one forward transform per radial node.
+ * get_algorithm reports the kernel of the last call: 1, 4, 5 for the isotropic
+ * register-resident kernel, 6 or 7. */
 int dgfs_vss_set_algorithm(void* handle, int algorithm);
 int dgfs_vss_get_algorithm(void* handle, int* algorithm);
 

@@ -161,6 +161,9 @@ def test_autotune_candidates_for_both_operators():
     assert sorted({x["algorithm"] for x in Autotuner.candidates(vss, 256)}) == [1, 4]
     vss.precision = 64
     assert sorted({x["algorithm"] for x in Autotuner.candidates(vss, 256)}) == [1]
+    vss.iso = True
+    assert sorted({x["algorithm"] for x in Autotuner.candidates(vss, 256)}) == [1, 7]
+    vss.iso = False
     assert _key(vss, 2048).startswith("vss-N24-p64-K144-S2-iso0")
     vhs = types.SimpleNamespace(Nv=32, precision=32, K=192, project=True, Nrho=16)
     assert sorted({x["algorithm"] for x in Autotuner.candidates(vhs, 256)}) == [1, 2, 3, 5]

@@ -113,8 +113,9 @@ CFG24 = (24, 6.0, 6, 12, 24, (1.0, 2.1), 1.0, (0.81, 0.805, 0.805, 0.80), (1.0, 
 @pytest.mark.parametrize("alpha", [1.0, 1.3])
 def test_oracle_parity_fp64_fp32_24(alpha, device):
     """24^3 against the FP64 CPU oracle on 2 bimodal mixture points: the FP64
-    operator (multi-pass) to 1e-10 and the FP32 one (register-resident, the
-    default at 24^3) to 1e-3."""
+    operator (multi-pass, the default at 24^3) to 1e-10 and the FP32 one
+    (register-resident, the default at 24^3) to 1e-3; for alpha = 1 the slab
+    kernel (algorithm 7) to 1e-10 as well."""
     cfg = CFG24[:6] + (alpha,) + CFG24[7:]
     N, L, Nrho, M, Mpre, masses = cfg[:6]
     vm = V.BiVelocityMesh(N, L, Nrho, M, masses)
@@ -124,8 +125,15 @@ def test_oracle_parity_fp64_fp32_24(alpha, device):
     op64, _ = make_op(*cfg, precision=64)
     Q64 = run_contig(op64, fs, device)
     e64 = [rel(Q64[p], Qref[p]) for p in range(2)]
-    say("  fp64", e64)
+    say("  fp64 algorithm", op64.get_algorithm(), e64)
+    assert op64.get_algorithm() == 1
     assert max(e64) < 1e-10
+    if alpha == 1.0:
+        op64.set_algorithm(7)
+        Qs = run_contig(op64, fs, device)
+        es = [rel(Qs[p], Qref[p]) for p in range(2)]
+        say("  fp64 slab kernel", es)
+        assert op64.get_algorithm() == 7 and max(es) < 1e-10
     op32, _ = make_op(*cfg, precision=32)
     Q32 = run_contig(op32, fs, device)
     say("  fp32 algorithm", op32.get_algorithm())
@@ -311,6 +319,54 @@ def test_register_resident_matches_multipass(N, L, masses, sym, alpha, device):
         from dgfs_hip import _lib
         with pytest.raises(_lib.DGFSUnsupported):
             op.set_algorithm(6)
+
+
+@pytest.mark.parametrize("N,L,masses,sym", [(16, 5.0, (1.0, 2.1), True), (24, 6.0, (1.0, 2.1), True),
+                                            (24, 6.0, (1.0, 2.1), False), (16, 5.0, (1.0, 2.1, 0.7), True)])
+def test_slab_kernel_matches_multipass(N, L, masses, sym, device):
+    """The slab gain for isotropic scattering (algorithm 7: x-slabs of t3 and
+    t4 in LDS, S_x in registers, x-forward sum in a second kernel), the FP64
+    default at Nv 16, against the multi-pass FP64 gain on the same states
+    (roundoff) and, in FP32, against the FP64 operator."""
+    S = len(masses)
+    Nrho, M, Mpre = 6, 12, 24
+    omegas = tuple(0.81 - 0.005 * (p + q) for p in range(S) for q in range(S))
+    op64, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=64)
+    op64.set_symmetry(sym)
+    fs = _mixture_states(V.BiVelocityMesh(N, L, Nrho, M, masses), 4, seed=37)
+    op64.set_algorithm(1)
+    Qm = run_contig(op64, fs, device)
+    for groups in (0, 1, 4):
+        say("vss fp64 slab kernel", N, S, sym, "groups", groups)
+        op64.set_algorithm(7)
+        op64.set_groups(groups)
+        Qs = run_contig(op64, fs, device)
+        assert op64.get_algorithm() == 7
+        d = [rel(Qs[p], Qm[p]) for p in range(S)]
+        say("  vs multi-pass", d)
+        assert max(d) < 1e-12
+    op64.set_algorithm(0)
+    op64.set_groups(0)
+    run_contig(op64, fs, device)
+    assert op64.get_algorithm() == (7 if N == 16 else 1)  # the FP64 default for isotropic scattering
+    op32, _ = make_op(N, L, Nrho, M, Mpre, masses, 1.0, omegas, (1.0,) * (S * S), precision=32)
+    op32.set_symmetry(sym)
+    op32.set_algorithm(7)
+    Q32 = run_contig(op32, fs, device)
+    assert op32.get_algorithm() == 7
+    e32 = [rel(Q32[p], Qm[p]) for p in range(S)]
+    say("  fp32 slab vs fp64", e32)
+    assert max(e32) < 1e-4
+
+
+def test_slab_kernel_needs_isotropic_scattering(device):
+    from dgfs_hip import _lib
+    op, _ = make_op(16, 5.0, 4, 12, 24, (1.0, 2.1), 1.3, (0.8,) * 4, (1,) * 4, precision=64)
+    with pytest.raises(_lib.DGFSUnsupported):
+        op.set_algorithm(7)
+    op8, _ = make_op(8, 5.0, 4, 12, 24, (1.0, 2.1), 1.0, (0.8,) * 4, (1,) * 4, precision=64)
+    with pytest.raises(_lib.DGFSUnsupported):
+        op8.set_algorithm(7)
 
 
 def test_register_resident_needs_fp32_and_grid(device):
