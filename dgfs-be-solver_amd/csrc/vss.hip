@@ -782,17 +782,18 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
 // Slab gain for isotropic scattering (eta = 0 for every pair), the FP64 form
 // at Nv 16 / 24.  A workgroup owns a (point, job, radial group, slab) item;
 // slab s holds the XS = N/NS planes x = s + NS j of the two inverse
-// transforms t3 = IFFT(E1 F_p) and t4 = IFFT(E2 F_q) in LDS.  Per direction
-// product:
-//   x-pass  pencils (ky, kz) of F_p and F_q times their x phase factors,
-//           folded over kx = r + XS m with exp(2 pi i m s / NS), the yz phase
-//           and the twiddle exp(2 pi i r s / N) applied after the fold, an
-//           XS-point inverse FFT: plane j, row ky, column kz of the t3 and
-//           the t4 slab;
-//   z-pass  one row (plane j, ky) of each slab per thread;
-//   y-pass  one column (plane j, z) of each: t3 goes back to its own column,
-//           t4 stays in registers, and S_x[y] += t3 t4 (complex, in
-//           registers over the job's directions).
+// transforms t3 = IFFT(E1 F_p) and t4 = IFFT(E2 F_q) in LDS, and its two
+// halves transform one each.  Per direction product:
+//   x-pass  pencils (ky, kz) of the half's spectrum times its x phase
+//           factors, folded over kx = r + XS m with exp(2 pi i m s / NS), the
+//           yz phase and the twiddle exp(2 pi i r s / N) applied after the
+//           fold, an XS-point inverse FFT: plane j, row ky, column kz of the
+//           half's slab;
+//   z-pass  one row (plane j, ky) per thread;
+//   y-pass  one column (plane j, z) per thread, back into its own column;
+//   product S_x[y] += t3 t4 (complex), the t3 half's thread of column (j, z)
+//           taking y < N/2 and the t4 half's the rest, in registers over the
+//           job's directions.
 // At the end of radial node x the slab of S_x is forward transformed along
 // y and z into T[job][point][x]; k_vss_xfwd completes the transform along x
 // and forms G^ = sum_x w_x b(x) FFT(S_x) for both outputs of the job.  Same
@@ -805,10 +806,12 @@ struct VssSlabShape {
   // 16^3 fits whole
   static constexpr int NS = (N == 24) ? 3 : 1;
   static constexpr int XS = N / NS;
-  static constexpr int NT = XS * N;  // one row per thread
+  static constexpr int NH = XS * N;  // threads per slab: one row of it each
+  static constexpr int NT = 2 * NH;  // the t3 half and the t4 half of the workgroup
   static constexpr int planes = XS * N * N;  // complex, per slab
   static_assert(NS * XS == N, "slabs must tile the x axis");
-  static_assert((N * N) % NT == 0, "x-pass pencils split evenly over the workgroup");
+  static_assert((N * N) % NH == 0, "x-pass pencils split evenly over a half");
+  static_assert(N % 2 == 0, "the product's y range splits in halves");
 };
 
 template <typename T>
@@ -819,23 +822,27 @@ struct VssSlabArgs {
 };
 
 template <typename T, int N, int S>
-__device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cpx<T>* X, const cpx<T> (*e)[N]) {
+__device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cpx<T>* X, const cpx<T> (*e)[N],
+                                                int th) {
   using SS = VssSlabShape<N>;
-  constexpr int NS = SS::NS, XS = SS::XS, NT = SS::NT, NN = N * N;
+  constexpr int NS = SS::NS, XS = SS::XS, NH = SS::NH, NN = N * N;
 #pragma unroll 1
-  for (int i = 0; i < NN / NT; ++i) {
-    const int idx = static_cast<int>(threadIdx.x) + NT * i;  // ky * N + kz
+  for (int i = 0; i < NN / NH; ++i) {
+    const int idx = th + NH * i;  // ky * N + kz
     const int ky = idx / N, kz = idx - (idx / N) * N;
     const cpx<T> eyz = e[1][ky] * e[2][kz];
     cpx<T> U[XS];
 #pragma unroll
     for (int r = 0; r < XS; ++r) U[r] = cpx<T>{T(0), T(0)};
+    // one fold term (XS loads) in flight at a time: the loads of the whole
+    // pencil held at once spill at two waves per SIMD
     static_for<0, NS>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       static_for<0, XS>([&](auto rc) {
         constexpr int r = decltype(rc)::value, kx = r + XS * m;
         U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(F[kx * NN + idx] * e[0][kx]);
       });
+      __builtin_amdgcn_sched_barrier(0);
     });
     static_for<0, XS>([&](auto rc) {
       constexpr int r = decltype(rc)::value;
@@ -847,62 +854,59 @@ __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cp
   }
 }
 
-// x-pass of both spectra (one after the other, so that one spectrum's loads
-// are in flight at a time)
+// x-pass of this half's spectrum (t3 half: F_p; t4 half: F_q); th is the
+// thread's index within its half
 template <typename T, int N>
-__device__ __forceinline__ void vss_slab_x_dispatch(int s, const cpx<T>* F1, const cpx<T>* F2, cpx<T>* X1,
-                                                    cpx<T>* X2, const cpx<T> (*e1)[N], const cpx<T> (*e2)[N]) {
+__device__ __forceinline__ void vss_slab_x_dispatch(int s, const cpx<T>* F, cpx<T>* X, const cpx<T> (*e)[N],
+                                                    int th) {
   constexpr int NS = VssSlabShape<N>::NS;
-#pragma unroll 1
-  for (int h = 0; h < 2; ++h) {
-    const cpx<T>* F = h ? F2 : F1;
-    cpx<T>* X = h ? X2 : X1;
-    const cpx<T> (*e)[N] = h ? e2 : e1;
-    if constexpr (NS == 1) {
-      vss_slab_x_pass<T, N, 0>(F, X, e);
-    } else {
-      static_assert(NS == 3, "slab counts 1 and 3");
-      if (s == 0) vss_slab_x_pass<T, N, 0>(F, X, e);
-      else if (s == 1) vss_slab_x_pass<T, N, 1>(F, X, e);
-      else vss_slab_x_pass<T, N, 2>(F, X, e);
-    }
+  if constexpr (NS == 1) {
+    vss_slab_x_pass<T, N, 0>(F, X, e, th);
+  } else {
+    static_assert(NS == 3, "slab counts 1 and 3");
+    if (s == 0) vss_slab_x_pass<T, N, 0>(F, X, e, th);
+    else if (s == 1) vss_slab_x_pass<T, N, 1>(F, X, e, th);
+    else vss_slab_x_pass<T, N, 2>(F, X, e, th);
   }
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabArgs<T> sa) {
   using SS = VssSlabShape<N>;
-  constexpr int NS = SS::NS, NN = N * N, n = NN * N;
+  constexpr int NS = SS::NS, NH = SS::NH, NN = N * N, n = NN * N, HY = N / 2;
   const VssGainArgs<T>& a = sa.g;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cpx<T>* X1 = reinterpret_cast<cpx<T>*>(smem_raw);  // t3 slab
   cpx<T>* X2 = X1 + SS::planes;                       // t4 slab
   __shared__ cpx<T> eph[2][2][3][N];  // [buffer][t3, t4][axis][k]: this product's phases and the next's
   const int tid = threadIdx.x;
-  const int j = tid / N, c = tid - (tid / N) * N;  // plane, row (z-pass) / column (y-pass)
+  // half h transforms t3 (h = 0) or t4 (h = 1); within it, thread th has
+  // plane j and row / column c
+  const int h = tid / NH, th = tid - (tid / NH) * NH;
+  const int j = th / N, c = th - (th / N) * N;
+  cpx<T>* Xh = h ? X2 : X1;
   // items run point-major, so one point's spectra serve its consecutive
   // items (jobs, groups, slabs) from one XCD's L2
   const int per_pt = sa.njobs * a.ngroups * NS;
-  cpx<T> sacc[N];
+  cpx<T> sacc[HY];  // S_x at y = h * N/2 + i of column (j, c)
   for (int item = xcd_slot(); item < a.nitems; item += gridDim.x) {
     const int pt = item / per_pt, rem = item - pt * per_pt;
     const int jb_i = rem / (a.ngroups * NS), rem2 = rem - jb_i * (a.ngroups * NS);
     const int grp = rem2 / NS, s = rem2 - grp * NS;
     const int x0 = grp * a.ppg, x1 = min(x0 + a.ppg, a.Nrho);
     const VssJob<T>& jb = a.job[jb_i];
-    const cpx<T>* F1 = a.F + (static_cast<size_t>(jb.sp) * a.npts + pt) * n;
-    const cpx<T>* F2 = a.F + (static_cast<size_t>(jb.sq) * a.npts + pt) * n;
+    const cpx<T>* Fh = a.F + (static_cast<size_t>(h ? jb.sq : jb.sp) * a.npts + pt) * n;
     const double mu1 = jb.mu1, mu2 = jb.mu2;
     // phase factors of direction product (x, iy) into buffer b
     auto phases = [&](int b, int x, int iy) {
       if (tid < 3 * N) {
         const int d = tid / N, k = tid - d * N;
         const int y = a.dirs[jb.dir_off + iy];
-        const double th = a.alpha[x] * a.sz[y * 3 + d] * lfreq(k, N);
+        const double th_ = a.alpha[x] * a.sz[y * 3 + d] * lfreq(k, N);
         double sn, cs;
-        sincos(mu1 * th, &sn, &cs);
+        sincos(mu1 * th_, &sn, &cs);
         eph[b][0][d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(sn)};
-        sincos(mu2 * th, &sn, &cs);
+        sincos(mu2 * th_, &sn, &cs);
         eph[b][1][d][k] = cpx<T>{static_cast<T>(cs), static_cast<T>(-sn)};
       }
     };
@@ -911,22 +915,19 @@ __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabAr
     int it = 0;  // products of this item: eph[it & 1] holds the current phases
     for (int x = x0; x < x1; ++x) {
 #pragma unroll
-      for (int y = 0; y < N; ++y) sacc[y] = cpx<T>{T(0), T(0)};
+      for (int i = 0; i < HY; ++i) sacc[i] = cpx<T>{T(0), T(0)};
       for (int iy = 0; iy < jb.ndir; ++iy, ++it) {
         __syncthreads();  // the previous product's readers of X1, X2 are done; eph[it & 1] is in place
-        vss_slab_x_dispatch<T, N>(s, F1, F2, X1, X2, eph[it & 1][0], eph[it & 1][1]);
+        vss_slab_x_dispatch<T, N>(s, Fh, Xh, eph[it & 1][h], th);
         __syncthreads();
-        {  // z-pass: row (j, ky = c) of both slabs
-#pragma unroll 1
-          for (int h = 0; h < 2; ++h) {
-            cpx<T>* row = (h ? X2 : X1) + j * NN + c * N;
-            cpx<T> v[N];
+        {  // z-pass: row (j, ky = c) of this half's slab
+          cpx<T>* row = Xh + j * NN + c * N;
+          cpx<T> v[N];
 #pragma unroll
-            for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
-            FFT<T, N, +1, true>::run(v);
+          for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+          FFT<T, N, +1, true>::run(v);
 #pragma unroll
-            for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
-          }
+          for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
         }
         __syncthreads();
         // the next product's phase factors into the other buffer (the x-pass
@@ -935,34 +936,49 @@ __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabAr
           const int iyn = (iy + 1 < jb.ndir) ? iy + 1 : 0, xn = (iy + 1 < jb.ndir) ? x : x + 1;
           if (xn < x1) phases((it + 1) & 1, xn, iyn);
         }
-        {  // y-pass: column (j, z = c); t3 back to its own column, S += t3 t4
-          cpx<T>* p1 = X1 + j * NN;
-          const cpx<T>* p2 = X2 + j * NN;
+        {  // y-pass: column (j, z = c) of this half's slab, back into its own column
+          cpx<T>* pl = Xh + j * NN;
           cpx<T> v[N];
 #pragma unroll
-          for (int ky = 0; ky < N; ++ky) v[ky] = p1[ky * N + nyq_swizzle<N>(c, ky)];
+          for (int ky = 0; ky < N; ++ky) v[ky] = pl[ky * N + nyq_swizzle<N>(c, ky)];
           FFT<T, N, +1, true>::run(v);
 #pragma unroll
-          for (int y = 0; y < N; ++y) p1[y * N + nyq_swizzle<N>(c, y)] = v[y];
+          for (int y = 0; y < N; ++y) pl[y * N + nyq_swizzle<N>(c, y)] = v[y];
+        }
+        __syncthreads();
+        {  // S_x += t3 t4 on this thread's half of column (j, c)
+          const cpx<T>* p1 = X1 + j * NN;
+          const cpx<T>* p2 = X2 + j * NN;
 #pragma unroll
-          for (int ky = 0; ky < N; ++ky) v[ky] = p2[ky * N + nyq_swizzle<N>(c, ky)];
-          FFT<T, N, +1, true>::run(v);
-#pragma unroll
-          for (int y = 0; y < N; ++y) sacc[y] = sacc[y] + p1[y * N + nyq_swizzle<N>(c, y)] * v[y];
+          for (int i = 0; i < HY; ++i) {
+            const int y = h * HY + i, o = y * N + nyq_swizzle<N>(c, y);
+            sacc[i] = sacc[i] + p1[o] * p2[o];
+          }
         }
       }
-      // forward transform of the slab of S_x along y (own column) and z
+      // forward transform of the slab of S_x along y and z (the t3 half's
+      // threads; the halves first gather each column's S in LDS)
+      __syncthreads();  // every product read of X1, X2 is done
       {
+        cpx<T>* pl = X1 + j * NN;
+#pragma unroll
+        for (int i = 0; i < HY; ++i) {
+          const int y = h * HY + i;
+          pl[y * N + nyq_swizzle<N>(c, y)] = sacc[i];
+        }
+      }
+      __syncthreads();
+      if (h == 0) {
+        cpx<T>* pl = X1 + j * NN;
         cpx<T> v[N];
 #pragma unroll
-        for (int y = 0; y < N; ++y) v[y] = sacc[y];
+        for (int y = 0; y < N; ++y) v[y] = pl[y * N + nyq_swizzle<N>(c, y)];
         FFT<T, N, -1, true>::run(v);
-        cpx<T>* pl = X1 + j * NN;
 #pragma unroll
         for (int ky = 0; ky < N; ++ky) pl[ky * N + nyq_swizzle<N>(c, ky)] = v[ky];
       }
       __syncthreads();
-      {
+      if (h == 0) {
         cpx<T>* row = X1 + j * NN + c * N;
         cpx<T> v[N];
 #pragma unroll
@@ -1328,10 +1344,9 @@ static int vss_chunk_points(const VssHandle* h, int total) {
 
 // gain algorithm of this call: 4 (register-resident, FP32 at Nv 16 / 24)
 // where it is built and not switched off; in FP64 with isotropic scattering
-// at Nv 16 the slab kernel (7: 3.1x the multi-pass kernel on MI355X, 21.8
-// against 67.0 ms per 2048 mixture points; at 24^3, three slabs of 8 planes
-// and three waves per CU, it is 291 against 301 ms, not worth its T buffer,
-// so it runs there only when set); else 1 (multi-pass)
+// at Nv 16 / 24 the slab kernel (7: 16.1 against 67.0 ms per 2048 mixture
+// points at 16^3, 192.3 against 301.2 ms at 24^3 on MI355X); else 1
+// (multi-pass)
 template <typename T, int N>
 static int vss_rr_kind(const VssHandle* h) {
   constexpr bool rr_ok = std::is_same<T, float>::value && (N == 16 || N == 24);
@@ -1339,8 +1354,8 @@ static int vss_rr_kind(const VssHandle* h) {
   if (h->algorithm == 1) return 1;
   if (h->algorithm == 7) return (slab_ok && h->iso) ? 7 : 1;
   if (rr_ok) return 4;  // algorithm 6 is validated to FP32 at 16 / 24 when set
-  // FP64 isotropic at 16^3: the slab kernel
-  if (N == 16 && h->iso && h->algorithm == 0) return 7;
+  // FP64 isotropic: the slab kernel
+  if (slab_ok && h->iso && h->algorithm == 0) return 7;
   return 1;
 }
 

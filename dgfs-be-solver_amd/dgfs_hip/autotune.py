@@ -98,7 +98,7 @@ class Autotuner:
         if op.precision == 32 and op.Nv in (16, 24):
             algos[:0] = [4, 6] if op.iso else [4]
         if op.precision == 64 and op.iso and op.Nv in (16, 24):
-            algos[:0] = [7]  # the default at 16^3; at 24^3 a candidate
+            algos[:0] = [7]
         groups = [0]
         g = 1
         while g <= op.Nrho:

@@ -203,7 +203,7 @@ int dgfs_vss_set_groups(void* handle, int ngroups);
  * workgroups; 0 = the occupancy limit. */
 int dgfs_vss_set_slots(void* handle, int nslots);
 /* Gain kernel: 0 automatic (register-resident in FP32 at Nv 16 and 24; the
- * slab kernel in FP64 with isotropic scattering at Nv 16; multi-pass
+ * slab kernel in FP64 with isotropic scattering at Nv 16 and 24; multi-pass
  * otherwise), 1 multi-pass (any precision and grid), 4 register-resident with
  * one pencil per thread (FP32 at Nv 16 and 24), 6 its isotropic form with t3
  * and t4 from one packed complex transform plus Nyquist-plane corrections (not
