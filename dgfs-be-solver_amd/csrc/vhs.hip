@@ -949,25 +949,40 @@ template <typename T, int N, int S>
 __device__ __forceinline__ void slab_x_pass(const cpx<T>* __restrict__ Ft, cpx<T>* X, const cpx<T> (*eph)[N]) {
   using SS = SlabShape<N>;
   constexpr int NS = SS::NS, XS = SS::XS, NT = SS::NT, NN = N * N;
-  // two pencils per round: their loads are in flight together (4.6 % faster
-  // than one at a time at 32^3 FP64 on MI355X)
-#pragma unroll 2
+  // one pencil per round; the XS spectrum values of fold term m + 1 are in
+  // flight while term m is accumulated, and the x phase factors are read
+  // from LDS per pencil (an opaque offset keeps the compiler from holding
+  // all N of them in registers across the pencils).  Two pencils per round
+  // with the factors held in registers left the FP64 kernel so short of
+  // registers that every spectrum load was waited for before the next was
+  // issued: 143.9 -> 95.0 ms per 2048 points at 32^3 FP64 on MI355X.
+#pragma unroll 1
   for (int i = 0; i < NN / NT; ++i) {
     const int idx = static_cast<int>(threadIdx.x) + NT * i;  // kz * N + ky
     const int kz = idx / N, ky = idx - (idx / N) * N;
     const cpx<T> eyz = eph[1][ky] * eph[2][kz];
     const T A = eyz.x + eyz.y, B = eyz.x - eyz.y;  // Re E + Im E = ex.x A + ex.y B
     const cpx<T>* P = Ft + idx;
-    cpx<T> U[XS];
+    int eo = 0;
+    asm volatile("" : "+v"(eo));
+    const cpx<T>* ex = &eph[0][0] + eo;
+    cpx<T> U[XS], ld[2][XS];
 #pragma unroll
-    for (int r = 0; r < XS; ++r) U[r] = cpx<T>{T(0), T(0)};
+    for (int r = 0; r < XS; ++r) {
+      U[r] = cpx<T>{T(0), T(0)};
+      ld[0][r] = P[r * NN];
+    }
     static_for<0, NS>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
+      if constexpr (m + 1 < NS) {
+#pragma unroll
+        for (int r = 0; r < XS; ++r) ld[(m + 1) & 1][r] = P[(r + XS * (m + 1)) * NN];
+      }
       static_for<0, XS>([&](auto rc) {
         constexpr int r = decltype(rc)::value, kx = r + XS * m;
-        const cpx<T> e = eph[0][kx];
+        const cpx<T> e = ex[kx];
         const T fac = e.x * A + e.y * B;
-        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(fac * P[kx * NN]);
+        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(fac * ld[m & 1][r]);
       });
     });
     static_for<1, XS>([&](auto rc) {

@@ -831,18 +831,30 @@ __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cp
     const int idx = th + NH * i;  // ky * N + kz
     const int ky = idx / N, kz = idx - (idx / N) * N;
     const cpx<T> eyz = e[1][ky] * e[2][kz];
-    cpx<T> U[XS];
+    // the XS loads of fold term m + 1 are in flight while term m is
+    // accumulated (the loads of the whole pencil held at once spill at two
+    // waves per SIMD); the x phase factors are read from LDS per pencil
+    // through an opaque offset, so they are not held in registers
+    int eo = 0;
+    asm volatile("" : "+v"(eo));
+    const cpx<T>* ex = &e[0][0] + eo;
+    const cpx<T>* P = F + idx;
+    cpx<T> U[XS], ld[2][XS];
 #pragma unroll
-    for (int r = 0; r < XS; ++r) U[r] = cpx<T>{T(0), T(0)};
-    // one fold term (XS loads) in flight at a time: the loads of the whole
-    // pencil held at once spill at two waves per SIMD
+    for (int r = 0; r < XS; ++r) {
+      U[r] = cpx<T>{T(0), T(0)};
+      ld[0][r] = P[r * NN];
+    }
     static_for<0, NS>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
+      if constexpr (m + 1 < NS) {
+#pragma unroll
+        for (int r = 0; r < XS; ++r) ld[(m + 1) & 1][r] = P[(r + XS * (m + 1)) * NN];
+      }
       static_for<0, XS>([&](auto rc) {
         constexpr int r = decltype(rc)::value, kx = r + XS * m;
-        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(F[kx * NN + idx] * e[0][kx]);
+        U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(ld[m & 1][r] * ex[kx]);
       });
-      __builtin_amdgcn_sched_barrier(0);
     });
     static_for<0, XS>([&](auto rc) {
       constexpr int r = decltype(rc)::value;
