@@ -94,6 +94,35 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// G[k] = (first ? 0 : G[k]) + (w b[k]) v[k] for k < N, C values at a time,
+// every load of a chunk issued before its first store.  G and b are pencil
+// accessors (G[k] a reference, b[k] a real or complex value).  Written as one
+// loop, each store could overlap the next loads as far as the compiler can
+// tell, so every load would wait for the one before it to return.
+template <typename T, int N, int C, class GA, class BA>
+__device__ __forceinline__ void pencil_accumulate(const GA& G, const BA& b, T w, const cpx<T> (&v)[N], bool first) {
+  static_assert(N % C == 0, "whole chunks");
+  using BT = std::decay_t<decltype(b[0])>;
+#pragma unroll
+  for (int k0 = 0; k0 < N; k0 += C) {
+    BT bv[C];
+    cpx<T> gv[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) bv[k] = b[k0 + k];
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < C; ++k) gv[k] = cpx<T>{T(0), T(0)};
+    } else {
+#pragma unroll
+      for (int k = 0; k < C; ++k) gv[k] = G[k0 + k];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < C; ++k) G[k0 + k] = gv[k] + (w * bv[k]) * v[k0 + k];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Consecutive workgroups of a launch go round-robin to the 8 XCDs of an
 // MI355X, each XCD with its own L2.  xcd_slot() renumbers block b so that the
 // blocks of one XCD hold consecutive slots (persistent grids then give the

@@ -880,14 +880,7 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
       const T wp = static_cast<T>(a.wgt[p]);
       const size_t o = static_cast<size_t>(g) * NN + l * N;
       const T* b = a.b1 + static_cast<size_t>(p) * n + o;
-      cpx<T>* gp = Gh + o;
-      if (p == p0) {
-#pragma unroll
-        for (int kz = 0; kz < N; ++kz) gp[kz] = (wp * b[kz]) * v[kz];
-      } else {
-#pragma unroll
-        for (int kz = 0; kz < N; ++kz) gp[kz] = gp[kz] + (wp * b[kz]) * v[kz];
-      }
+      pencil_accumulate<T, N, N>(Gh + o, b, wp, v, p == p0);
     }
   }
 }
@@ -1226,12 +1219,15 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_finalize(FinArgs<T> a) {
     double* Q = a.Q + static_cast<size_t>(pt) * n;
     pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z[kx * NN + idx]; },
                               [&](int idx, cpx<T>* v) {
+                                // the pencil of f is loaded before Q is stored, so the loads are in
+                                // flight together (f and Q may overlap as far as the compiler knows)
+                                double fv[N];
 #pragma unroll
-                                for (int x = 0; x < N; ++x) {
-                                  const int o = x * NN + idx;
-                                  Q[o] = a.prefac * (static_cast<double>(v[x].x) -
-                                                     f[o] * static_cast<double>(v[x].y));
-                                }
+                                for (int x = 0; x < N; ++x) fv[x] = f[x * NN + idx];
+#pragma unroll
+                                for (int x = 0; x < N; ++x)
+                                  Q[x * NN + idx] = a.prefac * (static_cast<double>(v[x].x) -
+                                                                fv[x] * static_cast<double>(v[x].y));
                               });
   }
 }

@@ -455,14 +455,12 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1(VssGainArgs
         {
           const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
                                        (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off + iy]) * n + o);
-#pragma unroll
-          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
+          pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G, b1, w, v, false);
         }
         if (two) {
           const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
                                        (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off + iy]) * n + o);
-#pragma unroll
-          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
+          pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G2, b2, w, v, false);
         }
       }
     }
@@ -565,24 +563,12 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_iso(VssGain
       {
         const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
                                      (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off]) * n + o);
-        if (x == x0) {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G[k] = (w * b1[k]) * v[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
-        }
+        pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G, b1, w, v, x == x0);
       }
       if (two) {
         const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
                                      (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off]) * n + o);
-        if (x == x0) {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G2[k] = (w * b2[k]) * v[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
-        }
+        pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G2, b2, w, v, x == x0);
       }
     }
   }
@@ -752,24 +738,12 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
       {
         const StridedPencil<T, N> b1(const_cast<cpx<T>*>(jb.tab1) +
                                      (static_cast<size_t>(x) * jb.tabM1 + a.rows1[jb.dir_off]) * n + o);
-        if (x == x0) {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G[k] = (w * b1[k]) * v[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G[k] = G[k] + (w * b1[k]) * v[k];
-        }
+        pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G, b1, w, v, x == x0);
       }
       if (two) {
         const StridedPencil<T, N> b2(const_cast<cpx<T>*>(jb.tab2) +
                                      (static_cast<size_t>(x) * jb.tabM2 + a.rows2[jb.dir_off]) * n + o);
-        if (x == x0) {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G2[k] = (w * b2[k]) * v[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < N; ++k) G2[k] = G2[k] + (w * b2[k]) * v[k];
-        }
+        pencil_accumulate<T, N, (N <= 16 ? 8 : 4)>(G2, b2, w, v, x == x0);
       }
     }
   }
@@ -1119,11 +1093,13 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vss_finalize(VssFinArgs<T> a) 
     double* Q = a.Q[p] + static_cast<size_t>(pt) * n;
     pencil_pass<T, N, NT, +1>([&](int kx, int idx) { return Z[kx * NN + idx]; },
                               [&](int idx, cpx<T>* v) {
+                                // f's pencil loaded before Q is stored (loads in flight together)
+                                double fv[N];
 #pragma unroll
-                                for (int x = 0; x < N; ++x) {
-                                  const int o = x * NN + idx;
-                                  Q[o] = static_cast<double>(v[x].x) - f[o] * static_cast<double>(v[x].y);
-                                }
+                                for (int x = 0; x < N; ++x) fv[x] = f[x * NN + idx];
+#pragma unroll
+                                for (int x = 0; x < N; ++x)
+                                  Q[x * NN + idx] = static_cast<double>(v[x].x) - fv[x] * static_cast<double>(v[x].y);
                               });
   }
 }
