@@ -91,12 +91,12 @@ class Autotuner:
     @staticmethod
     def _vss_candidates(op, cus):
         """Two-species operator: the register-resident gain (and its packed
-        form for isotropic scattering) where built, the slab gain for FP64
+        form for isotropic scattering) where built, the slab gain for
         isotropic scattering, the multi-pass gain with its persistent-grid
         caps; radial groups 1, 2, 4, ... for each."""
         algos = [1]
         if op.precision == 32 and op.Nv in (16, 24):
-            algos[:0] = [4, 6] if op.iso else [4]
+            algos[:0] = [4, 6, 7] if op.iso else [4]
         if op.precision == 64 and op.iso and op.Nv in (16, 24):
             algos[:0] = [7]
         groups = [0]

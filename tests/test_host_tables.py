@@ -154,7 +154,7 @@ def test_autotune_candidates_for_both_operators():
     vss = types.SimpleNamespace(kind="vss", Nv=24, precision=32, K=144, nspcs=2, iso=True, project=True, Nrho=12)
     c = Autotuner.candidates(vss, 256)
     algos = sorted({x["algorithm"] for x in c})
-    assert algos == [1, 4, 6]
+    assert algos == [1, 4, 6, 7]
     assert all(x["slots"] == 0 for x in c if x["algorithm"] >= 4)
     assert {x["groups"] for x in c} == {0, 1, 2, 4, 8}
     vss.iso = False
