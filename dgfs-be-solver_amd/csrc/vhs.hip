@@ -691,10 +691,10 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
           }
           __syncthreads();
         } else {
-          // this (p, q)'s Nyquist planes into LDS by the load unit, in
-          // flight during the z, y transforms (read after later barriers;
-          // 67.9 -> 66.7 ms per 4096-point launch at 32^3 on MI355X against
-          // loading them through registers before the transforms)
+          // this (p, q)'s Nyquist planes into LDS by the load unit (the
+          // compiler completes the copy at the next barrier; 67.9 -> 66.7 ms
+          // per 4096-point launch at 32^3 on MI355X against staging them
+          // through registers and an LDS store)
           const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
           lds_copy16<NT, 3 * NN * static_cast<int>(sizeof(cpx<T>)) / 16>(Wq, Wl);
           __syncthreads();  // eph is in place
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_gain_rr(GainRRArgs<T> a)
         rr_zy_inverse<T, N, G, WL>(v1, Ft, Xb, eph, g + G, g, l);
         T val[N];
         cpx<T> u[N];
-        if constexpr (!INL) lds_copy_wait();  // the Nyquist planes have landed (read after the barrier)
+        if constexpr (!INL) lds_copy_wait();  // the Nyquist planes have landed (read after the next barrier)
         // target slot 0 (y = g): both sub-rounds
         __syncthreads();
         rr_x_write<T, N, G>(v0, Xb, 0, g, l);
