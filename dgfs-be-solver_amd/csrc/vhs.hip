@@ -546,14 +546,14 @@ __device__ __forceinline__ void rr_x_compute(T (&val)[N], cpx<T> (&u)[N], const 
 }
 
 // x-FFT of the slot-r accumulator and the reverse exchange into the y-half r
-// of both plane slots (two sub-rounds)
-template <typename T, int N, int G>
+// of both plane slots (two sub-rounds); TF: tangent-form butterflies
+template <typename T, int N, int G, bool TF = true>
 __device__ __forceinline__ void rr_x_forward(const T (&sacc)[N], cpx<T> (&v0)[N], cpx<T> (&v1)[N], cpx<T>* Xb,
                                              int r, int g, int l) {
   cpx<T> u[N];
 #pragma unroll
   for (int x = 0; x < N; ++x) u[x] = cpx<T>{sacc[x], T(0)};
-  FFT<T, N, -1, true>::run(u);
+  FFT<T, N, -1, TF>::run(u);
   __builtin_amdgcn_sched_barrier(0);
   cpx<T>* Xw = Xb + g * N + l;         // write (kxl, yy = g, z = l)
   const cpx<T>* Xr = Xb + (g * G) * N + l;
@@ -1232,6 +1232,247 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_finalize(FinArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident spectrum prep and finalize (FP32, N = 16, 32), in the
+// register-resident gain kernel's thread layout: a workgroup of N*N/2
+// threads per point, thread t = g N + l holding two pencils.  Neither stages
+// the field through a scratch cube in global memory, as the LDS-slab prep
+// and finalize do.  Their transforms use the plain butterflies (the
+// tangent form's extra rounding is not worth its saving outside the gain).
+//   prep:     x-pencils (y = g + (N/2) r, z = l) of f, x-FFT and the kx <-> y
+//             exchange (rr_x_forward), then per plane slot kx = g + (N/2) sr
+//             y-FFT, in-plane transpose, z-FFT; Ft = (1 + i) F / n, transposed
+//             to [kx][kz][ky], is stored from the z-pencils and F / n, after
+//             one more in-plane transpose, from the ky-pencils, so both
+//             stores are lane-coalesced.
+//   finalize: y-pencils (kx, kz = l) of sum_groups G^ + i b2 F (coalesced
+//             loads), y-IFFT, transpose, z-IFFT, transpose back to pencils
+//             over y at z = l, the gain kernel's exchange to x-pencils
+//             (y = g + (N/2) r, z = l), x-IFFT, Q = prefac (Re - f Im).  With
+//             PROJ the FP64 projection onto the five collision invariants
+//             follows in the same workgroup: Q of both pencils stays in
+//             registers while the moments are reduced over the workgroup,
+//             so Q is written once, corrected.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_prep_rr(PrepArgs<T> a) {
+  constexpr int G = RRShape<N>::G, NN = N * N, n = NN * N;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* Xb = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  const T rn = static_cast<T>(1.0 / static_cast<double>(n));
+  cpx<T> v0[N], v1[N];
+  for (int pt = xcd_slot(); pt < a.npts; pt += gridDim.x) {
+    const double* f = a.f + static_cast<size_t>(pt) * n;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      T s[N];
+      const double* fp = f + (g + G * r) * N + l;
+#pragma unroll
+      for (int x = 0; x < N; ++x) s[x] = static_cast<T>(fp[x * NN]);
+      rr_x_forward<T, N, G, false>(s, v0, v1, Xb, r, g, l);
+    }
+    cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    cpx<T>* Ft = a.Ft + static_cast<size_t>(pt) * n;
+    __syncthreads();  // the exchange buffer's readers are done before the wave-local transposes reuse it
+#pragma unroll
+    for (int sr = 0; sr < 2; ++sr) {
+      cpx<T>(&v)[N] = sr ? v1 : v0;
+      const int kx = g + G * sr;
+      FFT<T, N, -1>::run(v);  // y: (kx, ky, z = l)
+      __builtin_amdgcn_sched_barrier(0);
+      rr_plane_transpose<T, N, G, true, false>(v, Xb, g, l);  // (kx, ky = l, z)
+      FFT<T, N, -1>::run(v);                            // z: (kx, ky = l, kz)
+      __builtin_amdgcn_sched_barrier(0);
+      cpx<T>* ft = Ft + kx * NN + l;
+#pragma unroll
+      for (int kz = 0; kz < N; ++kz) ft[kz * N] = rn * cpx<T>{v[kz].x - v[kz].y, v[kz].x + v[kz].y};  // (1 + i) F
+      rr_plane_transpose<T, N, G, true, true>(v, Xb, g, l);  // (kx, ky, kz = l)
+      cpx<T>* fp = F + kx * NN + l;
+#pragma unroll
+      for (int ky = 0; ky < N; ++ky) fp[ky * N] = rn * v[ky];
+    }
+  }
+}
+
+template <typename T>
+struct FinRRArgs {
+  FinArgs<T> fa;
+  double Ainv[25];  // inverse Gram matrix of the invariants (projection)
+  double L, cw;
+};
+
+template <typename T, int N, bool PROJ>
+__global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_finalize_rr(FinRRArgs<T> ar) {
+  constexpr int NT = RRShape<N>::NT, G = RRShape<N>::G, NN = N * N, n = NN * N, NW = NT / 64;
+  const FinArgs<T>& a = ar.fa;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* Xb = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ double red[NW][5];
+  __shared__ double lam[5];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  const double hvc = 2.0 * ar.L / N;
+  cpx<T> v0[N], v1[N];
+  for (int pt = xcd_slot(); pt < a.npts; pt += gridDim.x) {
+    const cpx<T>* Gp = a.Ghat + static_cast<size_t>(pt) * a.ngroups * n;
+    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n;
+    // opaque per point, so the compiler does not hoist the b2 addresses and
+    // the velocity abscissae of every pencil element out of the point loop
+    // (they would hold over a hundred registers)
+    int zero = 0;
+    asm volatile("" : "+v"(zero));
+    const T* b2 = a.b2 + zero;
+    const double hv = hvc + static_cast<double>(zero), c0 = -ar.L + 0.5 * hv;
+#pragma unroll
+    for (int sr = 0; sr < 2; ++sr) {
+      cpx<T>(&v)[N] = sr ? v1 : v0;
+      const int o0 = (g + G * sr) * NN + l;  // y-pencil (kx, kz = l)
+#pragma unroll
+      for (int k0 = 0; k0 < N; k0 += 8) {  // 8 values in flight at a time (register pressure)
+#pragma unroll
+        for (int ky = k0; ky < k0 + 8; ++ky) v[ky] = Gp[o0 + ky * N];
+        for (int gr = 1; gr < a.ngroups; ++gr) {  // the other radial groups' partial sums
+          const cpx<T>* Gg = Gp + static_cast<size_t>(gr) * n + o0;
+#pragma unroll
+          for (int ky = k0; ky < k0 + 8; ++ky) v[ky] = v[ky] + Gg[ky * N];
+        }
+        // the loss term after the whole gain sum (added to one group's
+        // partial sum, b2 F is large enough against it to cost precision)
+#pragma unroll
+        for (int ky = k0; ky < k0 + 8; ++ky) {
+          const int o = o0 + ky * N;
+          v[ky] = v[ky] + mul_i(b2[o] * F[o]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      FFT<T, N, +1>::run(v);  // y: (kx, y, kz = l)
+      __builtin_amdgcn_sched_barrier(0);
+      if (sr == 0) __syncthreads();  // the previous point's exchange readers are done
+      rr_plane_transpose<T, N, G, true, false>(v, Xb, g, l);  // (kx, y = l, kz)
+      FFT<T, N, +1>::run(v);                            // z: (kx, y = l, z)
+      __builtin_amdgcn_sched_barrier(0);
+      rr_plane_transpose<T, N, G, true, true>(v, Xb, g, l);  // (kx, y, z = l)
+    }
+    // exchange to the x-pencils of both target slots (v0 and v1 are free
+    // after their last writes), then the x-IFFTs
+    cpx<T> u0[N], u1[N];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      cpx<T>(&u)[N] = r ? u1 : u0;
+      __syncthreads();
+      rr_x_write<T, N, G>(v0, Xb, r, g, l);
+      __syncthreads();
+      rr_x_read<T, N, G>(u, Xb, 0, g, l);
+      __syncthreads();
+      rr_x_write<T, N, G>(v1, Xb, r, g, l);
+      __syncthreads();
+      rr_x_read<T, N, G>(u, Xb, 1, g, l);
+    }
+    FFT<T, N, +1>::run(u0);  // x: (x, y = g, z = l)
+    __builtin_amdgcn_sched_barrier(0);
+    FFT<T, N, +1>::run(u1);  // x: (x, y = g + G, z = l)
+    __builtin_amdgcn_sched_barrier(0);
+    const double* f = a.f + static_cast<size_t>(pt) * n;
+    double* Q = a.Q + static_cast<size_t>(pt) * n;
+    const double cz = c0 + l * hv;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, l4 = 0.0;
+    if constexpr (PROJ) {
+      // m = sum over the thread's two pencils of Q phi_j, phi = (1, c, |c|^2);
+      // the y and z parts are applied to the pencil sums
+      double s0[2] = {0.0, 0.0}, sx = 0.0, sxx = 0.0;
+      const double* fp0 = f + g * N + l;
+      const double* fp1 = fp0 + G * N;
+#pragma unroll
+      for (int x0 = 0; x0 < N; x0 += 8) {
+#pragma unroll
+        for (int x = x0; x < x0 + 8; ++x) {
+          const double qa = a.prefac * (static_cast<double>(u0[x].x) - fp0[x * NN] * static_cast<double>(u0[x].y));
+          const double qb = a.prefac * (static_cast<double>(u1[x].x) - fp1[x * NN] * static_cast<double>(u1[x].y));
+          const double cx = c0 + x * hv, qs = qa + qb;
+          s0[0] += qa;
+          s0[1] += qb;
+          sx += qs * cx;
+          sxx += qs * (cx * cx);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const double cy0 = c0 + g * hv, cy1 = c0 + (g + G) * hv, st = s0[0] + s0[1];
+      double m[5] = {st, sx, s0[0] * cy0 + s0[1] * cy1, st * cz,
+                     sxx + s0[0] * (cy0 * cy0 + cz * cz) + s0[1] * (cy1 * cy1 + cz * cz)};
+      // moments over the workgroup: within each wave, then across the waves
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        double sj = m[j];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) sj += __shfl_xor(sj, d);
+        m[j] = sj;
+      }
+      if ((tid & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) red[tid >> 6][j] = m[j];
+      }
+      __syncthreads();
+      if (tid < 5) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          double sk = 0.0;
+#pragma unroll
+          for (int w = 0; w < NW; ++w) sk += red[w][k];
+          acc += ar.Ainv[tid * 5 + k] * (sk * ar.cw);
+        }
+        lam[tid] = acc;
+      }
+      __syncthreads();
+      l0 = lam[0]; l1 = lam[1]; l2 = lam[2]; l3 = lam[3]; l4 = lam[4];
+    }
+    // Q = prefac (Re - f Im), less the projection's correction.  f is
+    // re-read for it (L2 hits after the moment pass) through an opaque
+    // offset: otherwise the compiler keeps the moment pass's 64 values of Q
+    // per thread live across the reduction and spills them
+    int zf = 0;
+    asm volatile("" : "+v"(zf));
+    // likewise the x-pencils and the abscissae: the moment pass's FP64
+    // conversions and |c|^2 values are recomputed rather than kept
+    double hq = hv;
+    asm volatile("" : "+v"(hq));
+    const double cq = -ar.L + 0.5 * hq, czq = cq + l * hq;
+    if constexpr (PROJ) {
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        asm volatile("" : "+v"(u0[x].x), "+v"(u0[x].y));
+        asm volatile("" : "+v"(u1[x].x), "+v"(u1[x].y));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const cpx<T>(&u)[N] = r ? u1 : u0;
+      const int y = g + G * r;
+      const double cy = cq + y * hq;
+      const double* fp = f + y * N + l + zf;
+      double* qp = Q + y * N + l;
+#pragma unroll
+      for (int x0 = 0; x0 < N; x0 += 8) {
+        double fv[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) fv[x] = fp[(x0 + x) * NN];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          double q = a.prefac * (static_cast<double>(u[x0 + x].x) - fv[x] * static_cast<double>(u[x0 + x].y));
+          if constexpr (PROJ) {
+            const double cx = cq + (x0 + x) * hq;
+            q -= ar.cw * (l0 + l1 * cx + l2 * cy + l3 * czq + l4 * (cx * cx + cy * cy + czq * czq));
+          }
+          qp[(x0 + x) * NN] = q;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+}
+
 // FP64 projection of Q onto the 5 collision invariants {1, c, |c|^2}: the
 // minimum-norm correction d = -C^T (C C^T)^{-1} C Q with C_jr = phi_j(r) cw.
 struct ProjArgs {
@@ -1507,6 +1748,14 @@ static int occupancy_slots() {
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o3, reinterpret_cast<const void*>(k_vhs_finalize<T, N>), Shape<N>::NT, 0);
   occ = std::max(1, std::min(o1, std::min(o2, o3)));
   return occ * cu_count();
+}
+
+// resident workgroups of a register-resident prep or finalize kernel over
+// the device (the persistent grid size)
+static int rr_edge_grid(const void* kfn, int threads, size_t lds, int npts) {
+  int o = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, kfn, threads, lds) != hipSuccess || o < 1) o = 1;
+  return std::max(1, std::min(npts, o * cu_count()));
 }
 
 // the register-resident gain kernels exist for FP32: two pencils per thread
@@ -1787,8 +2036,22 @@ static int stage_prep(VhsHandle* h, const double* fc, int npts, int groups, int 
   PrepArgs<T> pa{fc, static_cast<cpx<T>*>(h->F), rr ? static_cast<cpx<T>*>(h->Ft) : nullptr,
                  static_cast<cpx<T>*>(h->Z), npts};
   KTimerT<VhsHandle> t0(h, 0, st);
-  hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, pa);
-  DGFS_LAUNCH_CHECK("vhs_prep");
+  bool done = false;
+  if constexpr (rr_available<T, N>()) {
+    if (rr) {  // register-resident prep: F and Ft without the scratch cube
+      const size_t lds = static_cast<size_t>(RRLds<N>::xsize) * sizeof(cpx<T>);
+      DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vhs_prep_rr<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      const int grid = rr_edge_grid(reinterpret_cast<const void*>(k_vhs_prep_rr<T, N>), RRShape<N>::NT, lds, npts);
+      hipLaunchKernelGGL((k_vhs_prep_rr<T, N>), dim3(grid), dim3(RRShape<N>::NT), lds, st, pa);
+      DGFS_LAUNCH_CHECK("vhs_prep_rr");
+      done = true;
+    }
+  }
+  if (!done) {
+    hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, pa);
+    DGFS_LAUNCH_CHECK("vhs_prep");
+  }
   t0.stop();
   *ga = GainArgs<T>{static_cast<const cpx<T>*>(h->F), static_cast<cpx<T>*>(h->Ghat),
                     static_cast<cpx<T>*>(h->Z), static_cast<T*>(h->S), static_cast<cpx<T>*>(h->W),
@@ -1803,6 +2066,27 @@ static int stage_finish(VhsHandle* h, const double* fc, double* Qc, int npts, in
   FinArgs<T> fa{static_cast<const cpx<T>*>(h->Ghat), static_cast<const cpx<T>*>(h->F),
                 static_cast<const T*>(h->b2), fc, Qc, static_cast<cpx<T>*>(h->Z),
                 h->prefac, npts, groups};
+  if constexpr (rr_available<T, N>()) {
+    // register-resident finalize, with the projection in the same kernel
+    FinRRArgs<T> fr;
+    fr.fa = fa;
+    std::memcpy(fr.Ainv, h->Ainv, sizeof(fr.Ainv));
+    fr.L = h->L;
+    fr.cw = h->cw;
+    const size_t lds = static_cast<size_t>(RRLds<N>::xsize) * sizeof(cpx<T>);
+    const void* kfn = h->project ? reinterpret_cast<const void*>(k_vhs_finalize_rr<T, N, true>)
+                                 : reinterpret_cast<const void*>(k_vhs_finalize_rr<T, N, false>);
+    DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const dim3 grid(rr_edge_grid(kfn, RRShape<N>::NT, lds, npts));
+    KTimerT<VhsHandle> t2(h, 2, st);
+    if (h->project)
+      hipLaunchKernelGGL((k_vhs_finalize_rr<T, N, true>), grid, dim3(RRShape<N>::NT), lds, st, fr);
+    else
+      hipLaunchKernelGGL((k_vhs_finalize_rr<T, N, false>), grid, dim3(RRShape<N>::NT), lds, st, fr);
+    DGFS_LAUNCH_CHECK("vhs_finalize_rr");
+    t2.stop();
+    return DGFS_OK;
+  }
   KTimerT<VhsHandle> t2(h, 2, st);
   hipLaunchKernelGGL((k_vhs_finalize<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, fa);
   DGFS_LAUNCH_CHECK("vhs_finalize");
