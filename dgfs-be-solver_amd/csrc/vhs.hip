@@ -800,8 +800,9 @@ __global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_gain_rr1(GainRRArgs<T> 
         }
         {
           // this (p, q)'s Nyquist planes through registers: their loads overlap
-          // the z-pencil loads below (a copy by the load unit, waited for before
-          // the products, measured 2.5 % slower here on MI355X)
+          // the z-pencil loads below (a copy by the load unit, which the
+          // compiler completes at the next barrier, measured 2.5 % slower
+          // here on MI355X)
           const cpx<T>* Wq = a.W + (static_cast<size_t>(pt) * K + p * a.M + q) * 3 * NN;
           static_assert((3 * NN) % NT == 0, "Nyquist planes split evenly over the workgroup");
           cpx<T> wv[3 * NN / NT];
@@ -1232,6 +1233,42 @@ __global__ __launch_bounds__(Shape<N>::NT) void k_vhs_finalize(FinArgs<T> a) {
   }
 }
 
+// Moments of Q over one point's velocity grid, reduced over the workgroup
+// (wave shuffles, then LDS), and the projection's multipliers lambda =
+// Ainv (cw m).  m: this thread's partial sums of Q (1, cx, cy, cz, |c|^2);
+// red holds NW x 5 doubles; every thread returns the five multipliers.
+template <int NW>
+__device__ __forceinline__ void block_invariant_multipliers(double (&m)[5], double (*red)[5], double* lam,
+                                                            const double* Ainv, double cw) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double sj = m[j];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) sj += __shfl_xor(sj, d);
+    m[j] = sj;
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) red[tid >> 6][j] = m[j];
+  }
+  __syncthreads();
+  if (tid < 5) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double sk = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sk += red[w][k];
+      acc += Ainv[tid * 5 + k] * (sk * cw);
+    }
+    lam[tid] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 5; ++j) m[j] = lam[j];
+}
+
 // ---------------------------------------------------------------------------
 // Register-resident spectrum prep and finalize (FP32, N = 16, 32), in the
 // register-resident gain kernel's thread layout: a workgroup of N*N/2
@@ -1401,32 +1438,8 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_finalize_rr(FinRRArgs<T>
       const double cy0 = c0 + g * hv, cy1 = c0 + (g + G) * hv, st = s0[0] + s0[1];
       double m[5] = {st, sx, s0[0] * cy0 + s0[1] * cy1, st * cz,
                      sxx + s0[0] * (cy0 * cy0 + cz * cz) + s0[1] * (cy1 * cy1 + cz * cz)};
-      // moments over the workgroup: within each wave, then across the waves
-#pragma unroll
-      for (int j = 0; j < 5; ++j) {
-        double sj = m[j];
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) sj += __shfl_xor(sj, d);
-        m[j] = sj;
-      }
-      if ((tid & 63) == 0) {
-#pragma unroll
-        for (int j = 0; j < 5; ++j) red[tid >> 6][j] = m[j];
-      }
-      __syncthreads();
-      if (tid < 5) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          double sk = 0.0;
-#pragma unroll
-          for (int w = 0; w < NW; ++w) sk += red[w][k];
-          acc += ar.Ainv[tid * 5 + k] * (sk * ar.cw);
-        }
-        lam[tid] = acc;
-      }
-      __syncthreads();
-      l0 = lam[0]; l1 = lam[1]; l2 = lam[2]; l3 = lam[3]; l4 = lam[4];
+      block_invariant_multipliers<NW>(m, red, lam, ar.Ainv, ar.cw);
+      l0 = m[0]; l1 = m[1]; l2 = m[2]; l3 = m[3]; l4 = m[4];
     }
     // Q = prefac (Re - f Im), less the projection's correction.  f is
     // re-read for it (L2 hits after the moment pass) through an opaque
@@ -1470,6 +1483,142 @@ __global__ __launch_bounds__(RRShape<N>::NT) void k_vhs_finalize_rr(FinRRArgs<T>
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+  }
+}
+
+// One-pencil register-resident prep and finalize (FP32, N = 24), in the
+// one-pencil gain kernel's layout: N*N threads per point, thread t = g N + l,
+// the field exchanged through LDS ([kx][y][z], kx stride N (N + 1)) and the
+// in-plane transposes through padded kx planes.
+//   prep:     x-pencil (y = g, z = l) of f, x-FFT, field exchange to the
+//             y-pencil (kx = g, z = l), y-FFT, transpose, z-FFT: Ft stored
+//             from the z-pencil, F after one more transpose (both coalesced).
+//   finalize: y-pencil (kx = g, kz = l) of sum_groups G^ + i b2 F, y-IFFT,
+//             transpose, z-IFFT, transpose to the pencil over y at z = l,
+//             field exchange to the x-pencil (y = g, z = l), x-IFFT,
+//             Q = prefac (Re - f Im), with PROJ the projection in the kernel.
+template <typename T, int N>
+__global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_prep_rr1(PrepArgs<T> a) {
+  constexpr int PP = RR1Shape<N>::PP, KS = RR1Shape<N>::KS, NN = N * N, n = NN * N;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  cpx<T>* Xp = X + g * N * PP;  // kx plane g
+  const T rn = static_cast<T>(1.0 / static_cast<double>(n));
+  cpx<T> v[N];
+  for (int pt = xcd_slot(); pt < a.npts; pt += gridDim.x) {
+    const double* fp = a.f + static_cast<size_t>(pt) * n + g * N + l;
+#pragma unroll
+    for (int x = 0; x < N; ++x) v[x] = cpx<T>{static_cast<T>(fp[x * NN]), T(0)};
+    FFT<T, N, -1>::run(v);  // x: (kx, y = g, z = l)
+    __syncthreads();        // the previous point's readers of the planes are done
+#pragma unroll
+    for (int kx = 0; kx < N; ++kx) X[kx * KS + g * N + l] = v[kx];
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < N; ++y) v[y] = X[g * KS + y * N + l];
+    FFT<T, N, -1>::run(v);  // y: (kx = g, ky, z = l)
+    __syncthreads();
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) Xp[ky * PP + l] = v[ky];
+    __syncthreads();
+#pragma unroll
+    for (int z = 0; z < N; ++z) v[z] = Xp[l * PP + z];
+    FFT<T, N, -1>::run(v);  // z: (kx = g, ky = l, kz)
+    cpx<T>* ft = a.Ft + static_cast<size_t>(pt) * n + g * NN + l;
+#pragma unroll
+    for (int kz = 0; kz < N; ++kz) ft[kz * N] = rn * cpx<T>{v[kz].x - v[kz].y, v[kz].x + v[kz].y};  // (1 + i) F
+    __syncthreads();
+#pragma unroll
+    for (int kz = 0; kz < N; ++kz) Xp[l * PP + kz] = v[kz];
+    __syncthreads();
+    cpx<T>* fo = a.F + static_cast<size_t>(pt) * n + g * NN + l;
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) fo[ky * N] = rn * Xp[ky * PP + l];
+  }
+}
+
+template <typename T, int N, bool PROJ>
+__global__ __launch_bounds__(RR1Shape<N>::NT) void k_vhs_finalize_rr1(FinRRArgs<T> ar) {
+  constexpr int NT = RR1Shape<N>::NT, PP = RR1Shape<N>::PP, KS = RR1Shape<N>::KS, NN = N * N, n = NN * N;
+  constexpr int NW = (NT + 63) / 64;
+  const FinArgs<T>& a = ar.fa;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  cpx<T>* X = reinterpret_cast<cpx<T>*>(smem_raw);
+  __shared__ double red[NW][5];
+  __shared__ double lam[5];
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid - (tid / N) * N;
+  cpx<T>* Xp = X + g * N * PP;  // kx plane g
+  const double hvc = 2.0 * ar.L / N;
+  cpx<T> v[N];
+  for (int pt = xcd_slot(); pt < a.npts; pt += gridDim.x) {
+    const cpx<T>* Gp = a.Ghat + static_cast<size_t>(pt) * a.ngroups * n + g * NN + l;
+    const cpx<T>* F = a.F + static_cast<size_t>(pt) * n + g * NN + l;
+    const T* b2 = a.b2 + g * NN + l;
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) v[ky] = Gp[ky * N];
+    for (int gr = 1; gr < a.ngroups; ++gr) {  // the other radial groups' partial sums
+      const cpx<T>* Gg = Gp + static_cast<size_t>(gr) * n;
+#pragma unroll
+      for (int ky = 0; ky < N; ++ky) v[ky] = v[ky] + Gg[ky * N];
+    }
+#pragma unroll
+    for (int ky = 0; ky < N; ++ky) v[ky] = v[ky] + mul_i(b2[ky * N] * F[ky * N]);  // loss term after the gain sum
+    FFT<T, N, +1>::run(v);  // y: (kx = g, y, kz = l)
+    __syncthreads();        // the previous point's readers of the planes are done
+#pragma unroll
+    for (int y = 0; y < N; ++y) Xp[y * PP + l] = v[y];
+    __syncthreads();
+#pragma unroll
+    for (int kz = 0; kz < N; ++kz) v[kz] = Xp[l * PP + kz];
+    FFT<T, N, +1>::run(v);  // z: (kx = g, y = l, z)
+    __syncthreads();
+#pragma unroll
+    for (int z = 0; z < N; ++z) Xp[l * PP + z] = v[z];
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < N; ++y) v[y] = Xp[y * PP + l];  // (kx = g, y, z = l)
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < N; ++y) X[g * KS + y * N + l] = v[y];
+    __syncthreads();
+#pragma unroll
+    for (int kx = 0; kx < N; ++kx) v[kx] = X[kx * KS + g * N + l];
+    FFT<T, N, +1>::run(v);  // x: (x, y = g, z = l)
+    const double* fp = a.f + static_cast<size_t>(pt) * n + g * N + l;
+    double* qp = a.Q + static_cast<size_t>(pt) * n + g * N + l;
+    double q[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+      q[x] = a.prefac * (static_cast<double>(v[x].x) - fp[x * NN] * static_cast<double>(v[x].y));
+    if constexpr (PROJ) {
+      // abscissae from an opaque grid step, so the compiler does not hoist
+      // all N of them (and their squares) out of the point loop
+      double hv = hvc;
+      asm volatile("" : "+v"(hv));
+      const double c0 = -ar.L + 0.5 * hv, cy = c0 + g * hv, cz = c0 + l * hv;
+      double sq = 0.0, sx = 0.0, sxx = 0.0;
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const double cx = c0 + x * hv;
+        sq += q[x];
+        sx += q[x] * cx;
+        sxx += q[x] * (cx * cx);
+      }
+      double m[5] = {sq, sx, sq * cy, sq * cz, sxx + sq * (cy * cy + cz * cz)};
+      block_invariant_multipliers<NW>(m, red, lam, ar.Ainv, ar.cw);
+      asm volatile("" : "+v"(hv));  // recomputed, not kept across the reduction
+      const double c0b = -ar.L + 0.5 * hv, cyb = c0b + g * hv, czb = c0b + l * hv;
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const double cx = c0b + x * hv;
+        q[x] -= ar.cw * (m[0] + m[1] * cx + m[2] * cyb + m[3] * czb + m[4] * (cx * cx + cyb * cyb + czb * czb));
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < N; ++x) qp[x * NN] = q[x];
   }
 }
 
@@ -2047,6 +2196,16 @@ static int stage_prep(VhsHandle* h, const double* fc, int npts, int groups, int 
       DGFS_LAUNCH_CHECK("vhs_prep_rr");
       done = true;
     }
+  } else if constexpr (rr1_available<T, N>()) {
+    if (rr) {  // one-pencil register-resident prep
+      const size_t lds = static_cast<size_t>(RR1Shape<N>::field) * sizeof(cpx<T>);
+      DGFS_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vhs_prep_rr1<T, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      const int grid = rr_edge_grid(reinterpret_cast<const void*>(k_vhs_prep_rr1<T, N>), RR1Shape<N>::NT, lds, npts);
+      hipLaunchKernelGGL((k_vhs_prep_rr1<T, N>), dim3(grid), dim3(RR1Shape<N>::NT), lds, st, pa);
+      DGFS_LAUNCH_CHECK("vhs_prep_rr1");
+      done = true;
+    }
   }
   if (!done) {
     hipLaunchKernelGGL((k_vhs_prep<T, N>), dim3(std::min(npts, h->cap_slots)), dim3(Shape<N>::NT), 0, st, pa);
@@ -2084,6 +2243,25 @@ static int stage_finish(VhsHandle* h, const double* fc, double* Qc, int npts, in
     else
       hipLaunchKernelGGL((k_vhs_finalize_rr<T, N, false>), grid, dim3(RRShape<N>::NT), lds, st, fr);
     DGFS_LAUNCH_CHECK("vhs_finalize_rr");
+    t2.stop();
+    return DGFS_OK;
+  } else if constexpr (rr1_available<T, N>()) {
+    FinRRArgs<T> fr;
+    fr.fa = fa;
+    std::memcpy(fr.Ainv, h->Ainv, sizeof(fr.Ainv));
+    fr.L = h->L;
+    fr.cw = h->cw;
+    const size_t lds = static_cast<size_t>(RR1Shape<N>::field) * sizeof(cpx<T>);
+    const void* kfn = h->project ? reinterpret_cast<const void*>(k_vhs_finalize_rr1<T, N, true>)
+                                 : reinterpret_cast<const void*>(k_vhs_finalize_rr1<T, N, false>);
+    DGFS_HIP_TRY(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const dim3 grid(rr_edge_grid(kfn, RR1Shape<N>::NT, lds, npts));
+    KTimerT<VhsHandle> t2(h, 2, st);
+    if (h->project)
+      hipLaunchKernelGGL((k_vhs_finalize_rr1<T, N, true>), grid, dim3(RR1Shape<N>::NT), lds, st, fr);
+    else
+      hipLaunchKernelGGL((k_vhs_finalize_rr1<T, N, false>), grid, dim3(RR1Shape<N>::NT), lds, st, fr);
+    DGFS_LAUNCH_CHECK("vhs_finalize_rr1");
     t2.stop();
     return DGFS_OK;
   }

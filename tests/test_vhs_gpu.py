@@ -119,6 +119,40 @@ def test_projection_conserves(device):
     assert rel(outs[32], outs[64]) < 1e-3
 
 
+@pytest.mark.parametrize("N,L,Nrho,M", [(16, 6.0, 8, 6), (24, 7.0, 12, 12), (32, 6.0, 16, 12)])
+def test_projection_in_the_register_resident_finalize(N, L, Nrho, M, device):
+    """FP32 at Nv 16 / 24 / 32 projects inside the register-resident finalize
+    (moments reduced over the workgroup, Q written once).  The projection is
+    linear and FP64: it must equal the oracle's FP64 projection of the same
+    operator's unprojected FP32 output to roundoff, conserve the invariants,
+    and stay within the FP32 bar of the projected FP64 oracle."""
+    ov = O.VelocityMesh(N, L, Nrho, M)
+    f = smooth_states(ov, 5, seed=3 * N)
+    op_p, _ = make_op(N, L, Nrho, M, 0.38, 1.0, precision=32, project=True)
+    op_u, _ = make_op(N, L, Nrho, M, 0.38, 1.0, precision=32, project=False)
+    for groups in (0, 3):
+        say("fused projection N", N, "groups", groups)
+        op_p.set_groups(groups)
+        op_u.set_groups(groups)
+        Qp = run_contig(op_p, f, device)
+        Qu = run_contig(op_u, f, device)
+        e = rel(Qp, O.project_invariants(ov, Qu))
+        inv = O.collision_invariants(ov, Qp)
+        # residuals against int |Q| (1 + |c|^2): the energy moment's own weight
+        # reaches 3 L^2, so a bar on int |Q| alone holds it to a tighter
+        # relative standard than the mass
+        absq = O.collision_invariants(ov, np.abs(Qp))
+        scale = absq[:, 0:1] + absq[:, 4:5]
+        say("  vs projecting the unprojected output", e, "max |invariant| / int|Q|(1+|c|^2)",
+            np.abs(inv / scale).max())
+        assert e < 1e-12
+        assert np.all(np.abs(inv) <= 1e-12 * scale)
+    Qref_p = O.project_invariants(ov, O.VHSOracle(ov, 0.38, 1.0).q(f))
+    e32 = rel(Qp, Qref_p)
+    say("  vs the projected FP64 oracle", e32)
+    assert e32 < 1e-3
+
+
 def test_bkw_on_gpu(device):
     op, vm = make_op(32, 6.0, 16, 6, 0.0, 1.0 / (4 * math.pi))
     ov = O.VelocityMesh(32, 6.0, 16, 6)
