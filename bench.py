@@ -302,13 +302,15 @@ def bench_vss(args, dev, stream):
     gain_ms, gain_n = tim["gain"]
     algo = op32.get_algorithm()
     kname = {4: f"k_vss_gain_rr1<float,{nv}>", 5: f"k_vss_gain_rr1_iso<float,{nv}>",
-             6: f"k_vss_gain_rr1_isop<float,{nv}>"}.get(algo, f"k_vss_gain<float,{nv}>")
+             6: f"k_vss_gain_rr1_isop<float,{nv}>",
+             7: f"k_vss_gain_slab<float,{nv}> + k_vss_xfwd<float,{nv}>"}.get(algo, f"k_vss_gain<float,{nv}>")
     bq = vss_bytes(nv, K, S, 8)
     achieved = bq * P * args.vss_steps / (gain_ms / 1e3)
     # HBM bytes per launch from the committed counter runs, scaled to this batch
     tr = read_traffic({4: f"vss_gain_rr1_fp32_n{nv}_traffic.json",
                        5: f"vss_gain_rr1_iso_fp32_n{nv}_traffic.json",
-                       6: f"vss_gain_rr1_isop_fp32_n{nv}_traffic.json"}.get(algo, ""))
+                       6: f"vss_gain_rr1_isop_fp32_n{nv}_traffic.json",
+                       7: f"vss_gain_slab_fp32_n{nv}_traffic.json"}.get(algo, ""))
     traffic = (tr["bytes_per_dispatch"] * P * args.vss_steps / max(gain_n, 1) / tr["points_per_dispatch"]
                if tr else None)
     out = {

@@ -774,15 +774,24 @@ __global__ __launch_bounds__(VssRR1Shape<N>::NT) void k_vss_gain_rr1_isop(VssGai
 // jobs, tables and sums as k_vss_gain<T, N, true>; only the order of the
 // transforms' stages differs.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, typename T = double>
 struct VssSlabShape {
-  // slabs per point: the two FP64 slabs fill the LDS at NS = 3 for 24^3;
-  // 16^3 fits whole
-  static constexpr int NS = (N == 24) ? 3 : 1;
+  // slabs per point: the two FP64 slabs fill the LDS at NS = 3 for 24^3
+  // (FP32: NS = 2, 576 threads); 16^3 fits whole
+  static constexpr int NS = (N == 24) ? (sizeof(T) == 4 ? 2 : 3) : 1;
   static constexpr int XS = N / NS;
   static constexpr int NH = XS * N;  // threads per slab: one row of it each
   static constexpr int NT = 2 * NH;  // the t3 half and the t4 half of the workgroup
-  static constexpr int planes = XS * N * N;  // complex, per slab
+  // plane rows padded to N + 1 (row and column accesses conflict free with
+  // immediate offsets); swizzled rows made the compiler hold one LDS offset
+  // per element across the item loop and spill (MI355X, 24^3: FP64 100.7 ->
+  // 84.1 ms, FP32 with NS = 2 50.1 -> 42.5 ms per 2048 mixture points)
+  static constexpr int RP = N + 1;
+  static constexpr int PL = N * RP;          // complex per plane
+  static constexpr int planes = XS * PL;     // complex per slab (storage)
+  static constexpr int cells = XS * N * N;   // values per slab
+  // offset of row r, column c within a plane
+  static __device__ __forceinline__ int at(int r, int c) { return r * RP + c; }
   static_assert(NS * XS == N, "slabs must tile the x axis");
   static_assert((N * N) % NH == 0, "x-pass pencils split evenly over a half");
   static_assert(N % 2 == 0, "the product's y range splits in halves");
@@ -798,7 +807,7 @@ struct VssSlabArgs {
 template <typename T, int N, int S>
 __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cpx<T>* X, const cpx<T> (*e)[N],
                                                 int th) {
-  using SS = VssSlabShape<N>;
+  using SS = VssSlabShape<N, T>;
   constexpr int NS = SS::NS, XS = SS::XS, NH = SS::NH, NN = N * N;
 #pragma unroll 1
   for (int i = 0; i < NN / NH; ++i) {
@@ -819,16 +828,24 @@ __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cp
       U[r] = cpx<T>{T(0), T(0)};
       ld[0][r] = P[r * NN];
     }
+    // (with the 576-thread FP32 slab, NS = 2, registers allow one term at a
+    // time: the prefetch spilled)
+    constexpr bool PF = SS::NT <= 512;
     static_for<0, NS>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      if constexpr (m + 1 < NS) {
+      if constexpr (PF && m + 1 < NS) {
 #pragma unroll
         for (int r = 0; r < XS; ++r) ld[(m + 1) & 1][r] = P[(r + XS * (m + 1)) * NN];
+      }
+      if constexpr (!PF && m > 0) {
+#pragma unroll
+        for (int r = 0; r < XS; ++r) ld[m & 1][r] = P[(r + XS * m) * NN];
       }
       static_for<0, XS>([&](auto rc) {
         constexpr int r = decltype(rc)::value, kx = r + XS * m;
         U[r] = U[r] + FFT<T, NS, +1>::template tw<(m * S) % NS>(ld[m & 1][r] * ex[kx]);
       });
+      if constexpr (!PF) __builtin_amdgcn_sched_barrier(0);
     });
     static_for<0, XS>([&](auto rc) {
       constexpr int r = decltype(rc)::value;
@@ -836,7 +853,7 @@ __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cp
     });
     FFT<T, XS, +1, true>::run(U);
 #pragma unroll
-    for (int j = 0; j < XS; ++j) X[j * NN + ky * N + nyq_swizzle<N>(kz, ky)] = U[j];
+    for (int j = 0; j < XS; ++j) X[j * SS::PL + SS::at(ky, kz)] = U[j];
   }
 }
 
@@ -845,11 +862,14 @@ __device__ __forceinline__ void vss_slab_x_pass(const cpx<T>* __restrict__ F, cp
 template <typename T, int N>
 __device__ __forceinline__ void vss_slab_x_dispatch(int s, const cpx<T>* F, cpx<T>* X, const cpx<T> (*e)[N],
                                                     int th) {
-  constexpr int NS = VssSlabShape<N>::NS;
+  constexpr int NS = VssSlabShape<N, T>::NS;
   if constexpr (NS == 1) {
     vss_slab_x_pass<T, N, 0>(F, X, e, th);
+  } else if constexpr (NS == 2) {
+    if (s == 0) vss_slab_x_pass<T, N, 0>(F, X, e, th);
+    else vss_slab_x_pass<T, N, 1>(F, X, e, th);
   } else {
-    static_assert(NS == 3, "slab counts 1 and 3");
+    static_assert(NS == 3, "slab counts 1, 2 and 3");
     if (s == 0) vss_slab_x_pass<T, N, 0>(F, X, e, th);
     else if (s == 1) vss_slab_x_pass<T, N, 1>(F, X, e, th);
     else vss_slab_x_pass<T, N, 2>(F, X, e, th);
@@ -857,8 +877,8 @@ __device__ __forceinline__ void vss_slab_x_dispatch(int s, const cpx<T>* F, cpx<
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabArgs<T> sa) {
-  using SS = VssSlabShape<N>;
+__global__ __launch_bounds__((VssSlabShape<N, T>::NT)) void k_vss_gain_slab(VssSlabArgs<T> sa) {
+  using SS = VssSlabShape<N, T>;
   constexpr int NS = SS::NS, NH = SS::NH, NN = N * N, n = NN * N, HY = N / 2;
   const VssGainArgs<T>& a = sa.g;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -907,13 +927,13 @@ __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabAr
         vss_slab_x_dispatch<T, N>(s, Fh, Xh, eph[it & 1][h], th);
         __syncthreads();
         {  // z-pass: row (j, ky = c) of this half's slab
-          cpx<T>* row = Xh + j * NN + c * N;
+          cpx<T>* pl = Xh + j * SS::PL;
           cpx<T> v[N];
 #pragma unroll
-          for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+          for (int b = 0; b < N; ++b) v[b] = pl[SS::at(c, b)];
           FFT<T, N, +1, true>::run(v);
 #pragma unroll
-          for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+          for (int b = 0; b < N; ++b) pl[SS::at(c, b)] = v[b];
         }
         __syncthreads();
         // the next product's phase factors into the other buffer (the x-pass
@@ -923,21 +943,21 @@ __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabAr
           if (xn < x1) phases((it + 1) & 1, xn, iyn);
         }
         {  // y-pass: column (j, z = c) of this half's slab, back into its own column
-          cpx<T>* pl = Xh + j * NN;
+          cpx<T>* pl = Xh + j * SS::PL;
           cpx<T> v[N];
 #pragma unroll
-          for (int ky = 0; ky < N; ++ky) v[ky] = pl[ky * N + nyq_swizzle<N>(c, ky)];
+          for (int ky = 0; ky < N; ++ky) v[ky] = pl[SS::at(ky, c)];
           FFT<T, N, +1, true>::run(v);
 #pragma unroll
-          for (int y = 0; y < N; ++y) pl[y * N + nyq_swizzle<N>(c, y)] = v[y];
+          for (int y = 0; y < N; ++y) pl[SS::at(y, c)] = v[y];
         }
         __syncthreads();
         {  // S_x += t3 t4 on this thread's half of column (j, c)
-          const cpx<T>* p1 = X1 + j * NN;
-          const cpx<T>* p2 = X2 + j * NN;
+          const cpx<T>* p1 = X1 + j * SS::PL;
+          const cpx<T>* p2 = X2 + j * SS::PL;
 #pragma unroll
           for (int i = 0; i < HY; ++i) {
-            const int y = h * HY + i, o = y * N + nyq_swizzle<N>(c, y);
+            const int y = h * HY + i, o = SS::at(y, c);
             sacc[i] = sacc[i] + p1[o] * p2[o];
           }
         }
@@ -946,39 +966,39 @@ __global__ __launch_bounds__(VssSlabShape<N>::NT) void k_vss_gain_slab(VssSlabAr
       // threads; the halves first gather each column's S in LDS)
       __syncthreads();  // every product read of X1, X2 is done
       {
-        cpx<T>* pl = X1 + j * NN;
+        cpx<T>* pl = X1 + j * SS::PL;
 #pragma unroll
         for (int i = 0; i < HY; ++i) {
           const int y = h * HY + i;
-          pl[y * N + nyq_swizzle<N>(c, y)] = sacc[i];
+          pl[SS::at(y, c)] = sacc[i];
         }
       }
       __syncthreads();
       if (h == 0) {
-        cpx<T>* pl = X1 + j * NN;
+        cpx<T>* pl = X1 + j * SS::PL;
         cpx<T> v[N];
 #pragma unroll
-        for (int y = 0; y < N; ++y) v[y] = pl[y * N + nyq_swizzle<N>(c, y)];
+        for (int y = 0; y < N; ++y) v[y] = pl[SS::at(y, c)];
         FFT<T, N, -1, true>::run(v);
 #pragma unroll
-        for (int ky = 0; ky < N; ++ky) pl[ky * N + nyq_swizzle<N>(c, ky)] = v[ky];
+        for (int ky = 0; ky < N; ++ky) pl[SS::at(ky, c)] = v[ky];
       }
       __syncthreads();
       if (h == 0) {
-        cpx<T>* row = X1 + j * NN + c * N;
+        cpx<T>* pl = X1 + j * SS::PL;
         cpx<T> v[N];
 #pragma unroll
-        for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+        for (int b = 0; b < N; ++b) v[b] = pl[SS::at(c, b)];
         FFT<T, N, -1, true>::run(v);
 #pragma unroll
-        for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+        for (int b = 0; b < N; ++b) pl[SS::at(c, b)] = v[b];
       }
       __syncthreads();
       {
         cpx<T>* Tp = sa.Tb + ((static_cast<size_t>(jb_i) * a.npts + pt) * a.Nrho + x) * n;
-        for (int e = tid; e < SS::planes; e += SS::NT) {
+        for (int e = tid; e < SS::cells; e += SS::NT) {
           const int jj = e / NN, r = e - jj * NN, ky = r / N, kz = r - ky * N;
-          Tp[static_cast<size_t>(s + NS * jj) * NN + r] = X1[jj * NN + ky * N + nyq_swizzle<N>(kz, ky)];
+          Tp[static_cast<size_t>(s + NS * jj) * NN + r] = X1[jj * SS::PL + SS::at(ky, kz)];
         }
       }
     }
@@ -1342,10 +1362,10 @@ static int vss_rr_kind(const VssHandle* h) {
   if (h->algorithm == 1) return 1;
   if (h->algorithm == 7) return (slab_ok && h->iso) ? 7 : 1;
   if (rr_ok) {
-    // FP32 isotropic at 16^3: the slab kernel (10.6 against 12.0 ms per
-    // 2048 mixture points on MI355X); at 24^3 the one-pencil kernel (49.0
-    // against 60.4 ms).  Algorithm 6 is validated to FP32 at 16 / 24 when set.
-    if (N == 16 && h->iso && h->algorithm == 0) return 7;
+    // FP32 isotropic: the slab kernel (per 2048 mixture points on MI355X,
+    // 16^3: 10.6 against 12.0 ms for the one-pencil kernel; 24^3: 42.5
+    // against 48.9 ms).  Algorithm 6 is validated to FP32 at 16 / 24 when set.
+    if (h->iso && h->algorithm == 0) return 7;
     return 4;
   }
   // FP64 isotropic: the slab kernel
@@ -1362,7 +1382,7 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   const int njobs = static_cast<int>(jobs.size());
   const int kind = vss_rr_kind<T, N>(h);
   int slab_ns = 1;
-  if constexpr (N == 16 || N == 24) slab_ns = VssSlabShape<N>::NS;
+  if constexpr (N == 16 || N == 24) slab_ns = VssSlabShape<N, T>::NS;
   int groups = h->groups_override;
   if (groups <= 0) {
     if (kind == 7) {  // (point, job, group, slab) items, one workgroup per CU
@@ -1438,7 +1458,7 @@ static int vss_run_chunk(VssHandle* h, const double* const* fs, double* const* Q
   ga.Nrho = h->Nrho;
   if (kind == 7) {
     if constexpr (N == 16 || N == 24) {
-      using SS = VssSlabShape<N>;
+      using SS = VssSlabShape<N, T>;
       const size_t tbytes = static_cast<size_t>(njobs) * npts * h->Nrho * n * sizeof(cpx<T>);
       if (tbytes > h->cap_tb) {
         (void)hipFree(h->Tb);

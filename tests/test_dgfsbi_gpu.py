@@ -228,7 +228,7 @@ def test_bi_fp32_steps_track_fp64(device):
         st.advance_to(0.01)
         be.synchronize()
         if prec == "single":
-            assert sysm.sm.op.get_algorithm() == 5   # alpha = 1: the isotropic register-resident gain
+            assert sysm.sm.op.get_algorithm() == 7   # alpha = 1: the isotropic slab gain
         out[prec] = [sysm.species_register(p, 0).get() for p in range(2)]
     W = OF.QuadFR(ORDER).W
     for p in range(2):
