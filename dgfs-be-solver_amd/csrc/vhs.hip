@@ -919,7 +919,18 @@ struct SlabShape {
   static constexpr int NS = (N == 32) ? 4 : (N == 24) ? 3 : 1;
   static constexpr int XS = N / NS;                                              // planes per slab
   static constexpr int NT = XS * N;                                              // one row per thread
-  static constexpr int planes = XS * N * N;                                      // complex
+  // plane rows padded to N + 1 where the LDS holds it (row and column
+  // accesses conflict free with immediate offsets), else swizzled: the
+  // swizzled offsets, one per element, are held across the item loop by the
+  // compiler and cost registers
+  static constexpr int RP = (N == 24) ? N + 1 : N;
+  static constexpr int PL = N * RP;                                              // complex per plane
+  static constexpr int planes = XS * PL;                                         // complex (storage)
+  static constexpr int cells = XS * N * N;                                       // values per slab
+  static __device__ __forceinline__ int at(int r, int c) {
+    if constexpr (RP == N) return r * N + nyq_swizzle<N>(c, r);
+    else return r * RP + c;
+  }
   static constexpr int wl = N * N + 2 * XS * N;  // complex: W0 plane, the slab's W1 and W2 rows
   static constexpr int wpt = wl / NT;            // Nyquist-plane values per thread
   static_assert(NS * XS == N, "slabs must tile the x axis");
@@ -985,7 +996,7 @@ __device__ __forceinline__ void slab_x_pass(const cpx<T>* __restrict__ Ft, cpx<T
     });
     FFT<T, XS, +1, true>::run(U);
 #pragma unroll
-    for (int j = 0; j < XS; ++j) X[j * NN + ky * N + nyq_swizzle<N>(kz, ky)] = U[j];
+    for (int j = 0; j < XS; ++j) X[j * SS::PL + SS::at(ky, kz)] = U[j];
   }
 }
 
@@ -1070,13 +1081,13 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
         }
         __syncthreads();
         {  // z-pass: row (j, ky = c)
-          cpx<T>* row = X + j * NN + c * N;
+          cpx<T>* pl = X + j * SS::PL;
           cpx<T> v[N];
 #pragma unroll
-          for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+          for (int b = 0; b < N; ++b) v[b] = pl[SS::at(c, b)];
           FFT<T, N, +1, true>::run(v);
 #pragma unroll
-          for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+          for (int b = 0; b < N; ++b) pl[SS::at(c, b)] = v[b];
         }
         __syncthreads();
         // the next (p, q)'s phase factors into the other buffer (read after
@@ -1086,10 +1097,10 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
           if (pn < p1) (&eph[(it + 1) & 1][0][0])[tid] = a.ephtab[(static_cast<size_t>(pn) * a.M + qn) * 3 * N + tid];
         }
         {  // y-pass: column (j, z = c), then the products
-          const cpx<T>* pl = X + j * NN;
+          const cpx<T>* pl = X + j * SS::PL;
           cpx<T> v[N];
 #pragma unroll
-          for (int ky = 0; ky < N; ++ky) v[ky] = pl[ky * N + nyq_swizzle<N>(c, ky)];
+          for (int ky = 0; ky < N; ++ky) v[ky] = pl[SS::at(ky, c)];
           FFT<T, N, +1, true>::run(v);
           const cpx<T> w1 = W1[j * N + c];
 #pragma unroll
@@ -1108,26 +1119,26 @@ __global__ __launch_bounds__(SlabShape<N>::NT) void k_vhs_gain_slab(GainSlabArgs
 #pragma unroll
         for (int y = 0; y < N; ++y) v[y] = cpx<T>{sacc[y], T(0)};
         FFT<T, N, -1, true>::run(v);
-        cpx<T>* pl = X + j * NN;
+        cpx<T>* pl = X + j * SS::PL;
 #pragma unroll
-        for (int ky = 0; ky < N; ++ky) pl[ky * N + nyq_swizzle<N>(c, ky)] = v[ky];
+        for (int ky = 0; ky < N; ++ky) pl[SS::at(ky, c)] = v[ky];
       }
       __syncthreads();
       {
-        cpx<T>* row = X + j * NN + c * N;
+        cpx<T>* pl = X + j * SS::PL;
         cpx<T> v[N];
 #pragma unroll
-        for (int b = 0; b < N; ++b) v[b] = row[nyq_swizzle<N>(b, c)];
+        for (int b = 0; b < N; ++b) v[b] = pl[SS::at(c, b)];
         FFT<T, N, -1, true>::run(v);
 #pragma unroll
-        for (int b = 0; b < N; ++b) row[nyq_swizzle<N>(b, c)] = v[b];
+        for (int b = 0; b < N; ++b) pl[SS::at(c, b)] = v[b];
       }
       __syncthreads();
       {
         cpx<T>* Tp = a.Tb + (static_cast<size_t>(pt) * a.Nrho + p) * n;
-        for (int e = tid; e < SS::planes; e += SS::NT) {
+        for (int e = tid; e < SS::cells; e += SS::NT) {
           const int jj = e / NN, rem = e - jj * NN, ky = rem / N, kz = rem - ky * N;
-          Tp[static_cast<size_t>(s + NS * jj) * NN + rem] = X[jj * NN + ky * N + nyq_swizzle<N>(kz, ky)];
+          Tp[static_cast<size_t>(s + NS * jj) * NN + rem] = X[jj * SS::PL + SS::at(ky, kz)];
         }
       }
     }
