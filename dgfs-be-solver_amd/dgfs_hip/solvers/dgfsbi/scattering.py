@@ -89,7 +89,7 @@ class VSSOperator:
     def set_algorithm(self, algorithm):
         """Gain kernel: 0 automatic, 1 multi-pass, 4 register-resident (FP32,
         Nv 16 / 24), 6 packed isotropic register-resident (eta = 0 only), 7 slab
-        kernel (eta = 0, Nv 16 / 24; the FP64 default there)."""
+        kernel (eta = 0, Nv 16 / 24; the default there in either precision)."""
         _lib.call("dgfs_vss_set_algorithm", self._h, int(algorithm))
 
     def get_algorithm(self):

@@ -202,9 +202,9 @@ int dgfs_vss_set_groups(void* handle, int ngroups);
 /* Cap the persistent grid (and so the scratch cubes in use) at nslots
  * workgroups; 0 = the occupancy limit. */
 int dgfs_vss_set_slots(void* handle, int nslots);
-/* Gain kernel: 0 automatic (register-resident in FP32 at Nv 16 and 24; the
- * slab kernel in FP64 with isotropic scattering at Nv 16 and 24; multi-pass
- * otherwise), 1 multi-pass (any precision and grid), 4 register-resident with
+/* Gain kernel: 0 automatic (the slab kernel with isotropic scattering at Nv
+ * 16 and 24, either precision; register-resident in FP32 at Nv 16 and 24
+ * otherwise; multi-pass otherwise), 1 multi-pass (any precision and grid), 4 register-resident with
  * one pencil per thread (FP32 at Nv 16 and 24), 6 its isotropic form with t3
  * and t4 from one packed complex transform plus Nyquist-plane corrections (not
  * the automatic choice: slower than 4 as measured), 7 slab kernel (eta = 0 for
